@@ -1,0 +1,1218 @@
+// capi.cpp -- the C-ABI of include/emerge_hip.h: a C++ mirror of the
+// reference's Simulation lifecycle (solver/lifecycle.rs, solver/particles.rs)
+// that owns device memory and drives the substep kernels.
+//
+// There is no CPU compute path here: every substep, and the spawn-time volume
+// estimate, runs on the GPU; a call that would need the GPU fails with
+// EMH_ERR_DEVICE when no device is present.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/emerge_hip.h"
+#include "kernels.hpp"
+#include "spawn_kernels.hpp"
+
+using namespace emh;
+
+static_assert(sizeof(emh_particle) == 128, "Particle is 128 bytes (particle.rs:144)");
+static_assert(sizeof(emh_material_params) == 96, "MaterialParams is 96 bytes (params.rs:101)");
+static_assert(sizeof(emh_config) == 116, "emh_config layout");
+static_assert(sizeof(emh_material_ext) == 32, "emh_material_ext layout");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(EMH_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+int device_present() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void ext_defaults(emh_material_ext* e) {
+    std::memset(e, 0, sizeof(*e));
+    e->min_density = 1.0e-6f;
+    e->min_volume = 1.0e-6f;
+    e->dp_volume_correction = 1.0f;
+    e->nacc_hardening_enabled = 2u;
+}
+
+// f32::to_radians: self * (PI / 180.0) evaluated in f32.
+float to_radians(float deg) { return deg * (3.14159265358979323846f / 180.0f); }
+
+int validate_config(const emh_config& c) {  // SimConfig::validate  config.rs:247-299
+    if (c.grid_res < 4) return fail(EMH_ERR_INVALID_ARG, "grid_res must be >= 4");
+    if (!(c.grid_cell_size > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "grid_cell_size must be positive");
+    if (!(c.dt > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "dt must be positive");
+    if (!(c.cfl_coefficient > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "cfl_coefficient must be positive");
+    if (!(c.material_cfl_coefficient > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "material_cfl_coefficient must be positive");
+    if (!(c.viscous_timestep_coefficient > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "viscous_timestep_coefficient must be positive");
+    if (!(c.min_dt > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "min_dt must be positive");
+    if (!(c.min_dt <= c.dt)) return fail(EMH_ERR_INVALID_ARG, "min_dt must be <= dt");
+    if (!(c.projection_min_density > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "projection_min_density must be positive");
+    if (!(c.projection_min_volume > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "projection_min_volume must be positive");
+    if (!(c.projection_min_deformation_j > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "projection_min_deformation_j must be positive");
+    if (!(c.particle_mass > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "particle_mass must be positive");
+    if (!(c.contact_friction >= 0.0f)) return fail(EMH_ERR_INVALID_ARG, "contact_friction must be non-negative");
+    if (c.max_substeps_per_step == 0) return fail(EMH_ERR_INVALID_ARG, "max_substeps_per_step must be > 0");
+    if (!(c.default_initial_volume > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "default_initial_volume must be positive");
+    if (!(c.j_max > 1.0f)) return fail(EMH_ERR_INVALID_ARG, "j_max must be > 1.0");
+    if (!(c.apic_blend >= 0.0f && c.apic_blend <= 1.0f))
+        return fail(EMH_ERR_INVALID_ARG, "apic_blend must be in [0, 1]");
+    if (!(c.boundary_thickness > 0 && c.boundary_thickness < c.grid_res - 1))
+        return fail(EMH_ERR_INVALID_ARG, "boundary_thickness must be in [1, grid_res-2]");
+    return EMH_OK;
+}
+
+int check_supported_config(const emh_config& c) {
+    if (c.sleep_threshold != 0.0f) return fail(EMH_ERR_UNSUPPORTED, "sleep_threshold > 0 (sleep partition) is not supported");
+    if (c.asflip_blend != 0.0f) return fail(EMH_ERR_UNSUPPORTED, "asflip_blend > 0 is not supported");
+    if (c.grid_res > 16384) return fail(EMH_ERR_UNSUPPORTED, "grid_res above 16384 is not supported");
+    return EMH_OK;
+}
+
+int check_supported_material(const emh_material_params& p) {
+    switch (p.model) {
+        case EMH_MODEL_FALLBACK:
+        case EMH_MODEL_FLUID:
+        case EMH_MODEL_NEOHOOKEAN:
+        case EMH_MODEL_COROTATED:
+        case EMH_MODEL_SNOW:
+        case EMH_MODEL_DRUCKER_PRAGER:
+        case EMH_MODEL_NACC:
+            break;
+        default:
+            return fail(EMH_ERR_UNSUPPORTED, "constitutive model %u is not implemented on the GPU path", p.model);
+    }
+    if (p.active_stress_coeff != 0.0f && (p.model == EMH_MODEL_NEOHOOKEAN || p.model == EMH_MODEL_COROTATED))
+        return fail(EMH_ERR_UNSUPPORTED, "active stress (active_stress_coeff != 0) is not supported");
+    if (p.thermal_expansion != 0.0f && (p.model == EMH_MODEL_NEOHOOKEAN || p.model == EMH_MODEL_COROTATED))
+        return fail(EMH_ERR_UNSUPPORTED, "thermal_expansion != 0 is not supported");
+    if (p.thermal_viscosity_coeff != 0.0f && p.model == EMH_MODEL_FLUID)
+        return fail(EMH_ERR_UNSUPPORTED, "thermal_viscosity_coeff != 0 is not supported");
+    return EMH_OK;
+}
+
+// Device record of one material: the model's own fields recovered from the
+// 96 B MaterialParams (each model's params(), see emerge_hip.h) plus the
+// extension, and host-evaluated constants.
+DevMat make_dev_mat(const emh_material_params& p, const emh_material_ext& e) {
+    DevMat d;
+    std::memset(&d, 0, sizeof(d));
+    d.model = p.model;
+    d.lambda = p.lambda;
+    d.mu = p.mu;
+    d.min_density = e.min_density;
+    d.min_volume = e.min_volume;
+    d.wave_lambda = p.lambda;
+    switch (p.model) {
+        case EMH_MODEL_NEOHOOKEAN:
+            d.damage_rate = p.cohesion_coeff;
+            d.nh_viscosity = p.dynamic_viscosity;
+            break;
+        case EMH_MODEL_FLUID:
+            d.rest_density = p.rest_density;
+            d.dyn_visc = p.dynamic_viscosity;
+            d.eos_k = p.eos_stiffness;
+            d.eos_p = p.eos_power;
+            d.p_floor = p.pressure_floor;
+            d.bulk_visc = p.bulk_viscosity;
+            d.surf_tension = p.surface_tension_coeff;
+            d.settling = p.dp_h0;
+            d.fl_c2_denom = std::fmax(p.rest_density, e.min_density);
+            break;
+        case EMH_MODEL_DRUCKER_PRAGER:
+            d.dp_phi0 = p.dp_h0;
+            d.dp_h1 = p.dp_h1;
+            d.dp_h2 = p.dp_h2;
+            d.dp_h3 = p.dp_h3;
+            d.dp_vol_corr = e.dp_volume_correction;
+            d.dp_dilat = p.compression_limit;
+            d.dp_sin_dilat = sinf(p.compression_limit);
+            d.dp_cohesion_term = p.stretch_limit / (2.0f * p.mu);
+            d.dp_ratio = (p.lambda + p.mu) / p.mu;
+            d.dp_min_vj = p.volume_ratio_min;
+            d.dp_qmax = 5.0f / std::fmax(p.dp_h2, 1e-6f);
+            d.dp_sqrt23 = sqrtf(2.0f / 3.0f);
+            break;
+        case EMH_MODEL_SNOW:
+            d.sn_xi = p.hardening_exponent;
+            d.sn_lo = 1.0f - p.compression_limit;
+            d.sn_hi = 1.0f + p.stretch_limit;
+            d.sn_jmin = p.volume_ratio_min;
+            d.sn_jmax = p.volume_ratio_max;
+            d.sn_coh = p.cohesion_coeff;
+            break;
+        case EMH_MODEL_NACC: {
+            float kappa = e.nacc_kappa != 0.0f ? e.nacc_kappa : p.lambda + p.mu;
+            d.na_kappa = kappa;
+            d.na_m = p.stretch_limit;
+            d.na_beta = p.compression_limit;
+            d.na_xi = p.hardening_exponent;
+            bool hard = e.nacc_hardening_enabled == 2u ? (p.hardening_exponent > 0.0f) : (e.nacc_hardening_enabled != 0u);
+            d.flags = (hard ? 1u : 0u) | (e.nacc_advance_f ? 2u : 0u);
+            d.wave_lambda = kappa - p.mu;
+            break;
+        }
+        default:
+            break;
+    }
+    return d;
+}
+
+// init_particle: corotated.rs:57-60, sand.rs:237-246, snow.rs:104-107, nacc.rs:271-273
+void init_particle(const emh_material_params& p, emh_particle& q) {
+    switch (p.model) {
+        case EMH_MODEL_COROTATED:
+            q.hardening_scale = 1.0f;
+            q.plastic_volume_ratio = 1.0f;
+            break;
+        case EMH_MODEL_DRUCKER_PRAGER:
+            q.friction_hardening = p.dp_h1 > 0.0f ? p.dp_h3 / p.dp_h1 : 0.0f;
+            break;
+        case EMH_MODEL_SNOW:
+            q.plastic_volume_ratio = 1.0f;
+            q.hardening_scale = 1.0f;
+            break;
+        case EMH_MODEL_NACC:
+            q.log_volume_strain = 0.0f;
+            break;
+        default:
+            break;
+    }
+}
+
+struct LcgRng {  // solver/mod.rs:154-175
+    uint32_t state;
+    uint32_t next_u32() {
+        state = state * 1664525u + 1013904223u;
+        return state;
+    }
+    float next_f32() { return (float)next_u32() / ((float)4294967295u + 1.0f); }
+};
+
+// initialize_particles  solver/mod.rs:88-152
+std::vector<emh_particle> lattice(const emh_config& cfg, const emh_spawn_region& sp) {
+    std::vector<emh_particle> out;
+    LcgRng rng{sp.rng_seed};
+    const float mass = sp.has_mass_override ? sp.mass_override : cfg.particle_mass;
+    const float hx = (float)sp.box_size[0] * 0.5f, hy = (float)sp.box_size[1] * 0.5f;
+    const float x0 = sp.box_center[0] - hx, y0 = sp.box_center[1] - hy;
+    const float x1 = sp.box_center[0] + hx, y1 = sp.box_center[1] + hy;
+    for (float i = x0; i < x1; i += sp.spacing) {
+        for (float j = y0; j < y1; j += sp.spacing) {
+            if (sp.shape == EMH_SHAPE_DISK) {
+                float dx = i - sp.box_center[0], dy = j - sp.box_center[1];
+                if (!(sqrtf(dx * dx + dy * dy) <= sp.disk_radius)) continue;
+            }
+            const float jm = sp.position_jitter * sp.spacing;
+            const float jx = (rng.next_f32() - 0.5f) * 2.0f * jm;
+            const float jy = (rng.next_f32() - 0.5f) * 2.0f * jm;
+            const float rx = rng.next_f32(), ry = rng.next_f32();
+            emh_particle p;
+            std::memset(&p, 0, sizeof(p));
+            p.x[0] = i + jx;
+            p.x[1] = j + jy;
+            p.v[0] = (rx - 0.5f) * sp.initial_velocity_scale;
+            p.v[1] = (ry - 0.5f) * sp.initial_velocity_scale;
+            std::memcpy(p.deformation_gradient, sp.initial_deformation_gradient, sizeof(float) * 4);
+            p.mass = mass;
+            p.initial_volume = cfg.default_initial_volume;
+            p.volume = cfg.default_initial_volume;
+            p.density = mass / cfg.default_initial_volume;
+            p.material_id = sp.material_id;
+            p.plastic_volume_ratio = 1.0f;
+            p.hardening_scale = 1.0f;
+            out.push_back(p);
+        }
+    }
+    return out;
+}
+
+bool spawn_fits(const emh_spawn_region& sp, const emh_config& cfg) {  // config.rs:472-482
+    if (!(sp.spacing > 0.0f) || sp.box_size[0] <= 0 || sp.box_size[1] <= 0) return false;
+    const float hx = (float)sp.box_size[0] * 0.5f, hy = (float)sp.box_size[1] * 0.5f;
+    const float dmin = (float)cfg.boundary_thickness;
+    const float dmax = (float)(cfg.grid_res > cfg.boundary_thickness ? cfg.grid_res - cfg.boundary_thickness : 0u);
+    return sp.box_center[0] - hx >= dmin && sp.box_center[1] - hy >= dmin && sp.box_center[0] + hx <= dmax &&
+           sp.box_center[1] + hy <= dmax;
+}
+
+template <class T> T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    return (T*)p;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ emh_sim
+struct emh_sim {
+    emh_config cfg{};
+    std::vector<emh_material_params> mats;
+    std::vector<emh_material_ext> exts;
+    std::vector<uint32_t> slips;
+    std::vector<emh_particle> host;  // authoritative when !device_ahead
+    uint32_t next_tag = 1;
+    bool host_dirty = true;     // host changed since the last upload
+    bool device_ahead = false;  // device stepped since the last download
+    emh_step_stats stats{};
+    uint32_t predicted_substeps = 1;
+
+    // per-kernel event timing
+    bool profiling = false;
+    std::vector<SlotEvents> slot_events;
+    double prof_ms[kKernelCount] = {};
+    uint64_t prof_launches[kKernelCount] = {};
+
+    // device
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pipeline pipe{};
+    uint32_t cap = 0;  // particle capacity of the device buffers
+    DevMat* d_mats = nullptr;
+    void* grid_block = nullptr;
+    uint32_t* d_cell_count = nullptr;
+    uint32_t* d_cell_start = nullptr;
+    uint32_t* d_tile_count = nullptr;
+    uint32_t* d_tile_start = nullptr;
+    Ctrl* d_ctrl = nullptr;
+    std::vector<void*> particle_allocs;
+
+    ~emh_sim() { release(); }
+
+    void release_particles() {
+        for (void* p : particle_allocs) (void)hipFree(p);
+        particle_allocs.clear();
+        cap = 0;
+    }
+    void release_events() {
+        for (auto& se : slot_events)
+            for (int k = 0; k < kKernelCount; ++k)
+                for (int w = 0; w < 2; ++w) (void)hipEventDestroy(se.ev[k][w]);
+        slot_events.clear();
+    }
+    int ensure_slot_events(uint32_t n) {
+        while (slot_events.size() < n) {
+            SlotEvents se;
+            std::memset(&se, 0, sizeof(se));
+            for (int k = 0; k < kKernelCount; ++k)
+                for (int w = 0; w < 2; ++w) HIP_TRY(hipEventCreate(&se.ev[k][w]));
+            slot_events.push_back(se);
+        }
+        return EMH_OK;
+    }
+    SlotEvents* slot_ev(uint32_t k) { return profiling ? &slot_events[k] : nullptr; }
+    int accumulate_profile(uint32_t active_slots) {
+        for (uint32_t k = 0; k < active_slots && k < slot_events.size(); ++k)
+            for (int id = 0; id < kKernelCount; ++id) {
+                if (!slot_events[k].used[id]) continue;
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, slot_events[k].ev[id][0], slot_events[k].ev[id][1]));
+                prof_ms[id] += ms;
+                prof_launches[id] += 1;
+            }
+        return EMH_OK;
+    }
+
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        release_events();
+        release_particles();
+        if (d_mats) (void)hipFree(d_mats);
+        if (grid_block) (void)hipFree(grid_block);
+        if (d_cell_count) (void)hipFree(d_cell_count);
+        if (d_cell_start) (void)hipFree(d_cell_start);
+        if (d_tile_count) (void)hipFree(d_tile_count);
+        if (d_tile_start) (void)hipFree(d_tile_start);
+        if (d_ctrl) (void)hipFree(d_ctrl);
+        if (stream) (void)hipStreamDestroy(stream);
+        d_mats = nullptr;
+        grid_block = nullptr;
+        d_cell_count = d_cell_start = d_tile_count = d_tile_start = nullptr;
+        d_ctrl = nullptr;
+        stream = nullptr;
+    }
+
+    uint32_t res() const { return cfg.grid_res; }
+    uint32_t tpd() const { return (cfg.grid_res + 15u) / 16u; }
+    uint32_t n_tiles() const { return tpd() * tpd(); }
+    uint32_t n_bins() const { return n_tiles() * (uint32_t)kTileCells; }
+
+    int init_device() {
+        if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
+        HIP_TRY(hipGetDevice(&device));
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        const size_t R2 = (size_t)res() * res();
+        // grid: m[2], mx[2], my[2], dmass (7 R^2 floats) + vel (2 R^2 floats)
+        HIP_TRY(hipMalloc(&grid_block, sizeof(float) * R2 * 9));
+        float* g = (float*)grid_block;
+        pipe.grid.m[0] = g + 0 * R2;
+        pipe.grid.m[1] = g + 1 * R2;
+        pipe.grid.mx[0] = g + 2 * R2;
+        pipe.grid.mx[1] = g + 3 * R2;
+        pipe.grid.my[0] = g + 4 * R2;
+        pipe.grid.my[1] = g + 5 * R2;
+        pipe.grid.dmass = g + 6 * R2;
+        pipe.grid.vel = (float2*)(g + 7 * R2);
+        d_cell_count = dalloc<uint32_t>(n_bins());
+        d_cell_start = dalloc<uint32_t>(n_bins() + 1);
+        d_tile_count = dalloc<uint32_t>(n_tiles());
+        d_tile_start = dalloc<uint32_t>(n_tiles() + 1);
+        d_ctrl = dalloc<Ctrl>(1);
+        d_mats = dalloc<DevMat>(EMH_MAX_MATERIALS);
+        if (!d_cell_count || !d_cell_start || !d_tile_count || !d_tile_start || !d_ctrl || !d_mats)
+            return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+        pipe.sort.cell_count = d_cell_count;
+        pipe.sort.cell_start = d_cell_start;
+        pipe.sort.tile_count = d_tile_count;
+        pipe.sort.tile_start = d_tile_start;
+        pipe.ctrl = d_ctrl;
+        pipe.mats = d_mats;
+        return EMH_OK;
+    }
+
+    int ensure_capacity(uint32_t n) {
+        if (n <= cap && cap > 0) return EMH_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        release_particles();
+        uint32_t c = std::max<uint32_t>(n, 1024u);
+        auto take = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            particle_allocs.push_back(p);
+            return p;
+        };
+        for (int b = 0; b < 2; ++b) {
+            PBuf& pb = pipe.buf[b];
+            pb.x = (float2*)take(sizeof(float2) * c);
+            pb.v = (float2*)take(sizeof(float2) * c);
+            pb.c = (float4*)take(sizeof(float4) * c);
+            pb.f = (float4*)take(sizeof(float4) * c);
+            pb.mass = (float*)take(sizeof(float) * c);
+            pb.v0 = (float*)take(sizeof(float) * c);
+            pb.vol = (float*)take(sizeof(float) * c);
+            pb.dens = (float*)take(sizeof(float) * c);
+            pb.jp = (float*)take(sizeof(float) * c);
+            pb.h = (float*)take(sizeof(float) * c);
+            pb.q = (float*)take(sizeof(float) * c);
+            pb.lvs = (float*)take(sizeof(float) * c);
+            pb.mat = (uint32_t*)take(sizeof(uint32_t) * c);
+            pb.id = (uint32_t*)take(sizeof(uint32_t) * c);
+            pb.key = (uint32_t*)take(sizeof(uint32_t) * c);
+            pb.rank = (uint32_t*)take(sizeof(uint32_t) * c);
+        }
+        pipe.sort.perm = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.cfl_partial = (float2*)take(sizeof(float2) * ((c + kBlock - 1) / kBlock));
+        for (void* p : particle_allocs)
+            if (!p) {
+                release_particles();
+                return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation of %u particles failed", c);
+            }
+        cap = c;
+        return EMH_OK;
+    }
+
+    void fill_dev_config() {
+        DevConfig& d = pipe.cfg;
+        std::memset(&d, 0, sizeof(d));
+        d.grid_res = cfg.grid_res;
+        d.cell = cfg.grid_cell_size;
+        d.dt = cfg.dt;
+        d.adaptive = cfg.adaptive_timestep;
+        d.cfl_affine = cfg.cfl_include_affine_speed;
+        d.cfl_coefficient = cfg.cfl_coefficient;
+        d.material_cfl = cfg.material_cfl_coefficient;
+        d.viscous_cfl = cfg.viscous_timestep_coefficient;
+        d.min_dt = cfg.min_dt;
+        d.project = cfg.project_invalid_state;
+        d.proj_min_density = cfg.projection_min_density;
+        d.proj_min_volume = cfg.projection_min_volume;
+        d.proj_min_j = cfg.projection_min_deformation_j;
+        d.gx = cfg.gravity[0];
+        d.gy = cfg.gravity[1];
+        d.bt = cfg.boundary_thickness;
+        d.default_initial_volume = cfg.default_initial_volume;
+        d.particle_mass = cfg.particle_mass;
+        d.max_substeps = cfg.max_substeps_per_step;
+        d.apic_blend = cfg.apic_blend;
+        d.j_max = cfg.j_max;
+        d.n_slip = (uint32_t)slips.size();
+        for (size_t i = 0; i < slips.size(); ++i) {
+            d.slip[i] = slips[i];
+            d.slip_lo[i] = (float)(slips[i] > 0 ? slips[i] - 1 : 0u);
+            d.slip_hi[i] = (float)(cfg.grid_res > slips[i] ? cfg.grid_res - slips[i] : 0u);
+        }
+        const uint32_t bt = cfg.boundary_thickness;
+        d.proj_lo = (float)(bt > 0 ? bt - 1 : 0u);
+        d.proj_hi = (float)(cfg.grid_res > bt ? cfg.grid_res - bt : 0u);
+        d.tiles_per_dim = tpd();
+        d.n_bins = n_bins();
+        bool dens = cfg.recompute_density_each_step != 0;
+        for (auto& m : mats)
+            if (m.model == EMH_MODEL_FLUID) dens = true;  // needs_density_recompute  fluid.rs:261-263
+        d.density_pass = dens ? 1u : 0u;
+        pipe.n_mats = (uint32_t)mats.size();
+    }
+
+    int upload_materials() {
+        std::vector<DevMat> dm(mats.size());
+        for (size_t i = 0; i < mats.size(); ++i) dm[i] = make_dev_mat(mats[i], exts[i]);
+        HIP_TRY(hipMemcpyAsync(d_mats, dm.data(), sizeof(DevMat) * dm.size(), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return EMH_OK;
+    }
+
+    // Host AoS -> device SoA (buffer 0, caller order), then keys/ranks/CFL.
+    int upload() {
+        for (auto& p : host) {
+            if (p.contact_group != 0)
+                return fail(EMH_ERR_UNSUPPORTED, "particles with contact_group != 0 (multi-field contact) are not supported");
+        }
+        const uint32_t n = (uint32_t)host.size();
+        int rc = ensure_capacity(n);
+        if (rc) return rc;
+        fill_dev_config();
+        rc = upload_materials();
+        if (rc) return rc;
+        pipe.n = n;
+        std::vector<float2> x(n), v(n);
+        std::vector<float4> c(n), f(n);
+        std::vector<float> mass(n), v0(n), vol(n), dens(n), jp(n), h(n), q(n), lvs(n);
+        std::vector<uint32_t> mat(n), id(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const emh_particle& p = host[i];
+            x[i] = make_float2(p.x[0], p.x[1]);
+            v[i] = make_float2(p.v[0], p.v[1]);
+            c[i] = make_float4(p.velocity_gradient[0], p.velocity_gradient[1], p.velocity_gradient[2],
+                               p.velocity_gradient[3]);
+            f[i] = make_float4(p.deformation_gradient[0], p.deformation_gradient[1], p.deformation_gradient[2],
+                               p.deformation_gradient[3]);
+            mass[i] = p.mass;
+            v0[i] = p.initial_volume;
+            vol[i] = p.volume;
+            dens[i] = p.density;
+            jp[i] = p.plastic_volume_ratio;
+            h[i] = p.hardening_scale;
+            q[i] = p.friction_hardening;
+            lvs[i] = p.log_volume_strain;
+            mat[i] = (p.material_id & kMatMask) | (p.pinned ? kPinnedBit : 0u);
+            if (p.material_id > kMatMask) mat[i] = kMatMask;  // unregistered: falls back to slot 0
+            id[i] = i;
+        }
+        const PBuf& b = pipe.buf[0];
+        auto put = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+            return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream) : hipSuccess;
+        };
+        HIP_TRY(put(b.x, x.data(), sizeof(float2) * n));
+        HIP_TRY(put(b.v, v.data(), sizeof(float2) * n));
+        HIP_TRY(put(b.c, c.data(), sizeof(float4) * n));
+        HIP_TRY(put(b.f, f.data(), sizeof(float4) * n));
+        HIP_TRY(put(b.mass, mass.data(), sizeof(float) * n));
+        HIP_TRY(put(b.v0, v0.data(), sizeof(float) * n));
+        HIP_TRY(put(b.vol, vol.data(), sizeof(float) * n));
+        HIP_TRY(put(b.dens, dens.data(), sizeof(float) * n));
+        HIP_TRY(put(b.jp, jp.data(), sizeof(float) * n));
+        HIP_TRY(put(b.h, h.data(), sizeof(float) * n));
+        HIP_TRY(put(b.q, q.data(), sizeof(float) * n));
+        HIP_TRY(put(b.lvs, lvs.data(), sizeof(float) * n));
+        HIP_TRY(put(b.mat, mat.data(), sizeof(uint32_t) * n));
+        HIP_TRY(put(b.id, id.data(), sizeof(uint32_t) * n));
+        Ctrl ctrl;
+        std::memset(&ctrl, 0, sizeof(ctrl));
+        ctrl.total_substeps = stats.total_substeps;
+        ctrl.last_dt = stats.last_step_dt;
+        ctrl.cur_buf = 0;
+        ctrl.cur_grid = 0;
+        ctrl.n_cfl_blocks = (n + kBlock - 1) / kBlock;
+        HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+        HIP_TRY(launch_zero_counters(pipe, stream));
+        HIP_TRY(launch_prologue(pipe, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        host_dirty = false;
+        device_ahead = false;
+        return EMH_OK;
+    }
+
+    // Device SoA (current buffer, sorted order) -> host AoS in caller order.
+    int download() {
+        if (!device_ahead) return EMH_OK;
+        const uint32_t n = pipe.n;
+        Ctrl ctrl;
+        HIP_TRY(hipMemcpyAsync(&ctrl, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        const PBuf& b = pipe.buf[ctrl.cur_buf & 1u];
+        std::vector<float2> x(n), v(n);
+        std::vector<float4> c(n), f(n);
+        std::vector<float> mass(n), v0(n), vol(n), dens(n), jp(n), h(n), q(n), lvs(n);
+        std::vector<uint32_t> id(n);
+        auto get = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+            return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream) : hipSuccess;
+        };
+        HIP_TRY(get(x.data(), b.x, sizeof(float2) * n));
+        HIP_TRY(get(v.data(), b.v, sizeof(float2) * n));
+        HIP_TRY(get(c.data(), b.c, sizeof(float4) * n));
+        HIP_TRY(get(f.data(), b.f, sizeof(float4) * n));
+        HIP_TRY(get(mass.data(), b.mass, sizeof(float) * n));
+        HIP_TRY(get(v0.data(), b.v0, sizeof(float) * n));
+        HIP_TRY(get(vol.data(), b.vol, sizeof(float) * n));
+        HIP_TRY(get(dens.data(), b.dens, sizeof(float) * n));
+        HIP_TRY(get(jp.data(), b.jp, sizeof(float) * n));
+        HIP_TRY(get(h.data(), b.h, sizeof(float) * n));
+        HIP_TRY(get(q.data(), b.q, sizeof(float) * n));
+        HIP_TRY(get(lvs.data(), b.lvs, sizeof(float) * n));
+        HIP_TRY(get(id.data(), b.id, sizeof(uint32_t) * n));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = id[k];
+            if (i >= n) return fail(EMH_ERR_DEVICE, "corrupt particle id %u at slot %u", i, k);
+            emh_particle& p = host[i];
+            p.x[0] = x[k].x;
+            p.x[1] = x[k].y;
+            p.v[0] = v[k].x;
+            p.v[1] = v[k].y;
+            p.velocity_gradient[0] = c[k].x;
+            p.velocity_gradient[1] = c[k].y;
+            p.velocity_gradient[2] = c[k].z;
+            p.velocity_gradient[3] = c[k].w;
+            p.deformation_gradient[0] = f[k].x;
+            p.deformation_gradient[1] = f[k].y;
+            p.deformation_gradient[2] = f[k].z;
+            p.deformation_gradient[3] = f[k].w;
+            p.mass = mass[k];
+            p.initial_volume = v0[k];
+            p.volume = vol[k];
+            p.density = dens[k];
+            p.plastic_volume_ratio = jp[k];
+            p.hardening_scale = h[k];
+            p.friction_hardening = q[k];
+            p.log_volume_strain = lvs[k];
+        }
+        device_ahead = false;
+        return EMH_OK;
+    }
+
+    int host_current() { return download(); }
+
+    // Simulation::step x n  (step.rs:23-60), one device sync per step.
+    int step(uint32_t n_steps) {
+        if (host_dirty) {
+            int rc = upload();
+            if (rc) return rc;
+        }
+        if (pipe.n == 0) {  // no particles: the loop still advances time
+            for (uint32_t s = 0; s < n_steps; ++s) {
+                float remaining = cfg.dt;
+                uint32_t k = 0;
+                while (remaining > kF32Eps && k < cfg.max_substeps_per_step) {
+                    float dt = cfg.adaptive_timestep ? remaining : std::fmin(remaining, cfg.dt);
+                    dt = std::fmin(dt, remaining);
+                    remaining -= dt;
+                    stats.last_step_dt = dt;
+                    ++k;
+                }
+                stats.last_substeps = k;
+                stats.total_substeps += k;
+                stats.last_sim_time_dropped = std::fmax(remaining, 0.0f);
+                stats.last_vel_clamp_count = 0;
+                stats.last_j_projection_count = 0;
+                stats.frame_index++;
+            }
+            return EMH_OK;
+        }
+        if (profiling) {
+            int rc = ensure_slot_events(cfg.max_substeps_per_step + 2);
+            if (rc) return rc;
+        }
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            HIP_TRY(launch_begin_step(pipe, stream));
+            uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
+            slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, stream, slot_ev(k)));
+            Ctrl ctrl;
+            uint32_t launched = slots;
+            while (true) {
+                HIP_TRY(hipMemcpyAsync(&ctrl, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                bool more = ctrl.remaining > kF32Eps && ctrl.substeps < cfg.max_substeps_per_step;
+                if (!more) break;
+                if (launched > cfg.max_substeps_per_step + 1)
+                    return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
+                HIP_TRY(launch_substep(pipe, stream, slot_ev(launched)));
+                ++launched;
+            }
+            if (profiling) {
+                int rc = accumulate_profile(ctrl.substeps);
+                if (rc) return rc;
+            }
+            predicted_substeps = ctrl.substeps;
+            stats.last_substeps = ctrl.substeps;
+            stats.last_step_dt = ctrl.last_dt;
+            stats.last_sim_time_dropped = std::fmax(ctrl.remaining, 0.0f);
+            stats.last_vel_clamp_count = ctrl.clamp_count;
+            stats.last_j_projection_count = ctrl.proj_count;
+            stats.total_substeps = ctrl.total_substeps;
+            stats.frame_index++;
+        }
+        device_ahead = true;
+        return EMH_OK;
+    }
+
+    // estimate_particle_volumes(_local) on the device for particles [from, n).
+    int estimate_volumes(uint32_t from, bool use_box, const float box[4], bool write_initial) {
+        const uint32_t n = (uint32_t)host.size();
+        if (from >= n) return EMH_OK;
+        float2* dx = dalloc<float2>(n);
+        float* dm = dalloc<float>(n);
+        float* dd = dalloc<float>(n);
+        float* dv = dalloc<float>(n);
+        uint32_t* dw = dalloc<uint32_t>(n);
+        int rc = EMH_OK;
+        if (!dx || !dm || !dd || !dv || !dw) rc = fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+        std::vector<float2> x(n);
+        std::vector<float> m(n), d(n), v(n);
+        std::vector<uint32_t> w(n);
+        if (rc == EMH_OK) {
+            for (uint32_t i = 0; i < n; ++i) {
+                x[i] = make_float2(host[i].x[0], host[i].x[1]);
+                m[i] = host[i].mass;
+            }
+            hipError_t e = hipMemcpyAsync(dx, x.data(), sizeof(float2) * n, hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(dm, m.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess)
+                e = estimate_volumes_device(dx, dm, n, from, res(), use_box, box, pipe.grid.dmass, dd, dv, dw, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(d.data(), dd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(v.data(), dv, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(w.data(), dw, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "volume estimate: %s", hipGetErrorString(e));
+            // the dense mass grid doubles as the substep's density-pass buffer: clear it again
+            if (rc == EMH_OK) {
+                e = hipMemsetAsync(pipe.grid.dmass, 0, sizeof(float) * (size_t)res() * res(), stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "volume estimate clear: %s", hipGetErrorString(e));
+            }
+        }
+        if (rc == EMH_OK) {
+            for (uint32_t i = from; i < n; ++i) {
+                if (!w[i]) continue;
+                host[i].density = d[i];
+                host[i].volume = v[i];
+                if (write_initial) host[i].initial_volume = v[i];
+            }
+        }
+        (void)hipFree(dx);
+        (void)hipFree(dm);
+        (void)hipFree(dd);
+        (void)hipFree(dv);
+        (void)hipFree(dw);
+        return rc;
+    }
+
+    int add_body(const emh_spawn_region& sp, uint32_t* tag_out) {  // solver/particles.rs:355-424
+        if (!spawn_fits(sp, cfg)) return fail(EMH_ERR_INVALID_ARG, "spawn region must stay inside the simulation domain");
+        int rc = host_current();
+        if (rc) return rc;
+        const uint32_t tag = next_tag++;
+        const uint32_t old_len = (uint32_t)host.size();
+        std::vector<emh_particle> np = lattice(cfg, sp);
+        const bool registered = sp.material_id < mats.size();
+        for (auto& p : np) {
+            p.user_tag = tag;
+            if (registered) init_particle(mats[sp.material_id], p);
+            host.push_back(p);
+        }
+        host_dirty = true;
+        // estimate_particle_volumes_local  density.rs:256-319
+        if (old_len < host.size()) {
+            float lo[2] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max()};
+            float hi[2] = {std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest()};
+            for (size_t i = old_len; i < host.size(); ++i)
+                for (int k = 0; k < 2; ++k) {
+                    lo[k] = std::fmin(lo[k], host[i].x[k]);
+                    hi[k] = std::fmax(hi[k], host[i].x[k]);
+                }
+            const float box[4] = {lo[0] - 3.0f, lo[1] - 3.0f, hi[0] + 3.0f, hi[1] + 3.0f};
+            rc = estimate_volumes(old_len, true, box, true);
+            if (rc) return rc;
+        }
+        if (tag_out) *tag_out = tag;
+        return EMH_OK;
+    }
+};
+
+// -------------------------------------------------------------------- C-ABI
+extern "C" {
+
+uint32_t emh_abi_version(void) { return EMH_ABI_VERSION; }
+const char* emh_last_error(void) { return g_err.c_str(); }
+
+int emh_device_count(int* count) {
+    if (!count) return fail(EMH_ERR_INVALID_ARG, "null count");
+    *count = device_present();
+    return EMH_OK;
+}
+
+int emh_config_default(emh_config* o) {  // config.rs:123-159
+    if (!o) return fail(EMH_ERR_INVALID_ARG, "null config");
+    std::memset(o, 0, sizeof(*o));
+    o->grid_res = 64;
+    o->grid_cell_size = 1.0f;
+    o->dt = 1.0f;
+    o->adaptive_timestep = 1;
+    o->cfl_include_affine_speed = 1;
+    o->cfl_coefficient = 0.9f;
+    o->material_cfl_coefficient = 0.5f;
+    o->viscous_timestep_coefficient = 0.5f;
+    o->min_dt = 1.0e-3f;
+    o->project_invalid_state = 1;
+    o->projection_min_density = 1.0e-6f;
+    o->projection_min_volume = 1.0e-6f;
+    o->projection_min_deformation_j = 1.0e-6f;
+    o->gravity[0] = 0.0f;
+    o->gravity[1] = -0.05f;
+    o->boundary_thickness = 2;
+    o->default_initial_volume = 1.0f;
+    o->recompute_density_each_step = 0;
+    o->particle_mass = 1.0f;
+    o->max_substeps_per_step = 64;
+    o->apic_blend = 1.0f;
+    o->j_max = 50.0f;
+    o->sleep_threshold = 0.0f;
+    o->contact_friction = 0.5f;
+    o->asflip_blend = 0.0f;
+    o->mixture_drag_coefficient = 0.0f;
+    o->mixture_pressure_iterations = 0;
+    o->dx_meters = 1.0f;
+    o->dt_seconds = 1.0f;
+    return EMH_OK;
+}
+
+int emh_config_standard(uint32_t grid_res, float dt, float gx, float gy, emh_config* o) {  // config.rs:165-172
+    int rc = emh_config_default(o);
+    if (rc) return rc;
+    o->grid_res = grid_res;
+    o->dt = dt;
+    o->gravity[0] = gx;
+    o->gravity[1] = gy;
+    return EMH_OK;
+}
+
+int emh_config_unsafe_defaults(emh_config* o) {  // config.rs:179-185
+    int rc = emh_config_default(o);
+    if (rc) return rc;
+    o->adaptive_timestep = 0;
+    o->project_invalid_state = 0;
+    return EMH_OK;
+}
+
+int emh_config_validate(const emh_config* c) {
+    if (!c) return fail(EMH_ERR_INVALID_ARG, "null config");
+    return validate_config(*c);
+}
+
+int emh_lame_from_young(float e, float nu, float* lambda, float* mu) {  // utils.rs:183-193
+    if (!lambda || !mu) return fail(EMH_ERR_INVALID_ARG, "null output");
+    *lambda = e * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    *mu = e / (2.0f * (1.0f + nu));
+    return EMH_OK;
+}
+
+float emh_to_radians(float degrees) { return to_radians(degrees); }
+
+int emh_material_neohookean(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_NEOHOOKEAN;
+    p->lambda = lambda;
+    p->mu = mu;
+    return EMH_OK;
+}
+
+int emh_material_corotated(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_COROTATED;
+    p->lambda = lambda;
+    p->mu = mu;
+    return EMH_OK;
+}
+
+int emh_material_fluid(float rest_density, float visc, float eos_k, float eos_p, emh_material_params* p,
+                       emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_FLUID;
+    p->rest_density = rest_density;
+    p->eos_stiffness = eos_k;
+    p->eos_power = eos_p;
+    p->dynamic_viscosity = visc;
+    p->volume_ratio_max = 2.0f;
+    p->pressure_floor = -0.1f;
+    return EMH_OK;
+}
+
+int emh_material_fluid_low_viscosity(float rest_density, float eos_k, emh_material_params* p, emh_material_ext* e) {
+    return emh_material_fluid(rest_density, 1.0e-3f, eos_k, 7.0f, p, e);
+}
+
+int emh_material_drucker_prager(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_DRUCKER_PRAGER;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->dp_h0 = to_radians(35.0f);
+    p->dp_h1 = to_radians(9.0f);
+    p->dp_h2 = 0.2f;
+    p->dp_h3 = to_radians(10.0f);
+    p->volume_ratio_min = 0.6f;
+    return EMH_OK;
+}
+
+int emh_material_snow(float lambda, float mu, float xi, float tc, float ts, float jmin, float jmax,
+                      emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_SNOW;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->hardening_exponent = xi;
+    p->compression_limit = tc;
+    p->stretch_limit = ts;
+    p->volume_ratio_min = jmin;
+    p->volume_ratio_max = jmax;
+    return EMH_OK;
+}
+
+int emh_material_snow_from_young(float young, float nu, emh_material_params* p, emh_material_ext* e) {
+    float l, m;
+    emh_lame_from_young(young, nu, &l, &m);
+    return emh_material_snow(l, m, 10.0f, 0.025f, 0.0075f, 0.6f, 20.0f, p, e);
+}
+
+int emh_material_nacc(float mu, float kappa, float friction, float cohesion, float xi, emh_material_params* p,
+                      emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_NACC;
+    p->lambda = kappa - mu;
+    p->mu = mu;
+    p->hardening_exponent = xi;
+    p->compression_limit = cohesion;
+    p->stretch_limit = friction;
+    e->nacc_kappa = kappa;
+    e->nacc_hardening_enabled = xi > 0.0f ? 1u : 0u;
+    return EMH_OK;
+}
+
+int emh_material_nacc_from_young(float young, float nu, float friction, float cohesion, float xi,
+                                 emh_material_params* p, emh_material_ext* e) {
+    float l, m;
+    emh_lame_from_young(young, nu, &l, &m);
+    return emh_material_nacc(m, l + m, friction, cohesion, xi, p, e);
+}
+
+int emh_sim_create(const emh_config* cfg, emh_sim** out) {  // Simulation::empty  lifecycle.rs:26-55
+    if (!cfg || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int rc = validate_config(*cfg);
+    if (rc) return rc;
+    rc = check_supported_config(*cfg);
+    if (rc) return rc;
+    emh_sim* s = new emh_sim();
+    s->cfg = *cfg;
+    emh_material_params fb;
+    std::memset(&fb, 0, sizeof(fb));
+    emh_material_ext fe;
+    ext_defaults(&fe);
+    s->mats.push_back(fb);  // FallbackMaterial
+    s->exts.push_back(fe);
+    s->slips.push_back(cfg->boundary_thickness);
+    s->stats.last_step_dt = cfg->dt;
+    rc = s->init_device();
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return EMH_OK;
+}
+
+int emh_sim_create_from_spawn(const emh_config* cfg, const emh_spawn_region* sp, emh_sim** out) {
+    if (!sp) return fail(EMH_ERR_INVALID_ARG, "null spawn");
+    int rc = emh_sim_create(cfg, out);
+    if (rc) return rc;
+    emh_sim* s = *out;
+    if (!spawn_fits(*sp, s->cfg)) {
+        emh_sim_destroy(s);
+        *out = nullptr;
+        return fail(EMH_ERR_INVALID_ARG, "spawn region must stay inside the simulation domain");
+    }
+    s->host = lattice(s->cfg, *sp);  // Simulation::new  lifecycle.rs:57-105
+    s->host_dirty = true;
+    if (sp->precompute_initial_volumes) {
+        const float box[4] = {0, 0, 0, 0};
+        rc = s->estimate_volumes(0, false, box, true);
+        if (rc) {
+            emh_sim_destroy(s);
+            *out = nullptr;
+            return rc;
+        }
+    }
+    return EMH_OK;
+}
+
+void emh_sim_destroy(emh_sim* sim) { delete sim; }
+
+int emh_sim_set_material(emh_sim* s, uint32_t id, const emh_material_params* p, const emh_material_ext* e) {
+    if (!s || !p) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (id >= EMH_MAX_MATERIALS) return fail(EMH_ERR_INVALID_ARG, "material_id %u exceeds the limit of %u", id, EMH_MAX_MATERIALS);
+    if (id > s->mats.size())
+        return fail(EMH_ERR_INVALID_ARG, "material IDs must be registered contiguously starting at 0; expected id %zu, got %u",
+                    s->mats.size(), id);
+    int rc = check_supported_material(*p);
+    if (rc) return rc;
+    rc = s->host_current();
+    if (rc) return rc;
+    emh_material_ext ext;
+    if (e) ext = *e;
+    else ext_defaults(&ext);
+    if (id < s->mats.size()) {
+        s->mats[id] = *p;
+        s->exts[id] = ext;
+    } else {
+        s->mats.push_back(*p);
+        s->exts.push_back(ext);
+    }
+    if (id == 0) {  // with_default_material re-runs init_particle on every particle
+        for (auto& q : s->host) {
+            const uint32_t m = q.material_id < s->mats.size() ? q.material_id : 0u;
+            init_particle(s->mats[m], q);
+        }
+    }
+    s->host_dirty = true;
+    return EMH_OK;
+}
+
+int emh_sim_reinit_particle_state(emh_sim* s) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    int rc = s->host_current();
+    if (rc) return rc;
+    for (auto& q : s->host) {
+        const uint32_t m = q.material_id < s->mats.size() ? q.material_id : 0u;
+        init_particle(s->mats[m], q);
+    }
+    s->host_dirty = true;
+    return EMH_OK;
+}
+
+int emh_sim_add_slip_boundary(emh_sim* s, uint32_t t) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    if (s->slips.size() >= EMH_MAX_BOUNDARIES) return fail(EMH_ERR_UNSUPPORTED, "at most %u boundaries", EMH_MAX_BOUNDARIES);
+    if (t + 1 > s->cfg.grid_res) return fail(EMH_ERR_INVALID_ARG, "slip thickness %u too large for grid_res %u", t, s->cfg.grid_res);
+    int rc = s->host_current();
+    if (rc) return rc;
+    s->slips.push_back(t);
+    s->host_dirty = true;
+    return EMH_OK;
+}
+
+int emh_sim_clear_boundaries(emh_sim* s) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    int rc = s->host_current();
+    if (rc) return rc;
+    s->slips.clear();
+    s->host_dirty = true;
+    return EMH_OK;
+}
+
+int emh_sim_add_body(emh_sim* s, const emh_spawn_region* sp, uint32_t* tag) {
+    if (!s || !sp) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    return s->add_body(*sp, tag);
+}
+
+int emh_sim_set_particles(emh_sim* s, const emh_particle* p, size_t n) {
+    if (!s || (!p && n)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (n > 0x7fffffffu) return fail(EMH_ERR_INVALID_ARG, "too many particles");
+    s->host.assign(p, p + n);
+    s->device_ahead = false;
+    s->host_dirty = true;
+    return EMH_OK;
+}
+
+int emh_sim_num_particles(const emh_sim* s, size_t* n) {
+    if (!s || !n) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *n = s->host.size();
+    return EMH_OK;
+}
+
+int emh_sim_get_particles(emh_sim* s, emh_particle* out, size_t n) {
+    if (!s || (!out && n)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (n != s->host.size()) return fail(EMH_ERR_INVALID_ARG, "expected %zu particles, got %zu", s->host.size(), n);
+    int rc = s->host_current();
+    if (rc) return rc;
+    if (n) std::memcpy(out, s->host.data(), sizeof(emh_particle) * n);
+    return EMH_OK;
+}
+
+int emh_sim_get_config(const emh_sim* s, emh_config* out) {
+    if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *out = s->cfg;
+    return EMH_OK;
+}
+
+int emh_sim_step(emh_sim* s, uint32_t n) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    return s->step(n);
+}
+
+int emh_sim_stats(const emh_sim* s, emh_step_stats* out) {
+    if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *out = s->stats;
+    return EMH_OK;
+}
+
+int emh_sim_get_grid(emh_sim* s, float* mass, float* momentum, size_t n_cells) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    const size_t R2 = (size_t)s->res() * s->res();
+    if (n_cells != R2) return fail(EMH_ERR_INVALID_ARG, "expected %zu cells", R2);
+    Ctrl ctrl;
+    HIP_TRY(hipMemcpyAsync(&ctrl, s->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const uint32_t g = ctrl.cur_grid & 1u;
+    if (mass) HIP_TRY(hipMemcpy(mass, s->pipe.grid.m[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
+    if (momentum) {
+        std::vector<float> mx(R2), my(R2);
+        HIP_TRY(hipMemcpy(mx.data(), s->pipe.grid.mx[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(my.data(), s->pipe.grid.my[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < R2; ++i) {
+            momentum[2 * i] = mx[i];
+            momentum[2 * i + 1] = my[i];
+        }
+    }
+    return EMH_OK;
+}
+
+int emh_sim_synchronize(emh_sim* s) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return EMH_OK;
+}
+
+int emh_sim_set_profiling(emh_sim* s, int enable) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    s->profiling = enable != 0;
+    for (int k = 0; k < kKernelCount; ++k) {
+        s->prof_ms[k] = 0.0;
+        s->prof_launches[k] = 0;
+    }
+    return EMH_OK;
+}
+
+int emh_sim_kernel_times(emh_sim* s, emh_kernel_time* out, uint32_t cap, uint32_t* n_out) {
+    if (!s || (!out && cap)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    uint32_t n = 0;
+    for (int k = 0; k < kKernelCount; ++k) {
+        if (n < cap) {
+            std::memset(&out[n], 0, sizeof(out[n]));
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", kKernelNames[k]);
+            out[n].launches = s->prof_launches[k];
+            out[n].total_ms = s->prof_ms[k];
+        }
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    return EMH_OK;
+}
+
+uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t R) {
+    int32_t hi = (int32_t)R - 1;
+    cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
+    cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
+    uint32_t tpd = (R + 15u) / 16u;
+    return (((uint32_t)cx >> 4) * tpd + ((uint32_t)cy >> 4)) * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
+}
+
+int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out, uint32_t* perm_out) {
+    if ((!xy || !keys_out || !perm_out) && n) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (R < 1 || R > 16384) return fail(EMH_ERR_INVALID_ARG, "grid_res out of range");
+    if (n > 0x7fffffffu) return fail(EMH_ERR_INVALID_ARG, "too many keys");
+    if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
+    const uint32_t tpd = (R + 15u) / 16u, n_tiles = tpd * tpd, n_bins = n_tiles * 256u;
+    const uint32_t nn = (uint32_t)n;
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    float2* d_xy = dalloc<float2>(nn);
+    uint32_t* d_keys = dalloc<uint32_t>(nn);
+    uint32_t* d_perm = dalloc<uint32_t>(nn);
+    uint32_t* d_rank = dalloc<uint32_t>(nn);
+    uint32_t* d_pu = dalloc<uint32_t>(nn);
+    uint32_t* d_cc = dalloc<uint32_t>(n_bins);
+    uint32_t* d_cs = dalloc<uint32_t>(n_bins + 1);
+    uint32_t* d_tc = dalloc<uint32_t>(n_tiles);
+    uint32_t* d_ts = dalloc<uint32_t>(n_tiles + 1);
+    int rc = EMH_OK;
+    if (!d_xy || !d_keys || !d_perm || !d_rank || !d_pu || !d_cc || !d_cs || !d_tc || !d_ts) {
+        rc = fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+    } else {
+        hipError_t e = hipSuccess;
+        if (nn) e = hipMemcpyAsync(d_xy, xy, sizeof(float2) * nn, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = sort_cell_keys_device(d_xy, nn, R, d_keys, d_perm, d_rank, d_cc, d_cs, d_tc, d_ts, d_pu, st);
+        if (e == hipSuccess && nn) e = hipMemcpyAsync(keys_out, d_keys, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && nn) e = hipMemcpyAsync(perm_out, d_perm, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "sort: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(d_xy);
+    (void)hipFree(d_keys);
+    (void)hipFree(d_perm);
+    (void)hipFree(d_rank);
+    (void)hipFree(d_pu);
+    (void)hipFree(d_cc);
+    (void)hipFree(d_cs);
+    (void)hipFree(d_tc);
+    (void)hipFree(d_ts);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,643 @@
+// device_math.hpp -- per-particle MLS-MPM math for the gfx950 kernels.
+//
+// Every formula follows the reference crate's CPU path operation by operation
+// (file:line cited per function) so that, for the same inputs, a kernel lane
+// computes the same f32 value the reference computes: the library is built
+// with -ffp-contract=off (no a*b+c fusion, as Rust does not fuse) and with
+// correctly rounded f32 division and square root.  Transcendentals
+// (logf/expf/sinf/sinhf/powf) are the device library's and may differ from
+// glibc's by an ulp; that is the only source of per-particle difference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emh {
+
+#define EMH_DEV __device__ __forceinline__
+
+constexpr float kF32Eps = 1.1920929e-07f;  // f32::EPSILON
+constexpr float kLogClamp = 1e-10f;        // materials/utils.rs:6
+constexpr float kMinJ = 1e-6f;             // materials/utils.rs:11
+constexpr float kDInv = 4.0f;              // KERNEL_D_INVERSE, solver/config.rs:22
+constexpr float kInf = __builtin_huge_valf();
+
+// Packed flags in the per-particle material word.
+constexpr uint32_t kPinnedBit = 0x80000000u;
+constexpr uint32_t kMatMask = 0x0000ffffu;
+
+// ------------------------------------------------------------------ config
+// Device copy of the SimConfig fields the substep reads, plus derived values.
+struct DevConfig {
+    uint32_t grid_res;
+    float cell;  // grid_cell_size
+    float dt;    // config.dt (the step length)
+    uint32_t adaptive;
+    uint32_t cfl_affine;
+    float cfl_coefficient;
+    float material_cfl;
+    float viscous_cfl;
+    float min_dt;
+    uint32_t project;
+    float proj_min_density;
+    float proj_min_volume;
+    float proj_min_j;
+    float gx, gy;
+    uint32_t bt;  // boundary_thickness
+    float default_initial_volume;
+    float particle_mass;
+    uint32_t max_substeps;
+    float apic_blend;
+    float j_max;
+    uint32_t n_slip;
+    uint32_t slip[8];
+    // derived
+    float proj_lo, proj_hi;  // particle clamp range of project_particle_state_to_admissible
+    float slip_lo[8], slip_hi[8];
+    uint32_t tiles_per_dim;   // 16x16-cell tiles per grid side
+    uint32_t n_bins;          // tiles_per_dim^2 * 256
+    uint32_t density_pass;    // any material needs_density_recompute || config flag
+};
+
+// ---------------------------------------------------------------- materials
+// Device material record, filled on the host from MaterialParams + extension.
+// Constants that the reference recomputes from material fields only (for
+// example sin(dilatancy_angle)) are evaluated once on the host with the same
+// f32 arithmetic / libm the reference uses.
+struct DevMat {
+    uint32_t model;
+    uint32_t flags;  // bit0 NACC hardening enabled, bit1 NACC advance F
+    float lambda, mu;
+    float min_density, min_volume;
+    // NeoHookean
+    float damage_rate, nh_viscosity;
+    // Fluid
+    float rest_density, dyn_visc, eos_k, eos_p, p_floor, bulk_visc, surf_tension, settling;
+    float fl_c2_denom;  // rest_density.max(min_density)
+    // Drucker-Prager
+    float dp_phi0, dp_h1, dp_h2, dp_h3, dp_vol_corr, dp_dilat, dp_sin_dilat, dp_cohesion_term, dp_ratio,
+        dp_min_vj, dp_qmax, dp_sqrt23;
+    // Snow
+    float sn_xi, sn_lo, sn_hi, sn_jmin, sn_jmax, sn_coh;
+    // NACC
+    float na_kappa, na_m, na_beta, na_xi;
+    // CFL modulus helpers
+    float wave_lambda;  // lambda used by elastic_wave_dt (NACC: kappa - mu)
+    float pad[3];
+};
+
+constexpr uint32_t kModelFallback = 0, kModelFluid = 1, kModelNH = 2, kModelCorot = 3, kModelSnow = 4,
+                   kModelDP = 5, kModelNacc = 10;
+
+// ------------------------------------------------------------ small algebra
+struct M2 {
+    float a, b, c, d;  // column-major: (a, b) first column, (c, d) second
+};
+EMH_DEV M2 m2(float a, float b, float c, float d) { return M2{a, b, c, d}; }
+EMH_DEV M2 mmul(const M2& p, const M2& q) {
+    return M2{p.a * q.a + p.c * q.b, p.b * q.a + p.d * q.b, p.a * q.c + p.c * q.d, p.b * q.c + p.d * q.d};
+}
+EMH_DEV float2 mvec(const M2& p, float x, float y) { return make_float2(p.a * x + p.c * y, p.b * x + p.d * y); }
+EMH_DEV float mdet(const M2& p) { return p.a * p.d - p.b * p.c; }
+EMH_DEV M2 mtr(const M2& p) { return M2{p.a, p.c, p.b, p.d}; }
+EMH_DEV M2 mscale(const M2& p, float s) { return M2{p.a * s, p.b * s, p.c * s, p.d * s}; }
+EMH_DEV M2 madd(const M2& p, const M2& q) { return M2{p.a + q.a, p.b + q.b, p.c + q.c, p.d + q.d}; }
+EMH_DEV M2 msub(const M2& p, const M2& q) { return M2{p.a - q.a, p.b - q.b, p.c - q.c, p.d - q.d}; }
+EMH_DEV bool mfinite(const M2& p) {
+    return __builtin_isfinite(p.a) && __builtin_isfinite(p.b) && __builtin_isfinite(p.c) && __builtin_isfinite(p.d);
+}
+EMH_DEV M2 from4(float4 v) { return M2{v.x, v.y, v.z, v.w}; }
+EMH_DEV float4 to4(const M2& p) { return make_float4(p.a, p.b, p.c, p.d); }
+// Rust f32::max/min ignore a NaN operand, as fmaxf/fminf do.
+EMH_DEV float rmax(float a, float b) { return fmaxf(a, b); }
+EMH_DEV float rmin(float a, float b) { return fminf(a, b); }
+EMH_DEV float rclamp(float x, float lo, float hi) {  // Rust f32::clamp (NaN passes through)
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+EMH_DEV float len2(float x, float y) { return __fsqrt_rn(x * x + y * y); }
+
+// ------------------------------------------------------------ B-spline weights
+// grid/kernel.rs:10-25.  Base cell = floor(x); d = (x - base) - 0.5.
+struct Wts {
+    int bx, by;
+    float wx[3], wy[3];
+};
+EMH_DEV int sat_i32(float f) {  // Rust `as i32`: saturating, NaN -> 0
+    if (__builtin_isnan(f)) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (-2147483647 - 1);
+    return (int)f;
+}
+EMH_DEV void axis_w(float d, float w[3]) {
+    float a = 0.5f - d, b = 0.5f + d;
+    w[0] = 0.5f * (a * a);
+    w[1] = 0.75f - d * d;
+    w[2] = 0.5f * (b * b);
+}
+EMH_DEV Wts weights(float x, float y) {
+    Wts w;
+    w.bx = sat_i32(floorf(x));
+    w.by = sat_i32(floorf(y));
+    axis_w((x - (float)w.bx) - 0.5f, w.wx);
+    axis_w((y - (float)w.by) - 0.5f, w.wy);
+    return w;
+}
+
+// ---------------------------------------------------------------- svd2
+// materials/svd.rs:14-85 (signed second singular value, det U = det V = +1).
+EMH_DEV float2 orth_complement(float x, float y) {
+    float px = -y, py = x;
+    float l = len2(px, py);
+    if (l > 1e-10f) return make_float2(px / l, py / l);
+    return make_float2(1.0f, 0.0f);
+}
+EMH_DEV void svd2(const M2& f, M2& u, float2& sig, M2& vt) {
+    float c00 = f.a * f.a + f.b * f.b;
+    float c01 = f.a * f.c + f.b * f.d;
+    float c11 = f.c * f.c + f.d * f.d;
+    float mean = (c00 + c11) * 0.5f;
+    float hd = (c00 - c11) * 0.5f;
+    float disc = __fsqrt_rn(hd * hd + c01 * c01);
+    float l1 = mean + disc, l2 = mean - disc;
+    float v1x, v1y;
+    if (fabsf(c01) > 1e-10f * rmax(c00 + c11, 1e-30f)) {
+        float rx = c01, ry = l1 - c00;
+        float rl = len2(rx, ry);
+        v1x = rx / rl;
+        v1y = ry / rl;
+    } else if (c00 >= c11) {
+        v1x = 1.0f;
+        v1y = 0.0f;
+    } else {
+        v1x = 0.0f;
+        v1y = 1.0f;
+    }
+    float v2x = -v1y, v2y = v1x;
+    vt = M2{v1x, v2x, v1y, v2y};  // transpose of [v1 | v2]
+    float s1 = __fsqrt_rn(rmax(l1, 0.0f)), s2 = __fsqrt_rn(rmax(l2, 0.0f));
+    float2 fv1 = mvec(f, v1x, v1y), fv2 = mvec(f, v2x, v2y);
+    float2 u1 = s1 > 1e-10f ? make_float2(fv1.x / s1, fv1.y / s1) : orth_complement(fv2.x, fv2.y);
+    float2 u2 = s2 > 1e-10f ? make_float2(fv2.x / s2, fv2.y / s2) : orth_complement(u1.x, u1.y);
+    u = M2{u1.x, u1.y, u2.x, u2.y};
+    if (mdet(u) < 0.0f) {
+        u.c = -u.c;
+        u.d = -u.d;
+        s2 = -s2;
+    }
+    sig = make_float2(s1, s2);
+}
+// U diag(s) V^T  (utils.rs:67-69, nacc.rs:227-229)
+EMH_DEV M2 usv(const M2& u, float s1, float s2, const M2& vt) { return mmul(mmul(u, M2{s1, 0.0f, 0.0f, s2}), vt); }
+
+// materials/utils.rs:129-138
+EMH_DEV M2 polar_r(const M2& f) {
+    float x = f.a + f.d;
+    float y = f.b - f.c;
+    float n = __fsqrt_rn(x * x + y * y);
+    if (n > kF32Eps) return M2{x / n, y / n, -y / n, x / n};
+    return M2{1.0f, 0.0f, 0.0f, 1.0f};
+}
+
+// materials/utils.rs:145-164
+EMH_DEV float elastic_wave_dt(float lambda, float mu, float h, float density, float min_density, float cell,
+                              float mcfl) {
+    float rho = rmax(density, min_density);
+    float modulus = rmax((lambda + 2.0f * mu) * h, 0.0f);
+    if (modulus <= kF32Eps) return kInf;
+    float c = __fsqrt_rn(modulus / rho);
+    if (c <= kF32Eps) return kInf;
+    return mcfl * cell / c;
+}
+
+// --------------------------------------------------------- particle state
+struct PState {
+    float x, y, vx, vy;
+    M2 c, f;
+    float mass, v0, vol, dens, jp, h, q, lvs;
+    uint32_t mat;  // material id | pinned bit
+};
+
+// project_particle_state_to_admissible  solver/step.rs:347-429
+EMH_DEV bool project_state(PState& s, const DevConfig& cfg) {
+    bool p = false;
+    if (!(__builtin_isfinite(s.x) && __builtin_isfinite(s.y))) {
+        float c = (cfg.proj_lo + cfg.proj_hi) * 0.5f;
+        s.x = c;
+        s.y = c;
+        p = true;
+    } else {
+        s.x = rmin(rmax(s.x, cfg.proj_lo), cfg.proj_hi);
+        s.y = rmin(rmax(s.y, cfg.proj_lo), cfg.proj_hi);
+    }
+    if (!(__builtin_isfinite(s.vx) && __builtin_isfinite(s.vy))) {
+        s.vx = 0.0f;
+        s.vy = 0.0f;
+        p = true;
+    }
+    if (!mfinite(s.c)) {
+        s.c = M2{0.0f, 0.0f, 0.0f, 0.0f};
+        p = true;
+    }
+    if (!mfinite(s.f) || mdet(s.f) <= cfg.proj_min_j) {
+        s.f = M2{1.0f, 0.0f, 0.0f, 1.0f};
+        p = true;
+    } else {
+        float j = mdet(s.f);
+        if (j > cfg.j_max) {
+            s.f = mscale(s.f, __fsqrt_rn(cfg.j_max / j));
+            p = true;
+        }
+    }
+    if (!__builtin_isfinite(s.jp) || s.jp <= 0.0f) {
+        s.jp = 1.0f;
+        p = true;
+    }
+    if (!__builtin_isfinite(s.h) || s.h <= 0.0f) {
+        s.h = 1.0f;
+        p = true;
+    }
+    if (!__builtin_isfinite(s.q)) {
+        s.q = 0.0f;
+        p = true;
+    }
+    if (!__builtin_isfinite(s.lvs)) {
+        s.lvs = 0.0f;
+        p = true;
+    }
+    if (!__builtin_isfinite(s.mass) || s.mass <= 0.0f) {
+        s.mass = cfg.particle_mass;
+        p = true;
+    }
+    if (!__builtin_isfinite(s.v0) || s.v0 <= 0.0f) {
+        s.v0 = rmax(cfg.default_initial_volume, cfg.proj_min_volume);
+        p = true;
+    }
+    if (!__builtin_isfinite(s.vol) || s.vol <= 0.0f) {
+        s.vol = rmax(s.v0, cfg.proj_min_volume);
+        p = true;
+    }
+    if (!__builtin_isfinite(s.dens) || s.dens <= 0.0f) {
+        s.dens = rmax(s.mass / s.vol, cfg.proj_min_density);
+        p = true;
+    } else {
+        s.dens = rmax(s.dens, cfg.proj_min_density);
+    }
+    return p;
+}
+
+// ------------------------------------------------------- constitutive models
+// stress_volume: materials/mod.rs:118-120, fluid.rs:193-195
+EMH_DEV float stress_volume(const DevMat& m, const PState& s) {
+    return m.model == kModelFluid ? rmax(s.vol, m.min_volume) : s.v0;
+}
+
+EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
+    M2 r = polar_r(f);
+    M2 t1 = mmul(mscale(msub(f, r), 2.0f * mu), mtr(f));
+    float iso = (lambda * (j - 1.0f)) * j;
+    return madd(t1, M2{iso, 0.0f, 0.0f, iso});
+}
+
+// Kirchhoff stress: elastic.rs:180-255, corotated.rs:62-78, sand.rs:220-231,
+// snow.rs:109-136, fluid.rs:135-191, nacc.rs:236-249, Fallback mod.rs:113-115.
+EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
+    const M2 zero{0.0f, 0.0f, 0.0f, 0.0f};
+    switch (m.model) {
+        case kModelNH: {
+            float j = mdet(s.f);
+            if (j <= kMinJ) return zero;
+            float dscale = expf(-m.damage_rate * s.q);
+            float mu = m.mu * 1.0f * dscale;
+            float lambda = m.lambda * 1.0f * dscale;
+            M2 b = mmul(s.f, mtr(s.f));
+            float trb = b.a + b.d;
+            float hb = trb * 0.5f;
+            M2 devb = M2{b.a - hb, b.b - 0.0f, b.c - 0.0f, b.d - hb};
+            float k = lambda + mu;
+            M2 dev = mscale(devb, mu / j);
+            float vol = k * logf(j);
+            M2 t = madd(dev, M2{vol, 0.0f * vol, 0.0f * vol, vol});
+            if (m.nh_viscosity > 0.0f) {
+                M2 sym = madd(s.c, mtr(s.c));
+                M2 d = mscale(sym, 0.5f);
+                float hdt = (d.a + d.d) * 0.5f;
+                M2 ddev = M2{d.a - hdt, d.b - 0.0f, d.c - 0.0f, d.d - hdt};
+                t = madd(t, mscale(ddev, m.nh_viscosity));
+            } else {
+                t = madd(t, zero);
+            }
+            return t;
+        }
+        case kModelCorot: {
+            float j = mdet(s.f);
+            if (j <= kMinJ) return zero;
+            return corot_tau(s.f, j, m.mu * s.h * 1.0f, m.lambda * s.h * 1.0f);
+        }
+        case kModelDP: {
+            float j = mdet(s.f);
+            if (j <= kMinJ) return zero;
+            return corot_tau(s.f, j, m.mu, m.lambda);
+        }
+        case kModelSnow: {
+            float j = mdet(s.f);
+            if (j <= kMinJ) return zero;
+            M2 t = corot_tau(s.f, j, m.mu * s.h, m.lambda * s.h);
+            if (m.sn_coh > 0.0f && s.jp < 1.0f && j > 1.0f) {
+                float iso = ((m.sn_coh * s.jp) * (j - 1.0f)) * j;
+                t = madd(t, M2{iso, 0.0f, 0.0f, iso});
+            }
+            return t;
+        }
+        case kModelFluid: {
+            float dens = rmin(rmax(s.dens, m.min_density), m.rest_density * 2.0f);
+            float p = rmax(m.eos_k * (powf(dens / m.rest_density, m.eos_p) - 1.0f), m.p_floor);
+            M2 t{-p, 0.0f, 0.0f, -p};
+            M2 sym = madd(s.c, mtr(s.c));
+            float div = sym.a + sym.d;
+            float hd = div * 0.5f;
+            M2 sdev = M2{sym.a - hd, sym.b - 0.0f, sym.c - 0.0f, sym.d - hd};
+            t = madd(t, mscale(sdev, m.dyn_visc));
+            if (m.bulk_visc > 0.0f) {
+                float bv = m.bulk_visc * div * 0.5f;
+                t = madd(t, M2{bv, 0.0f, 0.0f, bv});
+            }
+            if (m.surf_tension != 0.0f) {
+                float j = s.f.a * s.f.d - s.f.b * s.f.c;
+                float st = m.surf_tension * j;
+                t = madd(t, M2{st, 0.0f, 0.0f, st});
+            }
+            return t;
+        }
+        case kModelNacc: {
+            float j = rmax(mdet(s.f), kMinJ);
+            M2 b = mmul(s.f, mtr(s.f));
+            float hb = (b.a + b.d) * 0.5f;
+            M2 devb = M2{b.a - hb, b.b - 0.0f, b.c - 0.0f, b.d - hb};
+            M2 dev = mscale(devb, m.mu / j);
+            float vol = (m.na_kappa * 0.5f) * (j * j - 1.0f);
+            return madd(dev, M2{vol, 0.0f, 0.0f, vol});
+        }
+        default:
+            return zero;
+    }
+}
+
+// timestep_bound: elastic.rs:290-327, corotated.rs:108-125, fluid.rs:228-259,
+// sand.rs:345-362, snow.rs:193-211, nacc.rs:279-298 (Fallback: +inf).
+EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevConfig& cfg) {
+    switch (m.model) {
+        case kModelNH: {
+            float e = elastic_wave_dt(m.lambda, m.mu, 1.0f, density, m.min_density, cfg.cell, cfg.material_cfl);
+            float v = kInf;
+            if (m.nh_viscosity > 0.0f) {
+                float d = rmax(density, 1.0e-6f);
+                float kin = m.nh_viscosity / d;
+                if (kin > kF32Eps) v = cfg.viscous_cfl * cfg.cell * cfg.cell / kin;
+            }
+            return rmin(e, v);
+        }
+        case kModelCorot:
+        case kModelSnow:
+            return elastic_wave_dt(m.lambda, m.mu, h, density, kMinJ, cfg.cell, cfg.material_cfl);
+        case kModelDP:
+            return elastic_wave_dt(m.lambda, m.mu, 1.0f, density, kMinJ, cfg.cell, cfg.material_cfl);
+        case kModelNacc:
+            return elastic_wave_dt(m.wave_lambda, m.mu, h, density, m.min_density, cfg.cell, cfg.material_cfl);
+        case kModelFluid: {
+            float d = rmax(density, m.min_density);
+            float ratio = rmax(d / m.fl_c2_denom, 1.0e-6f);
+            float out = kInf;
+            float c2 = m.eos_k * m.eos_p * powf(ratio, m.eos_p - 1.0f) / m.fl_c2_denom;
+            if (__builtin_isfinite(c2) && c2 > kF32Eps) out = rmin(out, cfg.material_cfl * cfg.cell / __fsqrt_rn(c2));
+            if (m.dyn_visc > 0.0f) {
+                float kin = m.dyn_visc / d;
+                if (kin > kF32Eps) out = rmin(out, cfg.viscous_cfl * cfg.cell * cfg.cell / kin);
+            }
+            return out;
+        }
+        default:
+            return kInf;
+    }
+}
+
+// DruckerPragerMaterial::alpha + project  sand.rs:135-194
+EMH_DEV bool dp_project(const DevMat& m, float s1, float s2, float lvs, float q, float& o1, float& o2,
+                        float& dq) {
+    float a1 = rmax(fabsf(s1), kLogClamp), a2 = rmax(fabsf(s2), kLogClamp);
+    float e1 = logf(a1) + lvs * 0.5f;
+    float e2 = logf(a2) + lvs * 0.5f;
+    float tr = e1 + e2;
+    float ht = tr * 0.5f;
+    float d1 = e1 - ht, d2 = e2 - ht;
+    float dn = len2(d1, d2);
+    if (dn == 0.0f || tr > 0.0f) {
+        o1 = 1.0f;
+        o2 = 1.0f;
+        dq = dn;
+        return true;
+    }
+    float phi = m.dp_phi0 + (m.dp_h1 * q - m.dp_h3) * expf(-m.dp_h2 * q);
+    float sp = sinf(phi);
+    float alpha = m.dp_sqrt23 * (2.0f * sp) / (3.0f - sp);
+    float gamma = dn + m.dp_ratio * tr * alpha - m.dp_cohesion_term;
+    if (gamma <= 0.0f) return false;
+    float h1 = e1 - gamma * (d1 / dn);
+    float h2 = e2 - gamma * (d2 / dn);
+    o1 = expf(h1);
+    o2 = expf(h2);
+    dq = gamma;
+    return true;
+}
+
+// NaccMaterial::project  nacc.rs:129-223
+EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, float& ao) {
+    const float xi = m.na_xi, beta = m.na_beta, mm = m.na_m, kappa = m.na_kappa;
+    const bool hard = (m.flags & 1u) != 0u;
+    M2 u, vt;
+    float2 sg;
+    svd2(f, u, sg, vt);
+    float q1 = sg.x * sg.x, q2 = sg.y * sg.y;
+    float qt = q1 + q2;
+    float p0 = kappa * (1e-5f + sinhf(xi * rmax(-alpha, 0.0f)));
+    float jtr = rmax(sg.x * sg.y, 1.0e-6f);
+    float mu_inv = m.mu * (1.0f / jtr);
+    float st1 = mu_inv * (q1 - qt * 0.5f), st2 = mu_inv * (q2 - qt * 0.5f);
+    float psi = (kappa * 0.5f) * (jtr - 1.0f / jtr);
+    float ptr = -psi * jtr;
+    if (ptr > p0) {
+        float jn = __fsqrt_rn(rmax(-2.0f * p0 / kappa + 1.0f, 1.0e-8f));
+        float sv = powf(jn, 0.5f);
+        if (hard) alpha = alpha + logf(jtr / jn);
+        fo = usv(u, sv, sv, vt);
+        ao = alpha;
+        return;
+    }
+    if (ptr < -beta * p0) {
+        float jn = __fsqrt_rn(rmax(2.0f * beta * p0 / kappa + 1.0f, 1.0e-8f));
+        float sv = powf(jn, 0.5f);
+        if (hard) alpha = alpha + logf(jtr / jn);
+        fo = usv(u, sv, sv, vt);
+        ao = alpha;
+        return;
+    }
+    float y0 = (1.0f + 2.0f * beta) * 2.0f;
+    float y1 = mm * mm * (ptr + beta * p0) * (ptr - p0);
+    float sn2 = st1 * st1 + st2 * st2;
+    float y = y0 * sn2 + y1;
+    if (y < 1.0e-4f) {
+        fo = f;
+        ao = alpha;
+        return;
+    }
+    if (hard && p0 > 1.0e-4f && ptr < p0 - 1.0e-4f && ptr > -beta * p0 + 1.0e-4f) {
+        float pc = (1.0f - beta) * p0 * 0.5f;
+        float qtr = __fsqrt_rn(2.0f) * len2(st1, st2);
+        float dx = pc - ptr, dy = -qtr;
+        float rcp = 1.0f / len2(dx, dy);
+        if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+            dx = dx * rcp;
+            dy = dy * rcp;
+        } else {
+            dx = 0.0f;
+            dy = 0.0f;
+        }
+        float c = mm * mm * (pc + beta * p0) * (pc - p0);
+        float b = mm * mm * dx * (2.0f * pc - p0 + beta * p0);
+        float a = mm * mm * dx * dx + (1.0f + 2.0f * beta) * dy * dy;
+        float disc = __fsqrt_rn(rmax(b * b - 4.0f * a * c, 0.0f));
+        float l1 = (-b + disc) / (2.0f * a);
+        float l2 = (-b - disc) / (2.0f * a);
+        float p1 = pc + l1 * dx;
+        float p2 = pc + l2 * dx;
+        float px = ((ptr - pc) * (p1 - pc) > 0.0f) ? p1 : p2;
+        float jex = __fsqrt_rn(rmax(fabsf(-2.0f * px / kappa + 1.0f), 1.0e-8f));
+        if (jex > 1.0e-4f) alpha = alpha + logf(jtr / jex);
+    }
+    float sc = __fsqrt_rn(rmax(-y1 / rmax(y0, 1.0e-10f), 0.0f)) * (jtr / m.mu);
+    float nx = st1, ny = st2;
+    float rcp = 1.0f / len2(nx, ny);
+    if (__builtin_isfinite(rcp) && rcp > 0.0f) {
+        nx = nx * rcp;
+        ny = ny * rcp;
+    } else {
+        nx = 0.0f;
+        ny = 0.0f;
+    }
+    float b1 = sc * nx + qt / 2.0f;
+    float b2 = sc * ny + qt / 2.0f;
+    fo = usv(u, __fsqrt_rn(rmax(b1, 1.0e-8f)), __fsqrt_rn(rmax(b2, 1.0e-8f)), vt);
+    ao = alpha;
+}
+
+EMH_DEV void sync_volume(PState& s) {
+    float j = rmax(mdet(s.f), kMinJ);
+    float v = rmax(s.v0 * j, 1.0e-6f);
+    s.vol = v;
+    s.dens = s.mass / v;
+}
+
+// update_particle: elastic.rs:262-269, corotated.rs:84-91, sand.rs:248-319,
+// snow.rs:142-176, fluid.rs:197-207, nacc.rs:255-269; Fallback: no-op.
+EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
+    switch (m.model) {
+        case kModelNH:
+        case kModelCorot: {
+            M2 fp = madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt));
+            s.f = mmul(fp, s.f);
+            sync_volume(s);
+            break;
+        }
+        case kModelDP: {
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            M2 u, vt;
+            float2 sg;
+            svd2(ft, u, sg, vt);
+            float n1 = sg.x, n2 = sg.y;
+            float p1, p2, dq;
+            if (dp_project(m, sg.x, sg.y, s.lvs, s.q, p1, p2, dq)) {
+                float a1 = rmax(fabsf(sg.x), kLogClamp), a2 = rmax(fabsf(sg.y), kLogClamp);
+                float prev = a1 * a2;
+                float nd = p1 * p2;
+                float diff = nd - prev;
+                float corr = diff > 0.0f ? nd : prev + diff * m.dp_vol_corr;
+                s.lvs = s.lvs + (logf(prev) - logf(corr));
+                s.q = rmin(s.q + dq, m.dp_qmax);
+                if (m.dp_dilat > 0.0f) s.lvs = s.lvs + m.dp_sin_dilat * dq;
+                n1 = p1;
+                n2 = p2;
+            }
+            n1 = fabsf(n1);
+            n2 = fabsf(n2);
+            float jn = n1 * n2;
+            if (jn < m.dp_min_vj) {
+                float k = __fsqrt_rn(m.dp_min_vj / rmax(jn, 1e-6f));
+                n1 = n1 * k;
+                n2 = n2 * k;
+            }
+            s.f = usv(u, n1, n2, vt);
+            sync_volume(s);
+            break;
+        }
+        case kModelSnow: {
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            M2 u, vt;
+            float2 sg;
+            svd2(ft, u, sg, vt);
+            float c1 = rclamp(sg.x, m.sn_lo, m.sn_hi), c2 = rclamp(sg.y, m.sn_lo, m.sn_hi);
+            float jpn = s.jp * (sg.x * sg.y) / (c1 * c2);
+            s.jp = rclamp(jpn, m.sn_jmin, m.sn_jmax);
+            s.h = rclamp(expf(m.sn_xi * (1.0f - s.jp)), 0.1f, 7.0f);
+            s.f = usv(u, c1, c2, vt);
+            sync_volume(s);
+            break;
+        }
+        case kModelFluid: {
+            float j = rclamp(mdet(s.f), 0.5f, 2.0f);
+            float r = __fsqrt_rn(j);
+            s.f = M2{r, 0.0f, 0.0f, r};
+            if (m.settling > 0.0f) {
+                float k = 1.0f - rmin(m.settling * dt, 0.5f);
+                s.vx = s.vx * k;
+                s.vy = s.vy * k;
+            }
+            break;
+        }
+        case kModelNacc: {
+            M2 fin = s.f;
+            if (m.flags & 2u) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
+            M2 nf;
+            float na;
+            nacc_project(m, fin, s.lvs, nf, na);
+            s.f = nf;
+            s.lvs = na;
+            float j = rmax(mdet(nf), kMinJ);
+            float vol = s.v0 * j;
+            s.vol = vol;
+            if (vol > kMinJ) s.dens = s.mass / vol;
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+// affine_cfl_speed_contribution  solver/step.rs:481-488
+EMH_DEV float affine_cfl(const M2& c, float cell) {
+    const float kCorner = 1.5f * 1.41421356237309504880f;
+    float g = __fsqrt_rn((c.a * c.a + c.b * c.b) + (c.c * c.c + c.d * c.d));
+    return g * kCorner * cell;
+}
+
+// Cell key: tile-major over 16x16-cell tiles, local cell x-major; cell
+// coordinates clamped into the grid.
+EMH_DEV uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t tpd) {
+    int hi = (int)R - 1;
+    cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
+    cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
+    uint32_t tile = ((uint32_t)cx >> 4) * tpd + ((uint32_t)cy >> 4);
+    return tile * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
+}
+
+}  // namespace emh

@@ -1,0 +1,312 @@
+/*
+ * emerge_hip.h -- C-ABI of the MI355X (gfx950) MLS-MPM substep library.
+ *
+ * The library is a drop-in backend for the reference crate's CPU
+ * `Simulation::step` (src/spacetime/solver/step.rs:23-60): a caller builds a
+ * simulation from the same SimConfig / MaterialParams / SlipBoundary / Particle
+ * data the reference uses, and `emh_sim_step(sim, n)` advances it exactly as
+ * `n` calls of `Simulation::step` would (adaptive CFL substeps, state
+ * projection, density re-estimate, P2G, grid update with gravity + slip walls,
+ * G2P and the constitutive/plasticity update), on one GPU.
+ *
+ * Conventions
+ *   - Every function returns EMH_OK (0) or a negative EMH_ERR_* code; the
+ *     message of the last failure on the calling thread is emh_last_error().
+ *     Nothing aborts the process (the reference panics/asserts instead,
+ *     src/spacetime/solver/config.rs:247-299, materials/registry.rs:29-49).
+ *   - The caller owns every host buffer; the library copies in and out and
+ *     keeps no caller pointer.  A handle is not thread-safe: drive one handle
+ *     from one host thread.
+ *   - Struct layouts below are byte-compatible with the reference's
+ *     repr(C) types where the reference has one (Particle 128 B,
+ *     MaterialParams 96 B); `emh_config` is a fixed-width mirror of SimConfig
+ *     (usize and bool fields become uint32_t).
+ */
+#ifndef EMERGE_HIP_H
+#define EMERGE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMH_ABI_VERSION 1u
+
+/* ---- status codes ------------------------------------------------------- */
+#define EMH_OK                 0
+#define EMH_ERR_INVALID_ARG   -1  /* null pointer, bad size, failed validate() */
+#define EMH_ERR_UNSUPPORTED   -2  /* a material/boundary/feature the GPU path does not implement */
+#define EMH_ERR_DEVICE        -3  /* HIP runtime error or device loss */
+#define EMH_ERR_OUT_OF_MEMORY -4
+#define EMH_ERR_STATE         -5  /* call made in a state that does not allow it */
+
+/* ---- constitutive model ids: ConstitutiveModel discriminants -------------
+ * src/matter/materials/mod.rs:53-66 (ABI-stable values asserted at :71-85). */
+#define EMH_MODEL_FALLBACK        0u
+#define EMH_MODEL_FLUID           1u
+#define EMH_MODEL_NEOHOOKEAN      2u
+#define EMH_MODEL_COROTATED       3u
+#define EMH_MODEL_SNOW            4u
+#define EMH_MODEL_DRUCKER_PRAGER  5u
+#define EMH_MODEL_NACC           10u
+
+#define EMH_MAX_MATERIALS  64u   /* MAX_MATERIAL_SLOTS, materials/registry.rs:6 */
+#define EMH_MAX_BOUNDARIES  8u
+
+/* ---- SimConfig mirror: src/spacetime/solver/config.rs:26-121 ------------ */
+typedef struct emh_config {
+    uint32_t grid_res;                    /* usize */
+    float    grid_cell_size;
+    float    dt;
+    uint32_t adaptive_timestep;           /* bool */
+    uint32_t cfl_include_affine_speed;    /* bool */
+    float    cfl_coefficient;
+    float    material_cfl_coefficient;
+    float    viscous_timestep_coefficient;
+    float    min_dt;
+    uint32_t project_invalid_state;       /* bool */
+    float    projection_min_density;
+    float    projection_min_volume;
+    float    projection_min_deformation_j;
+    float    gravity[2];
+    uint32_t boundary_thickness;          /* usize */
+    float    default_initial_volume;
+    uint32_t recompute_density_each_step; /* bool */
+    float    particle_mass;
+    uint32_t max_substeps_per_step;       /* usize */
+    float    apic_blend;
+    float    j_max;
+    float    sleep_threshold;             /* must be 0: sleep partition is not supported */
+    float    contact_friction;
+    float    asflip_blend;                /* must be 0: ASFLIP is not supported */
+    float    mixture_drag_coefficient;
+    uint32_t mixture_pressure_iterations;
+    float    dx_meters;
+    float    dt_seconds;
+} emh_config;                             /* 116 bytes */
+
+/* ---- MaterialParams mirror (96 B): src/matter/materials/params.rs:8-101 --
+ * Field meaning is per model, exactly as each model's `params()` fills it:
+ *   NeoHookean  elastic.rs:275-288   Corotated corotated.rs:97-106
+ *   Fluid       fluid.rs:209-226     DruckerPrager sand.rs:321-343
+ *   Snow        snow.rs:178-191
+ *   Nacc        nacc.rs:300-314 uploads model=NeoHookean; this library needs
+ *               model=EMH_MODEL_NACC (10) instead, with the same field use
+ *               (lambda = kappa - mu, hardening_exponent = xi,
+ *                compression_limit = beta, stretch_limit = M). */
+typedef struct emh_material_params {
+    uint32_t model;
+    float lambda;
+    float mu;
+    float hardening_exponent;
+    float compression_limit;
+    float stretch_limit;
+    float rest_density;
+    float eos_stiffness;
+    float eos_power;
+    float dynamic_viscosity;
+    float volume_ratio_min;
+    float volume_ratio_max;
+    float dp_h0;
+    float dp_h1;
+    float dp_h2;
+    float dp_h3;
+    float active_stress_coeff;
+    float hardening_modulus;
+    float thermal_viscosity_coeff;
+    float thermal_expansion;
+    float pressure_floor;
+    float bulk_viscosity;
+    float surface_tension_coeff;
+    float cohesion_coeff;
+} emh_material_params;
+
+/* Model fields that MaterialParams drops (each model struct's constructor
+ * default in brackets).  Pass NULL to emh_sim_set_material for the defaults. */
+typedef struct emh_material_ext {
+    float    min_density;            /* NH elastic.rs:55, Fluid fluid.rs:60, NACC nacc.rs:71 [1e-6] */
+    float    min_volume;             /* Fluid fluid.rs:61 [1e-6] */
+    float    dp_volume_correction;   /* DP sand.rs:89 [1.0] */
+    float    nacc_kappa;             /* NACC bulk modulus kappa (nacc.rs:40); 0 = lambda + mu */
+    uint32_t nacc_hardening_enabled; /* nacc.rs:70 [hardening_factor > 0]; 2 = derive */
+    uint32_t nacc_advance_f;         /* 0: F <- project(F_old) as CPU Simulation does (nacc.rs:255-269);
+                                        1: F <- project((I + dt C) F_old) as the reference GPU path does */
+    uint32_t reserved[2];
+} emh_material_ext;                  /* 32 bytes */
+
+/* ---- Particle mirror (128 B): src/matter/particle.rs:11-144 ------------- */
+typedef struct emh_particle {
+    float    x[2];
+    float    v[2];
+    float    velocity_gradient[4];    /* glam Mat2, column-major: c00, c10, c01, c11 */
+    float    deformation_gradient[4]; /* column-major */
+    float    mass;
+    float    initial_volume;
+    float    volume;
+    float    density;
+    uint32_t material_id;
+    float    plastic_volume_ratio;
+    float    hardening_scale;
+    float    friction_hardening;
+    float    log_volume_strain;
+    float    temperature;
+    uint32_t user_tag;
+    float    activation;
+    float    activation_dir[2];
+    uint32_t muscle_group_id;
+    uint32_t contact_group;
+    uint32_t sleeping;
+    uint32_t pinned;
+    float    scalar_field;
+    uint32_t _pad;
+} emh_particle;
+
+/* ---- SpawnRegion mirror: src/spacetime/solver/config.rs:317-364 --------- */
+#define EMH_SHAPE_BOX  0u
+#define EMH_SHAPE_DISK 1u
+typedef struct emh_spawn_region {
+    float    spacing;
+    int32_t  box_size[2];
+    float    box_center[2];
+    uint32_t shape;                    /* EMH_SHAPE_* */
+    float    disk_radius;
+    float    initial_deformation_gradient[4];
+    uint32_t precompute_initial_volumes;
+    float    initial_velocity_scale;
+    float    position_jitter;
+    uint32_t rng_seed;
+    uint32_t material_id;
+    uint32_t has_mass_override;
+    float    mass_override;
+} emh_spawn_region;
+
+/* ---- per-step diagnostics (Simulation fields, solver/mod.rs:60-66) ------ */
+typedef struct emh_step_stats {
+    uint32_t last_substeps;           /* last_substeps()  step.rs:323 */
+    float    last_step_dt;            /* effective_dt()   step.rs:319 */
+    float    last_sim_time_dropped;   /* step.rs:51 */
+    uint32_t last_vel_clamp_count;    /* step.rs:196 */
+    uint32_t last_j_projection_count; /* step.rs:69-71 */
+    uint32_t total_substeps;          /* since creation */
+    uint64_t frame_index;             /* step.rs:59 */
+} emh_step_stats;
+
+typedef struct emh_sim emh_sim;
+
+/* ---- library ------------------------------------------------------------ */
+uint32_t    emh_abi_version(void);
+const char* emh_last_error(void);
+/* Number of HIP devices visible; 0 without a GPU (does not fail). */
+int         emh_device_count(int* count);
+
+/* ---- config helpers ----------------------------------------------------- */
+/* SimConfig::default()  config.rs:123-159 */
+int emh_config_default(emh_config* out);
+/* SimConfig::standard(grid_res, dt, gravity)  config.rs:165-172 */
+int emh_config_standard(uint32_t grid_res, float dt, float gx, float gy, emh_config* out);
+/* SimConfig::unsafe_defaults()  config.rs:179-185 */
+int emh_config_unsafe_defaults(emh_config* out);
+/* SimConfig::validate()  config.rs:247-299 (error code instead of a panic) */
+int emh_config_validate(const emh_config* cfg);
+
+/* ---- material constructors (each fills the 96 B params + the extension) -- */
+/* lame_from_young  materials/utils.rs:183-193 */
+int emh_lame_from_young(float young_modulus, float poisson_ratio, float* lambda, float* mu);
+/* NeoHookeanMaterial::new  elastic.rs:51-61 */
+int emh_material_neohookean(float lambda, float mu, emh_material_params* p, emh_material_ext* e);
+/* CorotatedMaterial::new  corotated.rs:29-36 */
+int emh_material_corotated(float lambda, float mu, emh_material_params* p, emh_material_ext* e);
+/* NewtonianFluidMaterial::new  fluid.rs:48-67 */
+int emh_material_fluid(float rest_density, float dynamic_viscosity, float eos_stiffness,
+                       float eos_power, emh_material_params* p, emh_material_ext* e);
+/* NewtonianFluidMaterial::low_viscosity  fluid.rs:73-75 */
+int emh_material_fluid_low_viscosity(float rest_density, float eos_stiffness,
+                                     emh_material_params* p, emh_material_ext* e);
+/* DruckerPragerMaterial::new  sand.rs:81-94 */
+int emh_material_drucker_prager(float lambda, float mu, emh_material_params* p, emh_material_ext* e);
+/* StomakhinMaterial::new  snow.rs:40-59 */
+int emh_material_snow(float lambda, float mu, float hardening_exponent, float compression_limit,
+                      float stretch_limit, float min_plastic_jacobian, float max_plastic_jacobian,
+                      emh_material_params* p, emh_material_ext* e);
+/* StomakhinMaterial::from_young_modulus  snow.rs:68-71 */
+int emh_material_snow_from_young(float young_modulus, float poisson_ratio,
+                                 emh_material_params* p, emh_material_ext* e);
+/* NaccMaterial::new(mu, kappa, friction M, cohesion beta, hardening xi)  nacc.rs:63-73 */
+int emh_material_nacc(float mu, float kappa, float friction, float cohesion, float hardening_factor,
+                      emh_material_params* p, emh_material_ext* e);
+/* NaccMaterial::from_young_modulus  nacc.rs:77-90 */
+int emh_material_nacc_from_young(float young_modulus, float poisson_ratio, float friction,
+                                 float cohesion, float hardening_factor,
+                                 emh_material_params* p, emh_material_ext* e);
+/* f32::to_radians as Rust computes it (for friction angles) */
+float emh_to_radians(float degrees);
+
+/* ---- simulation lifecycle (solver/lifecycle.rs) -------------------------- */
+/* Simulation::empty(config)  lifecycle.rs:26-55: Fallback material 0 and one
+ * SlipBoundary(config.boundary_thickness). */
+int emh_sim_create(const emh_config* cfg, emh_sim** out);
+/* Simulation::new(config, spawn)  lifecycle.rs:57-105 */
+int emh_sim_create_from_spawn(const emh_config* cfg, const emh_spawn_region* spawn, emh_sim** out);
+void emh_sim_destroy(emh_sim* sim);
+/* with_material / set_material (id == 0: with_default_material, which also
+ * re-runs init_particle on every particle, lifecycle.rs:107-111,328-334).
+ * ext may be NULL.  Ids must be contiguous (registry.rs:29-49). */
+int emh_sim_set_material(emh_sim* sim, uint32_t material_id, const emh_material_params* params,
+                         const emh_material_ext* ext);
+/* Simulation::reinit_all_particle_state  lifecycle.rs:328-334 */
+int emh_sim_reinit_particle_state(emh_sim* sim);
+/* with_boundary(Box::new(SlipBoundary::new(t)))  lifecycle.rs:114-117, slip.rs:5-24 */
+int emh_sim_add_slip_boundary(emh_sim* sim, uint32_t thickness);
+/* clear_boundaries  lifecycle.rs:364-366 */
+int emh_sim_clear_boundaries(emh_sim* sim);
+/* add_body(spawn) -> tag  solver/particles.rs:355-424 */
+int emh_sim_add_body(emh_sim* sim, const emh_spawn_region* spawn, uint32_t* tag_out);
+/* Replace all particles (all active).  Particle::material_id must be registered. */
+int emh_sim_set_particles(emh_sim* sim, const emh_particle* particles, size_t n);
+int emh_sim_num_particles(const emh_sim* sim, size_t* n);
+/* particles() in index order (copies device state to the host). */
+int emh_sim_get_particles(emh_sim* sim, emh_particle* out, size_t n);
+int emh_sim_get_config(const emh_sim* sim, emh_config* out);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* n_steps x Simulation::step  step.rs:23-60 / step_n step.rs:327-331 */
+int emh_sim_step(emh_sim* sim, uint32_t n_steps);
+int emh_sim_stats(const emh_sim* sim, emh_step_stats* out);
+/* Grid after the last substep's P2G (before normalisation), dense, flat index
+ * x * grid_res + y as Grid uses (grid/mod.rs:89-99).  mass: grid_res^2 floats,
+ * momentum: 2 * grid_res^2 floats (x, y interleaved).  Either may be NULL. */
+int emh_sim_get_grid(emh_sim* sim, float* mass, float* momentum, size_t n_cells);
+/* Wait for all queued device work of this handle. */
+int emh_sim_synchronize(emh_sim* sim);
+
+/* ---- per-kernel timing ---------------------------------------------------
+ * HIP events around every substep kernel on the library's stream, summed over
+ * the substeps that ran (slots the device loop skipped are not counted).  The
+ * counterpart of GpuSimulation::enable_profiling / last_pass_timings_ns
+ * (src/systems/gpu/solver/profiling.rs:9-75).  Enabling resets the totals. */
+typedef struct emh_kernel_time {
+    char     name[32];
+    uint64_t launches;
+    double   total_ms;
+} emh_kernel_time;
+int emh_sim_set_profiling(emh_sim* sim, int enable);
+int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
+
+/* ---- the cell-key sort on its own ---------------------------------------
+ * No CPU counterpart in the reference (its WGSL sort is a coarse 16x16-block
+ * counting sort with a non-deterministic order inside a block,
+ * src/systems/gpu/shaders/particle_sort.wgsl:94-245).  This is the device
+ * sort the substep uses: key = emh_cell_key(floor(x), floor(y)), stable by
+ * input index.  keys_out[i] is the key of input i; perm_out[j] is the input
+ * index at sorted position j. */
+uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);
+int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
+                       uint32_t* keys_out, uint32_t* perm_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMERGE_HIP_H */

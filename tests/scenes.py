@@ -1,0 +1,237 @@
+"""Scene builders shared by the parity tests and bench.py.
+
+Each scene is built with the same calls on either backend:
+  "gpu"       emerge_hip.Simulation (the HIP library)
+  "oracle32"  the CPU oracle in f32 (restatement of the reference's CPU path)
+  "oracle64"  the same oracle in f64
+Materials are built with each backend's own constructors (the product's C
+functions, or the oracle's restatement of the Rust constructors); a CPU test
+checks that both produce identical bytes.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable, Dict
+
+import numpy as np
+
+import emerge_hip as eh
+
+BACKENDS = ("gpu", "oracle32", "oracle64")
+
+
+class Mats:
+    """Material constructors of one backend, with the Rust names."""
+
+    def __init__(self, backend: str):
+        self.backend = backend
+        if backend != "gpu":
+            import oracle_py
+            self._o = oracle_py
+
+    def lame(self, e, nu):
+        return eh.lame_from_young(e, nu) if self.backend == "gpu" else self._o.lame_from_young(e, nu)
+
+    def radians(self, d):
+        return eh.to_radians(d) if self.backend == "gpu" else self._o.to_radians(d)
+
+    def neohookean(self, lam, mu):
+        if self.backend == "gpu":
+            return eh.NeoHookeanMaterial.new(lam, mu)
+        return self._o.material("neohookean", lam, mu)
+
+    def corotated(self, lam, mu):
+        if self.backend == "gpu":
+            return eh.CorotatedMaterial.new(lam, mu)
+        return self._o.material("corotated", lam, mu)
+
+    def fluid(self, rest_density, visc, eos_k, eos_p):
+        if self.backend == "gpu":
+            return eh.NewtonianFluidMaterial.new(rest_density, visc, eos_k, eos_p)
+        return self._o.material("fluid", rest_density, visc, eos_k, eos_p)
+
+    def fluid_low_viscosity(self, rest_density, eos_k):
+        return self.fluid(rest_density, 1.0e-3, eos_k, 7.0)  # fluid.rs:73-75
+
+    def drucker_prager(self, e, nu, friction_deg=None):
+        lam, mu = self.lame(e, nu)
+        m = eh.DruckerPragerMaterial.new(lam, mu) if self.backend == "gpu" else self._o.material(
+            "drucker_prager", lam, mu)
+        if friction_deg is not None:
+            m.params.dp_h0 = self.radians(friction_deg)
+        return m
+
+    def snow(self, e, nu):  # StomakhinMaterial::from_young_modulus  snow.rs:68-71
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.StomakhinMaterial.new(lam, mu, 10.0, 0.025, 0.0075, 0.6, 20.0)
+        return self._o.material("snow", lam, mu, 10.0, 0.025, 0.0075, 0.6, 20.0)
+
+    def nacc(self, e, nu, m_slope, beta, xi, advance_f=False):  # NaccMaterial::from_young_modulus
+        lam, mu = self.lame(e, nu)
+        kappa = float(np.float32(lam) + np.float32(mu))
+        if self.backend == "gpu":
+            m = eh.NaccMaterial.new(mu, kappa, m_slope, beta, xi, advance_f=advance_f)
+        else:
+            m = self._o.material("nacc", mu, kappa, m_slope, beta, xi)
+            m.ext.nacc_advance_f = int(bool(advance_f))
+        return m
+
+
+def new_sim(backend: str, config: eh.SimConfig):
+    if backend == "gpu":
+        return eh.Simulation.empty(config)
+    import oracle_py
+    return oracle_py.OracleSim.empty(config, 64 if backend == "oracle64" else 32)
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    steps: int               # step() calls the scene is defined over
+    build: Callable          # build(backend) -> sim
+
+
+def quickstart(backend: str):
+    """BASELINE config 1 == the crate's README quick start (README.md:20-51)."""
+    cfg = eh.SimConfig.standard(64, 0.05, (0.0, -1.0))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.neohookean(400.0, 200.0))
+           .with_material(1, m.fluid_low_viscosity(1000.0, 1e4))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    base = eh.SpawnRegion.for_sim(cfg)
+    sim.add_body(dataclasses.replace(base, box_size=(12, 12), box_center=(24.0, 40.0),
+                                     precompute_initial_volumes=True))
+    sim.add_body(dataclasses.replace(base, box_size=(12, 8), box_center=(40.0, 36.0), material_id=1,
+                                     precompute_initial_volumes=True))
+    return sim
+
+
+def quickstart_impact(backend: str):
+    """The quick-start bodies dropped close to the floor under g = 10 so that
+    the 30 steps include impact, rebound and fluid spreading."""
+    cfg = eh.SimConfig.standard(64, 0.05, (0.0, -10.0))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.neohookean(400.0, 200.0))
+           .with_material(1, m.fluid_low_viscosity(1.0, 1e2))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    base = eh.SpawnRegion.for_sim(cfg)
+    sim.add_body(dataclasses.replace(base, box_size=(12, 12), box_center=(20.0, 14.0)))
+    sim.add_body(dataclasses.replace(base, box_size=(12, 8), box_center=(42.0, 10.0), material_id=1,
+                                     spacing=0.5))
+    return sim
+
+
+def sand_column(backend: str, res=64, width=16, height=24, spacing=0.5, dt=0.02, seed=7, g=-9.81):
+    """BASELINE config 2 in miniature: Drucker-Prager sand (E = 3.5e5, nu = 0.3,
+    friction 30 deg) column on the floor, jittered by +/-0.25 cell."""
+    cfg = eh.SimConfig.standard(res, dt, (0.0, g))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.drucker_prager(3.5e5, 0.3, friction_deg=30.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    cx = res * 0.5
+    sim.add_body(eh.SpawnRegion(spacing=spacing, box_size=(width, height), box_center=(cx, 2.0 + height * 0.5),
+                                position_jitter=0.25 / spacing, rng_seed=seed))
+    return sim
+
+
+def snow_drop(backend: str):
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.snow(1.4e5, 0.2))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(14, 10), box_center=(32.0, 10.0), position_jitter=0.3,
+                                rng_seed=3, initial_velocity_scale=2.0))
+    return sim
+
+
+def corotated_drop(backend: str):
+    cfg = eh.SimConfig.standard(48, 0.05, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.corotated(400.0, 200.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(10, 10), box_center=(24.0, 9.0), rng_seed=5,
+                                position_jitter=0.2))
+    return sim
+
+
+def nacc_block(backend: str, advance_f=True):
+    """NACC soil (E = 2e5, nu = 0.3, M = 1.85, beta = 0.5, xi = 0.8) block landing."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.nacc(2e5, 0.3, 1.85, 0.5, 0.8, advance_f=advance_f))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(12, 8), box_center=(32.0, 7.0), rng_seed=11,
+                                position_jitter=0.3, initial_velocity_scale=4.0))
+    return sim
+
+
+def multi_material(backend: str):
+    """BASELINE config 3 in miniature: NeoHookean, water, snow and corotated
+    boxes with N(0, 0.5)-like random velocities, one grid."""
+    cfg = eh.SimConfig.standard(96, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.neohookean(400.0, 200.0))
+           .with_material(1, m.fluid_low_viscosity(4.0, 1e2))
+           .with_material(2, m.snow(1.4e5, 0.2))
+           .with_material(3, m.corotated(400.0, 200.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    rng = np.random.default_rng(1234)
+    for k in range(8):
+        w, h = (int(v) for v in rng.integers(4, 14, size=2))
+        cx = float(rng.uniform(4 + w / 2 + 1, 92 - w / 2 - 1))
+        cy = float(rng.uniform(4 + h / 2 + 1, 60 - h / 2))
+        sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(w, h), box_center=(cx, cy), material_id=k % 4,
+                                    rng_seed=100 + k, initial_velocity_scale=1.0, position_jitter=0.1))
+    return sim
+
+
+SCENES: Dict[str, Scene] = {
+    "quickstart": Scene("quickstart", 60, quickstart),
+    "quickstart_impact": Scene("quickstart_impact", 30, quickstart_impact),
+    "sand_column": Scene("sand_column", 10, sand_column),
+    "snow_drop": Scene("snow_drop", 10, snow_drop),
+    "corotated_drop": Scene("corotated_drop", 20, corotated_drop),
+    "nacc_block": Scene("nacc_block", 10, nacc_block),
+    "multi_material": Scene("multi_material", 10, multi_material),
+}
+
+
+def state_matrix(p: np.ndarray) -> Dict[str, np.ndarray]:
+    """Per-particle fields compared by the parity tests, as float64."""
+    return {
+        "x": p["x"].astype(np.float64),
+        "v": p["v"].astype(np.float64),
+        "C": p["velocity_gradient"].astype(np.float64),
+        "F": p["deformation_gradient"].astype(np.float64),
+        "Jp": p["plastic_volume_ratio"].astype(np.float64),
+        "q": p["friction_hardening"].astype(np.float64),
+        "lvs": p["log_volume_strain"].astype(np.float64),
+        "volume": p["volume"].astype(np.float64),
+        "density": p["density"].astype(np.float64),
+    }
+
+
+def f64_state(a: np.ndarray) -> Dict[str, np.ndarray]:
+    """Same fields from the oracle's f64 export (20 doubles per particle)."""
+    return {"x": a[:, 0:2], "v": a[:, 2:4], "C": a[:, 4:8], "F": a[:, 8:12], "Jp": a[:, 16], "q": a[:, 18],
+            "lvs": a[:, 19], "volume": a[:, 14], "density": a[:, 15]}
+
+
+def rel_errors(a: Dict[str, np.ndarray], ref: Dict[str, np.ndarray]) -> Dict[str, float]:
+    """max |a - ref| / max(max |ref|, floor) per field (a field's own scale)."""
+    floors = {"x": 1.0, "v": 1e-3, "C": 1e-3, "F": 1.0, "Jp": 1.0, "q": 1e-3, "lvs": 1e-3, "volume": 1e-6,
+              "density": 1e-6}
+    out = {}
+    for k, r in ref.items():
+        d = np.max(np.abs(np.asarray(a[k], np.float64) - r)) if r.size else 0.0
+        s = max(float(np.max(np.abs(r))) if r.size else 0.0, floors[k])
+        out[k] = float(d / s)
+    return out
