@@ -116,7 +116,7 @@ EMH_DEV float rclamp(float x, float lo, float hi) {  // Rust f32::clamp (NaN pas
     if (x > hi) x = hi;
     return x;
 }
-EMH_DEV float len2(float x, float y) { return __fsqrt_rn(x * x + y * y); }
+EMH_DEV float len2(float x, float y) { return sqrtf(x * x + y * y); }
 
 // ------------------------------------------------------------ B-spline weights
 // grid/kernel.rs:10-25.  Base cell = floor(x); d = (x - base) - 0.5.
@@ -159,7 +159,7 @@ EMH_DEV void svd2(const M2& f, M2& u, float2& sig, M2& vt) {
     float c11 = f.c * f.c + f.d * f.d;
     float mean = (c00 + c11) * 0.5f;
     float hd = (c00 - c11) * 0.5f;
-    float disc = __fsqrt_rn(hd * hd + c01 * c01);
+    float disc = sqrtf(hd * hd + c01 * c01);
     float l1 = mean + disc, l2 = mean - disc;
     float v1x, v1y;
     if (fabsf(c01) > 1e-10f * rmax(c00 + c11, 1e-30f)) {
@@ -176,7 +176,7 @@ EMH_DEV void svd2(const M2& f, M2& u, float2& sig, M2& vt) {
     }
     float v2x = -v1y, v2y = v1x;
     vt = M2{v1x, v2x, v1y, v2y};  // transpose of [v1 | v2]
-    float s1 = __fsqrt_rn(rmax(l1, 0.0f)), s2 = __fsqrt_rn(rmax(l2, 0.0f));
+    float s1 = sqrtf(rmax(l1, 0.0f)), s2 = sqrtf(rmax(l2, 0.0f));
     float2 fv1 = mvec(f, v1x, v1y), fv2 = mvec(f, v2x, v2y);
     float2 u1 = s1 > 1e-10f ? make_float2(fv1.x / s1, fv1.y / s1) : orth_complement(fv2.x, fv2.y);
     float2 u2 = s2 > 1e-10f ? make_float2(fv2.x / s2, fv2.y / s2) : orth_complement(u1.x, u1.y);
@@ -195,7 +195,7 @@ EMH_DEV M2 usv(const M2& u, float s1, float s2, const M2& vt) { return mmul(mmul
 EMH_DEV M2 polar_r(const M2& f) {
     float x = f.a + f.d;
     float y = f.b - f.c;
-    float n = __fsqrt_rn(x * x + y * y);
+    float n = sqrtf(x * x + y * y);
     if (n > kF32Eps) return M2{x / n, y / n, -y / n, x / n};
     return M2{1.0f, 0.0f, 0.0f, 1.0f};
 }
@@ -206,7 +206,7 @@ EMH_DEV float elastic_wave_dt(float lambda, float mu, float h, float density, fl
     float rho = rmax(density, min_density);
     float modulus = rmax((lambda + 2.0f * mu) * h, 0.0f);
     if (modulus <= kF32Eps) return kInf;
-    float c = __fsqrt_rn(modulus / rho);
+    float c = sqrtf(modulus / rho);
     if (c <= kF32Eps) return kInf;
     return mcfl * cell / c;
 }
@@ -246,7 +246,7 @@ EMH_DEV bool project_state(PState& s, const DevConfig& cfg) {
     } else {
         float j = mdet(s.f);
         if (j > cfg.j_max) {
-            s.f = mscale(s.f, __fsqrt_rn(cfg.j_max / j));
+            s.f = mscale(s.f, sqrtf(cfg.j_max / j));
             p = true;
         }
     }
@@ -410,7 +410,7 @@ EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevC
             float ratio = rmax(d / m.fl_c2_denom, 1.0e-6f);
             float out = kInf;
             float c2 = m.eos_k * m.eos_p * powf(ratio, m.eos_p - 1.0f) / m.fl_c2_denom;
-            if (__builtin_isfinite(c2) && c2 > kF32Eps) out = rmin(out, cfg.material_cfl * cfg.cell / __fsqrt_rn(c2));
+            if (__builtin_isfinite(c2) && c2 > kF32Eps) out = rmin(out, cfg.material_cfl * cfg.cell / sqrtf(c2));
             if (m.dyn_visc > 0.0f) {
                 float kin = m.dyn_visc / d;
                 if (kin > kF32Eps) out = rmin(out, cfg.viscous_cfl * cfg.cell * cfg.cell / kin);
@@ -467,7 +467,7 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
     float psi = (kappa * 0.5f) * (jtr - 1.0f / jtr);
     float ptr = -psi * jtr;
     if (ptr > p0) {
-        float jn = __fsqrt_rn(rmax(-2.0f * p0 / kappa + 1.0f, 1.0e-8f));
+        float jn = sqrtf(rmax(-2.0f * p0 / kappa + 1.0f, 1.0e-8f));
         float sv = powf(jn, 0.5f);
         if (hard) alpha = alpha + logf(jtr / jn);
         fo = usv(u, sv, sv, vt);
@@ -475,7 +475,7 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
         return;
     }
     if (ptr < -beta * p0) {
-        float jn = __fsqrt_rn(rmax(2.0f * beta * p0 / kappa + 1.0f, 1.0e-8f));
+        float jn = sqrtf(rmax(2.0f * beta * p0 / kappa + 1.0f, 1.0e-8f));
         float sv = powf(jn, 0.5f);
         if (hard) alpha = alpha + logf(jtr / jn);
         fo = usv(u, sv, sv, vt);
@@ -493,7 +493,7 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
     }
     if (hard && p0 > 1.0e-4f && ptr < p0 - 1.0e-4f && ptr > -beta * p0 + 1.0e-4f) {
         float pc = (1.0f - beta) * p0 * 0.5f;
-        float qtr = __fsqrt_rn(2.0f) * len2(st1, st2);
+        float qtr = sqrtf(2.0f) * len2(st1, st2);
         float dx = pc - ptr, dy = -qtr;
         float rcp = 1.0f / len2(dx, dy);
         if (__builtin_isfinite(rcp) && rcp > 0.0f) {
@@ -506,16 +506,16 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
         float c = mm * mm * (pc + beta * p0) * (pc - p0);
         float b = mm * mm * dx * (2.0f * pc - p0 + beta * p0);
         float a = mm * mm * dx * dx + (1.0f + 2.0f * beta) * dy * dy;
-        float disc = __fsqrt_rn(rmax(b * b - 4.0f * a * c, 0.0f));
+        float disc = sqrtf(rmax(b * b - 4.0f * a * c, 0.0f));
         float l1 = (-b + disc) / (2.0f * a);
         float l2 = (-b - disc) / (2.0f * a);
         float p1 = pc + l1 * dx;
         float p2 = pc + l2 * dx;
         float px = ((ptr - pc) * (p1 - pc) > 0.0f) ? p1 : p2;
-        float jex = __fsqrt_rn(rmax(fabsf(-2.0f * px / kappa + 1.0f), 1.0e-8f));
+        float jex = sqrtf(rmax(fabsf(-2.0f * px / kappa + 1.0f), 1.0e-8f));
         if (jex > 1.0e-4f) alpha = alpha + logf(jtr / jex);
     }
-    float sc = __fsqrt_rn(rmax(-y1 / rmax(y0, 1.0e-10f), 0.0f)) * (jtr / m.mu);
+    float sc = sqrtf(rmax(-y1 / rmax(y0, 1.0e-10f), 0.0f)) * (jtr / m.mu);
     float nx = st1, ny = st2;
     float rcp = 1.0f / len2(nx, ny);
     if (__builtin_isfinite(rcp) && rcp > 0.0f) {
@@ -527,7 +527,7 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
     }
     float b1 = sc * nx + qt / 2.0f;
     float b2 = sc * ny + qt / 2.0f;
-    fo = usv(u, __fsqrt_rn(rmax(b1, 1.0e-8f)), __fsqrt_rn(rmax(b2, 1.0e-8f)), vt);
+    fo = usv(u, sqrtf(rmax(b1, 1.0e-8f)), sqrtf(rmax(b2, 1.0e-8f)), vt);
     ao = alpha;
 }
 
@@ -572,7 +572,7 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
             n2 = fabsf(n2);
             float jn = n1 * n2;
             if (jn < m.dp_min_vj) {
-                float k = __fsqrt_rn(m.dp_min_vj / rmax(jn, 1e-6f));
+                float k = sqrtf(m.dp_min_vj / rmax(jn, 1e-6f));
                 n1 = n1 * k;
                 n2 = n2 * k;
             }
@@ -595,7 +595,7 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
         }
         case kModelFluid: {
             float j = rclamp(mdet(s.f), 0.5f, 2.0f);
-            float r = __fsqrt_rn(j);
+            float r = sqrtf(j);
             s.f = M2{r, 0.0f, 0.0f, r};
             if (m.settling > 0.0f) {
                 float k = 1.0f - rmin(m.settling * dt, 0.5f);
@@ -626,7 +626,7 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
 // affine_cfl_speed_contribution  solver/step.rs:481-488
 EMH_DEV float affine_cfl(const M2& c, float cell) {
     const float kCorner = 1.5f * 1.41421356237309504880f;
-    float g = __fsqrt_rn((c.a * c.a + c.b * c.b) + (c.c * c.c + c.d * c.d));
+    float g = sqrtf((c.a * c.a + c.b * c.b) + (c.c * c.c + c.d * c.d));
     return g * kCorner * cell;
 }
 

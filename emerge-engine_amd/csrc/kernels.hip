@@ -350,6 +350,83 @@ __global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
         p.sort.cell_start[p.cfg.n_bins] = p.sort.tile_start[gridDim.x];
 }
 
+// ---------------------------------------------------- wave segmented sums
+// Lanes of a wave hold consecutive particles of a cell-sorted chunk, so runs
+// of lanes share a base cell and with it all nine stencil cells.  The run's
+// 27 contributions are summed across its lanes with DPP row shifts and row
+// broadcasts (VALU work, no LDS traffic), leaving each run's total in its
+// last lane, which alone issues the atomics.
+struct SegMask {
+    uint64_t heads;  // bit L: lane L starts a run
+    uint32_t lane;
+};
+// Lanes in (src, lane] contain no run head <=> lane src is in this lane's run.
+__device__ __forceinline__ bool same_run(const SegMask& m, int src) {
+    if (src < 0) return false;
+    const uint64_t upto_lane = (m.lane == 63) ? ~0ull : ((2ull << m.lane) - 1ull);
+    const uint64_t upto_src = (2ull << (uint32_t)src) - 1ull;
+    return (m.heads & (upto_lane & ~upto_src)) == 0ull;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_take(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, true));
+}
+// Six-step segmented inclusive scan over the wave (row_shr 1/2/4/8 inside
+// 16-lane rows, then row_bcast 15 and 31 across rows).
+template <int N>
+__device__ __forceinline__ void seg_scan(float (&v)[N], const SegMask& m) {
+    const int l = (int)m.lane, r = l & 15;
+    const bool p1 = r >= 1 && same_run(m, l - 1);
+    const bool p2 = r >= 2 && same_run(m, l - 2);
+    const bool p4 = r >= 4 && same_run(m, l - 4);
+    const bool p8 = r >= 8 && same_run(m, l - 8);
+    const bool pb15 = (l >= 16) && same_run(m, (l & ~15) - 1) && ((l >> 4) == 1 || (l >> 4) == 3);
+    const bool pb31 = (l >= 32) && same_run(m, 31);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x111, 0xf>(v[i]);  // row_shr:1
+        v[i] = v[i] + (p1 ? t : 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x112, 0xf>(v[i]);  // row_shr:2
+        v[i] = v[i] + (p2 ? t : 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x114, 0xf>(v[i]);  // row_shr:4
+        v[i] = v[i] + (p4 ? t : 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x118, 0xf>(v[i]);  // row_shr:8
+        v[i] = v[i] + (p8 ? t : 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x142, 0xa>(v[i]);  // row_bcast:15 into rows 1 and 3
+        v[i] = v[i] + (pb15 ? t : 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        float t = dpp_take<0x143, 0xc>(v[i]);  // row_bcast:31 into rows 2 and 3
+        v[i] = v[i] + (pb31 ? t : 0.0f);
+    }
+}
+// Run structure of the base cells (bx, by) across the wave; every lane must
+// call.  Invalid lanes pass a cell no valid lane can have (bx = INT_MIN + lane).
+__device__ __forceinline__ SegMask run_mask(int bx, int by) {
+    SegMask m;
+    m.lane = lane_id();
+    const int pbx = __shfl_up(bx, 1, 64), pby = __shfl_up(by, 1, 64);
+    const bool head = (m.lane == 0) || pbx != bx || pby != by;
+    m.heads = __ballot(head);
+    return m;
+}
+__device__ __forceinline__ bool run_tail(const SegMask& m) {
+    return m.lane == 63 || ((m.heads >> (m.lane + 1)) & 1ull);
+}
+
 // Chunk bounding box of base cells (positions projected) -> LDS box origin/size.
 struct Box {
     int ox, oy, w, h;
@@ -415,21 +492,38 @@ __global__ __launch_bounds__(kBlock) void k_density_scatter(const Pipeline p) {
         __syncthreads();
     }
     for (int k = 0; k < kChunk / kBlock; ++k) {
-        uint32_t j = base + k * kBlock + threadIdx.x;
-        if (j >= p.n) continue;
-        PState s = load_state(a, j);
-        if (p.cfg.project) project_state(s, p.cfg);
-        Wts w = weights(s.x, s.y);
-        for (int gx = 0; gx < 3; ++gx)
-            for (int gy = 0; gy < 3; ++gy) {
-                float wm = (w.wx[gx] * w.wy[gy]) * s.mass;
-                int cx = w.bx + gx - 1, cy = w.by + gy - 1;
-                if (bx.lds) {
-                    atomicAdd(&s_m[(cx - bx.ox) * bx.h + (cy - bx.oy)], wm);
-                } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
-                    gatomic_add(&p.grid.dmass[(uint32_t)cx * R + (uint32_t)cy], wm);
+        const uint32_t j = base + k * kBlock + threadIdx.x;
+        const bool valid = j < p.n;
+        float vals[9];
+        int bxc = INT_MIN + (int)lane_id(), byc = 0;
+        if (valid) {
+            PState s = load_state(a, j);
+            if (p.cfg.project) project_state(s, p.cfg);
+            Wts w = weights(s.x, s.y);
+            bxc = w.bx;
+            byc = w.by;
+#pragma unroll
+            for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+                for (int gy = 0; gy < 3; ++gy) vals[gx * 3 + gy] = (w.wx[gx] * w.wy[gy]) * s.mass;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) vals[q] = 0.0f;
+        }
+        const SegMask sm = run_mask(bxc, byc);
+        seg_scan(vals, sm);
+        if (valid && run_tail(sm)) {
+            for (int gx = 0; gx < 3; ++gx)
+                for (int gy = 0; gy < 3; ++gy) {
+                    const float wm = vals[gx * 3 + gy];
+                    const int cx = bxc + gx - 1, cy = byc + gy - 1;
+                    if (bx.lds) {
+                        atomicAdd(&s_m[(cx - bx.ox) * bx.h + (cy - bx.oy)], wm);
+                    } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
+                        gatomic_add(&p.grid.dmass[(uint32_t)cx * R + (uint32_t)cy], wm);
+                    }
                 }
-            }
+        }
     }
     if (bx.lds) {
         __syncthreads();
@@ -513,39 +607,59 @@ __global__ __launch_bounds__(kBlock) void k_p2g(const Pipeline p) {
         __syncthreads();
     }
     for (int k = 0; k < kChunk / kBlock; ++k) {
-        uint32_t j = base + k * kBlock + threadIdx.x;
-        if (j >= p.n) continue;
-        PState s = load_state(a, j);
-        if (PROJECT && p.cfg.project) project_state(s, p.cfg);
-        const DevMat& mt = mat_of(p, s.mat);
-        M2 tau = kirchhoff(mt, s);
-        float coeff = ((-stress_volume(mt, s)) * kDInv) * dt;
-        Wts w = weights(s.x, s.y);
-        for (int gx = 0; gx < 3; ++gx)
-            for (int gy = 0; gy < 3; ++gy) {
-                float weight = w.wx[gx] * w.wy[gy];
-                int cx = w.bx + gx - 1, cy = w.by + gy - 1;
-                float dx = ((float)cx - s.x) + 0.5f;
-                float dy = ((float)cy - s.y) + 0.5f;
-                float vcx = s.vx + (s.c.a * dx + s.c.c * dy);
-                float vcy = s.vy + (s.c.b * dx + s.c.d * dy);
-                float sdx = tau.a * dx + tau.c * dy;
-                float sdy = tau.b * dx + tau.d * dy;
-                float mx = weight * (s.mass * vcx + coeff * sdx);
-                float my = weight * (s.mass * vcy + coeff * sdy);
-                float mm = weight * s.mass;
-                if (bx.lds) {
-                    int li = (cx - bx.ox) * bx.h + (cy - bx.oy);
-                    atomicAdd(&s_m[li], mm);
-                    atomicAdd(&s_mx[li], mx);
-                    atomicAdd(&s_my[li], my);
-                } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
-                    uint32_t gi = (uint32_t)cx * R + (uint32_t)cy;
-                    gatomic_add(&gm[gi], mm);
-                    gatomic_add(&gmx[gi], mx);
-                    gatomic_add(&gmy[gi], my);
+        const uint32_t j = base + k * kBlock + threadIdx.x;
+        const bool valid = j < p.n;
+        // contributions to the 3x3 stencil: mass, momentum x, momentum y
+        float vals[27];
+        int bxc = INT_MIN + (int)lane_id(), byc = 0;
+        if (valid) {
+            PState s = load_state(a, j);
+            if (PROJECT && p.cfg.project) project_state(s, p.cfg);
+            const DevMat& mt = mat_of(p, s.mat);
+            const M2 tau = kirchhoff(mt, s);
+            const float coeff = ((-stress_volume(mt, s)) * kDInv) * dt;
+            const Wts w = weights(s.x, s.y);
+            bxc = w.bx;
+            byc = w.by;
+#pragma unroll
+            for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+                for (int gy = 0; gy < 3; ++gy) {
+                    const float weight = w.wx[gx] * w.wy[gy];
+                    const float dx = ((float)(w.bx + gx - 1) - s.x) + 0.5f;
+                    const float dy = ((float)(w.by + gy - 1) - s.y) + 0.5f;
+                    const float vcx = s.vx + (s.c.a * dx + s.c.c * dy);
+                    const float vcy = s.vy + (s.c.b * dx + s.c.d * dy);
+                    const float sdx = tau.a * dx + tau.c * dy;
+                    const float sdy = tau.b * dx + tau.d * dy;
+                    vals[gx * 3 + gy] = weight * s.mass;
+                    vals[9 + gx * 3 + gy] = weight * (s.mass * vcx + coeff * sdx);
+                    vals[18 + gx * 3 + gy] = weight * (s.mass * vcy + coeff * sdy);
                 }
-            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) vals[q] = 0.0f;
+        }
+        const SegMask sm = run_mask(bxc, byc);
+        seg_scan(vals, sm);
+        if (valid && run_tail(sm)) {
+            for (int gx = 0; gx < 3; ++gx)
+                for (int gy = 0; gy < 3; ++gy) {
+                    const int cx = bxc + gx - 1, cy = byc + gy - 1;
+                    const float mm = vals[gx * 3 + gy], mx = vals[9 + gx * 3 + gy], my = vals[18 + gx * 3 + gy];
+                    if (bx.lds) {
+                        const int li = (cx - bx.ox) * bx.h + (cy - bx.oy);
+                        atomicAdd(&s_m[li], mm);
+                        atomicAdd(&s_mx[li], mx);
+                        atomicAdd(&s_my[li], my);
+                    } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
+                        const uint32_t gi = (uint32_t)cx * R + (uint32_t)cy;
+                        gatomic_add(&gm[gi], mm);
+                        gatomic_add(&gmx[gi], mx);
+                        gatomic_add(&gmy[gi], my);
+                    }
+                }
+        }
     }
     if (bx.lds) {
         __syncthreads();

@@ -5,7 +5,7 @@ import sys
 import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for sub in ("emerge-engine_amd", "oracle", "tests"):
+for sub in ("", "emerge-engine_amd", "oracle", "tests"):
     p = os.path.join(REPO, sub)
     if p not in sys.path:
         sys.path.insert(0, p)
