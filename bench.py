@@ -32,15 +32,14 @@ import emerge_hip as eh  # noqa: E402
 METRIC = "particle-steps/sec (1M sand, 512^2)"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
-# Algorithmic bytes per particle for each kernel (Drucker-Prager, f32 SoA,
-# 4-byte material id): compulsory reads/writes of particle state only.
-ALGO_BYTES = {
-    # x, v, C, F, mass, V0, material
-    "k_p2g": 8 + 8 + 16 + 16 + 4 + 4 + 4,
-    # read x, F, mass, V0, material, q, lvs; write x, v, C, F, q, lvs, volume, density
-    "k_g2p": (8 + 16 + 4 + 4 + 4 + 4 + 4) + (8 + 8 + 16 + 16 + 4 + 4 + 4 + 4),
-}
-SUBSTEP_BYTES = 136  # whole substep (reads 72 B + writes 64 B per particle)
+# Algorithmic bytes per particle-substep (Drucker-Prager, f32 SoA, 4-byte
+# material id): compulsory reads x 8, v 8, C 16, F 16, mass 4, V0 4,
+# material 4, density 4, q 4, log-volume-strain 4 (72 B) and writes x, v, C,
+# F, q, log-volume-strain, volume, density (64 B).  k_substep carries the
+# whole substep's particle traffic (G2P + update + the next P2G), so one
+# launch moves 136 B per particle algorithmically.
+SUBSTEP_BYTES = 136
+ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 
 
 def build_workload(backend: str = "gpu"):
@@ -71,7 +70,7 @@ def cpu_baseline(target_s: float, first_dt: float) -> dict:
     sim = build_workload("oracle")
     n = sim.num_particles()
     t1 = sim.time_substeps(1, first_dt)
-    k = max(1, min(50, int(target_s / max(t1, 1e-6))))
+    k = max(1, min(400, int(target_s / max(t1, 1e-6))))
     t = sim.time_substeps(k, first_dt)
     return {"value": n * k / t, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
             "sample": f"{n} particles x {k} substeps (dt {first_dt:.3e}) of the same workload, after 1 untimed "

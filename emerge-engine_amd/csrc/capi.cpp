@@ -289,6 +289,8 @@ struct emh_sim {
     bool device_ahead = false;  // device stepped since the last download
     emh_step_stats stats{};
     uint32_t predicted_substeps = 1;
+    uint32_t substeps_since_reorder = 0;
+    uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
 
     // per-kernel event timing
     bool profiling = false;
@@ -375,17 +377,13 @@ struct emh_sim {
         HIP_TRY(hipGetDevice(&device));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         const size_t R2 = (size_t)res() * res();
-        // grid: m[2], mx[2], my[2], dmass (7 R^2 floats) + vel (2 R^2 floats)
-        HIP_TRY(hipMalloc(&grid_block, sizeof(float) * R2 * 9));
+        // grid: three sets of 5 P2G accumulators + three density-pass mass grids
+        HIP_TRY(hipMalloc(&grid_block, sizeof(float) * R2 * 3 * (kAccChannels + 1)));
         float* g = (float*)grid_block;
-        pipe.grid.m[0] = g + 0 * R2;
-        pipe.grid.m[1] = g + 1 * R2;
-        pipe.grid.mx[0] = g + 2 * R2;
-        pipe.grid.mx[1] = g + 3 * R2;
-        pipe.grid.my[0] = g + 4 * R2;
-        pipe.grid.my[1] = g + 5 * R2;
-        pipe.grid.dmass = g + 6 * R2;
-        pipe.grid.vel = (float2*)(g + 7 * R2);
+        for (int set = 0; set < 3; ++set) {
+            for (int ch = 0; ch < kAccChannels; ++ch) pipe.grid.acc[set][ch] = g + (size_t)(set * kAccChannels + ch) * R2;
+            pipe.grid.dm[set] = g + (size_t)(3 * kAccChannels + set) * R2;
+        }
         d_cell_count = dalloc<uint32_t>(n_bins());
         d_cell_start = dalloc<uint32_t>(n_bins() + 1);
         d_tile_count = dalloc<uint32_t>(n_tiles());
@@ -430,11 +428,9 @@ struct emh_sim {
             pb.lvs = (float*)take(sizeof(float) * c);
             pb.mat = (uint32_t*)take(sizeof(uint32_t) * c);
             pb.id = (uint32_t*)take(sizeof(uint32_t) * c);
-            pb.key = (uint32_t*)take(sizeof(uint32_t) * c);
-            pb.rank = (uint32_t*)take(sizeof(uint32_t) * c);
         }
-        pipe.sort.perm = (uint32_t*)take(sizeof(uint32_t) * c);
-        pipe.sort.cfl_partial = (float2*)take(sizeof(float2) * ((c + kBlock - 1) / kBlock));
+        pipe.sort.key = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.rank = (uint32_t*)take(sizeof(uint32_t) * c);
         for (void* p : particle_allocs)
             if (!p) {
                 release_particles();
@@ -484,6 +480,32 @@ struct emh_sim {
             if (m.model == EMH_MODEL_FLUID) dens = true;  // needs_density_recompute  fluid.rs:261-263
         d.density_pass = dens ? 1u : 0u;
         pipe.n_mats = (uint32_t)mats.size();
+        // optional fields: what the materials the particles resolve to use
+        std::vector<bool> used(mats.size(), false);
+        for (const auto& p : host) {
+            const uint32_t m = p.material_id > kMatMask ? kMatMask : p.material_id;
+            used[m < mats.size() ? m : 0u] = true;
+        }
+        uint32_t fields = 0, only = kModelMixed;
+        bool first = true;
+        for (size_t i = 0; i < mats.size(); ++i) {
+            if (!used[i]) continue;
+            if (first)
+                only = mats[i].model;
+            else if (only != mats[i].model)
+                only = kModelMixed;
+            first = false;
+            switch (mats[i].model) {
+                case EMH_MODEL_NEOHOOKEAN: fields |= kFQ; break;
+                case EMH_MODEL_DRUCKER_PRAGER: fields |= kFQ | kFLvs; break;
+                case EMH_MODEL_COROTATED: fields |= kFH; break;
+                case EMH_MODEL_SNOW: fields |= kFJp | kFH; break;
+                case EMH_MODEL_NACC: fields |= kFLvs | kFH | kFVolDens; break;
+                default: fields |= kFVolDens; break;  // fluid, fallback: volume/density carried over
+            }
+        }
+        d.fields = fields;
+        d.only_model = only;
     }
 
     int upload_materials() {
@@ -551,15 +573,20 @@ struct emh_sim {
         HIP_TRY(put(b.id, id.data(), sizeof(uint32_t) * n));
         Ctrl ctrl;
         std::memset(&ctrl, 0, sizeof(ctrl));
-        ctrl.total_substeps = stats.total_substeps;
-        ctrl.last_dt = stats.last_step_dt;
-        ctrl.cur_buf = 0;
-        ctrl.cur_grid = 0;
-        ctrl.n_cfl_blocks = (n + kBlock - 1) / kBlock;
+        ctrl.st[0].total_substeps = stats.total_substeps;
+        ctrl.st[0].last_dt = stats.last_step_dt;
+        ctrl.st[0].dt = stats.last_step_dt;
+        for (auto& r : ctrl.red) {
+            r.max_speed = 0u;
+            r.min_mdt = kInfBits;
+            r.dirty = 0u;
+        }
+        seq = 0;
         HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
-        HIP_TRY(launch_zero_counters(pipe, stream));
-        HIP_TRY(launch_prologue(pipe, stream));
+        HIP_TRY(launch_zero_all(pipe, stream));
+        HIP_TRY(launch_prologue(pipe, seq++, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        substeps_since_reorder = kReorderSubsteps;  // first step puts the upload in cell-key order
         host_dirty = false;
         device_ahead = false;
         return EMH_OK;
@@ -572,7 +599,7 @@ struct emh_sim {
         Ctrl ctrl;
         HIP_TRY(hipMemcpyAsync(&ctrl, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        const PBuf& b = pipe.buf[ctrl.cur_buf & 1u];
+        const PBuf& b = pipe.buf[ctrl.st[seq & 1u].cur_buf & 1u];
         std::vector<float2> x(n), v(n);
         std::vector<float4> c(n), f(n);
         std::vector<float> mass(n), v0(n), vol(n), dens(n), jp(n), h(n), q(n), lvs(n);
@@ -656,20 +683,27 @@ struct emh_sim {
             if (rc) return rc;
         }
         for (uint32_t s = 0; s < n_steps; ++s) {
-            HIP_TRY(launch_begin_step(pipe, stream));
+            const bool reorder = substeps_since_reorder >= kReorderSubsteps;
+            if (profiling)
+                for (auto& se : slot_events)
+                    for (int k = 0; k < kKernelCount; ++k) se.used[k] = false;
+            HIP_TRY(launch_begin_step(pipe, seq++, reorder, stream, slot_ev(0)));
+            if (reorder) substeps_since_reorder = 0;
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, stream, slot_ev(k)));
-            Ctrl ctrl;
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, slot_ev(k)));
+            Ctrl c;
+            CtrlState ctrl;
             uint32_t launched = slots;
             while (true) {
-                HIP_TRY(hipMemcpyAsync(&ctrl, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipMemcpyAsync(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
                 HIP_TRY(hipStreamSynchronize(stream));
-                bool more = ctrl.remaining > kF32Eps && ctrl.substeps < cfg.max_substeps_per_step;
-                if (!more) break;
+                ctrl = c.st[seq & 1u];  // the state the next launch would decide from
+                // while remaining > f32::EPSILON && substeps < max_substeps_per_step  (step.rs:34)
+                if (!(ctrl.remaining > kF32Eps && ctrl.substeps < cfg.max_substeps_per_step)) break;
                 if (launched > cfg.max_substeps_per_step + 1)
                     return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
-                HIP_TRY(launch_substep(pipe, stream, slot_ev(launched)));
+                HIP_TRY(launch_substep(pipe, seq++, stream, slot_ev(launched)));
                 ++launched;
             }
             if (profiling) {
@@ -677,11 +711,12 @@ struct emh_sim {
                 if (rc) return rc;
             }
             predicted_substeps = ctrl.substeps;
+            substeps_since_reorder += ctrl.substeps;
             stats.last_substeps = ctrl.substeps;
             stats.last_step_dt = ctrl.last_dt;
             stats.last_sim_time_dropped = std::fmax(ctrl.remaining, 0.0f);
-            stats.last_vel_clamp_count = ctrl.clamp_count;
-            stats.last_j_projection_count = ctrl.proj_count;
+            stats.last_vel_clamp_count = c.clamp_count;
+            stats.last_j_projection_count = c.proj_count;
             stats.total_substeps = ctrl.total_substeps;
             stats.frame_index++;
         }
@@ -711,7 +746,7 @@ struct emh_sim {
             hipError_t e = hipMemcpyAsync(dx, x.data(), sizeof(float2) * n, hipMemcpyHostToDevice, stream);
             if (e == hipSuccess) e = hipMemcpyAsync(dm, m.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream);
             if (e == hipSuccess)
-                e = estimate_volumes_device(dx, dm, n, from, res(), use_box, box, pipe.grid.dmass, dd, dv, dw, stream);
+                e = estimate_volumes_device(dx, dm, n, from, res(), use_box, box, pipe.grid.dm[0], dd, dv, dw, stream);
             if (e == hipSuccess) e = hipMemcpyAsync(d.data(), dd, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipMemcpyAsync(v.data(), dv, sizeof(float) * n, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipMemcpyAsync(w.data(), dw, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream);
@@ -719,7 +754,7 @@ struct emh_sim {
             if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "volume estimate: %s", hipGetErrorString(e));
             // the dense mass grid doubles as the substep's density-pass buffer: clear it again
             if (rc == EMH_OK) {
-                e = hipMemsetAsync(pipe.grid.dmass, 0, sizeof(float) * (size_t)res() * res(), stream);
+                e = hipMemsetAsync(pipe.grid.dm[0], 0, sizeof(float) * (size_t)res() * res(), stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(stream);
                 if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "volume estimate clear: %s", hipGetErrorString(e));
             }
@@ -1117,15 +1152,21 @@ int emh_sim_get_grid(emh_sim* s, float* mass, float* momentum, size_t n_cells) {
     Ctrl ctrl;
     HIP_TRY(hipMemcpyAsync(&ctrl, s->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    const uint32_t g = ctrl.cur_grid & 1u;
-    if (mass) HIP_TRY(hipMemcpy(mass, s->pipe.grid.m[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
+    // the accumulators the last substep's grid update read; momentum as that
+    // update formed it: m v + m C d plus dt times the stress moment
+    const CtrlState st = ctrl.st[s->seq & 1u];
+    const uint32_t g = st.cur_grid % 3u;
+    if (mass) HIP_TRY(hipMemcpy(mass, s->pipe.grid.acc[g][kAccM], sizeof(float) * R2, hipMemcpyDeviceToHost));
     if (momentum) {
-        std::vector<float> mx(R2), my(R2);
-        HIP_TRY(hipMemcpy(mx.data(), s->pipe.grid.mx[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(my.data(), s->pipe.grid.my[g], sizeof(float) * R2, hipMemcpyDeviceToHost));
+        std::vector<float> ch[4];
+        for (int k = 0; k < 4; ++k) {
+            ch[k].resize(R2);
+            HIP_TRY(hipMemcpy(ch[k].data(), s->pipe.grid.acc[g][kAccPx + k], sizeof(float) * R2, hipMemcpyDeviceToHost));
+        }
+        const float dt = st.last_dt;
         for (size_t i = 0; i < R2; ++i) {
-            momentum[2 * i] = mx[i];
-            momentum[2 * i + 1] = my[i];
+            momentum[2 * i] = ch[0][i] + dt * ch[2][i];
+            momentum[2 * i + 1] = ch[1][i] + dt * ch[3][i];
         }
     }
     return EMH_OK;
@@ -1167,8 +1208,10 @@ uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t R) {
     int32_t hi = (int32_t)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
-    uint32_t tpd = (R + 15u) / 16u;
-    return (((uint32_t)cx >> 4) * tpd + ((uint32_t)cy >> 4)) * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
+    const uint32_t tpd = (R + 15u) / 16u;
+    const uint32_t tx = (uint32_t)cx >> 4, ty = (uint32_t)cy >> 4;
+    const uint32_t tile = tx * tpd + ((tx & 1u) ? (tpd - 1u - ty) : ty);
+    return tile * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
 }
 
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out, uint32_t* perm_out) {

@@ -57,7 +57,18 @@ struct DevConfig {
     uint32_t tiles_per_dim;   // 16x16-cell tiles per grid side
     uint32_t n_bins;          // tiles_per_dim^2 * 256
     uint32_t density_pass;    // any material needs_density_recompute || config flag
+    uint32_t fields;          // kF* bits: optional particle fields the particles' materials use
+    uint32_t only_model;      // the model every particle's material has, or kModelMixed
 };
+
+// Optional particle fields (DevConfig::fields).  A field no registered
+// material reads or writes stays untouched in memory between reorders.
+constexpr uint32_t kFQ = 1u;        // friction_hardening (NeoHookean damage, Drucker-Prager)
+constexpr uint32_t kFLvs = 2u;      // log_volume_strain (Drucker-Prager, NACC alpha)
+constexpr uint32_t kFJp = 4u;       // plastic_volume_ratio (snow)
+constexpr uint32_t kFH = 8u;        // hardening_scale (corotated, snow, NACC)
+constexpr uint32_t kFVolDens = 16u; // volume/density read back (fluid, NACC, fallback)
+constexpr uint32_t kFAll = 31u;
 
 // ---------------------------------------------------------------- materials
 // Device material record, filled on the host from MaterialParams + extension.
@@ -87,7 +98,7 @@ struct DevMat {
 };
 
 constexpr uint32_t kModelFallback = 0, kModelFluid = 1, kModelNH = 2, kModelCorot = 3, kModelSnow = 4,
-                   kModelDP = 5, kModelNacc = 10;
+                   kModelDP = 5, kModelNacc = 10, kModelMixed = 0xffffffffu;
 
 // ------------------------------------------------------------ small algebra
 struct M2 {
@@ -288,9 +299,15 @@ EMH_DEV bool project_state(PState& s, const DevConfig& cfg) {
 }
 
 // ------------------------------------------------------- constitutive models
+// Template argument ONLY of the per-model functions here and below: -1 dispatches on
+// the material's model id at run time; a model id compiles that model alone
+// (used when every particle resolves to materials of one model).
+
 // stress_volume: materials/mod.rs:118-120, fluid.rs:193-195
+template <int ONLY = -1>
 EMH_DEV float stress_volume(const DevMat& m, const PState& s) {
-    return m.model == kModelFluid ? rmax(s.vol, m.min_volume) : s.v0;
+    const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
+    return model == kModelFluid ? rmax(s.vol, m.min_volume) : s.v0;
 }
 
 EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
@@ -302,9 +319,11 @@ EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
 
 // Kirchhoff stress: elastic.rs:180-255, corotated.rs:62-78, sand.rs:220-231,
 // snow.rs:109-136, fluid.rs:135-191, nacc.rs:236-249, Fallback mod.rs:113-115.
+template <int ONLY = -1>
 EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
     const M2 zero{0.0f, 0.0f, 0.0f, 0.0f};
-    switch (m.model) {
+    const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
+    switch (model) {
         case kModelNH: {
             float j = mdet(s.f);
             if (j <= kMinJ) return zero;
@@ -386,8 +405,10 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
 
 // timestep_bound: elastic.rs:290-327, corotated.rs:108-125, fluid.rs:228-259,
 // sand.rs:345-362, snow.rs:193-211, nacc.rs:279-298 (Fallback: +inf).
+template <int ONLY = -1>
 EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevConfig& cfg) {
-    switch (m.model) {
+    const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
+    switch (model) {
         case kModelNH: {
             float e = elastic_wave_dt(m.lambda, m.mu, 1.0f, density, m.min_density, cfg.cell, cfg.material_cfl);
             float v = kInf;
@@ -540,8 +561,10 @@ EMH_DEV void sync_volume(PState& s) {
 
 // update_particle: elastic.rs:262-269, corotated.rs:84-91, sand.rs:248-319,
 // snow.rs:142-176, fluid.rs:197-207, nacc.rs:255-269; Fallback: no-op.
+template <int ONLY = -1>
 EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
-    switch (m.model) {
+    const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
+    switch (model) {
         case kModelNH:
         case kModelCorot: {
             M2 fp = madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt));
@@ -630,13 +653,16 @@ EMH_DEV float affine_cfl(const M2& c, float cell) {
     return g * kCorner * cell;
 }
 
-// Cell key: tile-major over 16x16-cell tiles, local cell x-major; cell
-// coordinates clamped into the grid.
+// Cell key: 16x16-cell tiles in serpentine order (tile column tx runs up in
+// y when tx is even and down when odd, so consecutive tiles always share an
+// edge), local cell x-major inside the tile; cell coordinates are clamped
+// into the grid.
 EMH_DEV uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t tpd) {
     int hi = (int)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
-    uint32_t tile = ((uint32_t)cx >> 4) * tpd + ((uint32_t)cy >> 4);
+    const uint32_t tx = (uint32_t)cx >> 4, ty = (uint32_t)cy >> 4;
+    const uint32_t tile = tx * tpd + ((tx & 1u) ? (tpd - 1u - ty) : ty);
     return tile * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
 }
 

@@ -1,31 +1,42 @@
 // kernels.hip -- the MLS-MPM substep as gfx950 HIP kernels.
 //
-// One substep (solver/step.rs:62-317) is, on the device:
-//   k_control      1 workgroup: the CFL dt of choose_substep_dt/cfl_bound
-//                  (step.rs:434-479) from per-block partials, the device-side
-//                  substep loop of Simulation::step (step.rs:34-49), and the
-//                  exclusive scan of the per-tile particle counts.
-//   k_cell_scan    per 16x16 tile: exclusive scan of per-cell counts -> the
-//                  start slot of every cell-key bin.
-//  [k_density_*]   estimate_particle_volumes (solver/density.rs:216-247) when
-//                  a fluid material needs it.
-//   k_p2g          scatter_particles_to_grid (transfer/p2g.rs:24-72): chunks
-//                  of 1024 particles (already in cell-key order from the
-//                  previous substep) accumulate their 3x3 stencils in an LDS
-//                  box and flush it with one global f32 atomic per touched
-//                  cell; the chunk also scatters the permutation for this
-//                  substep's sort.
-//   k_grid_update  Grid::update_velocities + SlipBoundary + speed clamp
-//                  (grid/mod.rs:264-291, boundary/mod.rs:78-101,
-//                  step.rs:150-165), and clears the buffers the next substep
-//                  accumulates into.
-//   k_g2p          gather_grid_to_particles + update_particle
-//                  (transfer/g2p.rs:151-343) for the particle at each sorted
-//                  slot, written out in cell-key order; its epilogue computes
-//                  the next substep's CFL partials, cell key and bin rank.
-// Particle state projection (step.rs:347-429) is applied on the fly by every
-// kernel that reads the state of a substep (it is a pure function of that
-// state), and persisted by the kernel that writes the substep's output.
+// Simulation::step (solver/step.rs:23-60) runs on the device as:
+//   k_begin_step     1 lane: starts the step's substep loop (step.rs:34).
+//  [k_keys]          on reorder steps: cell key and arrival rank of every
+//                    particle, per-cell and per-tile counts.
+//  [k_tile_scan]     on reorder steps: start slot of every 16x16 tile.
+//  [k_cell_scan]     on reorder steps: per tile, the start slot of each
+//                    cell-key bin.
+// then, per substep:
+//  [k_density_p2g]   only when a material re-estimates density
+//                    (solver/density.rs:216-247): gathers the density of the
+//                    mass grid the previous substep scattered, then P2G.
+//   k_substep        one workgroup per 1024 particles, in cell-key order:
+//                    0. the substep's dt (choose_substep_dt/cfl_bound,
+//                       step.rs:434-479) and the loop condition (step.rs:34),
+//                       derived by every workgroup from the control state and
+//                       the CFL reduction the previous launch left;
+//                    1. grid update (grid/mod.rs:264-291 + SlipBoundary
+//                       boundary/mod.rs:78-101 + speed clamp step.rs:150-165)
+//                       of the chunk's bounding box of cells, staged in LDS;
+//                    2. G2P + update_particle (transfer/g2p.rs:151-343);
+//                    3. the CFL contribution of the new state (step.rs:445-466);
+//                    4. the NEXT substep's P2G (transfer/p2g.rs:24-72) from the
+//                       new state held in registers, accumulated in LDS with
+//                       wave-level segmented sums and flushed with one global
+//                       f32 atomic per touched cell and channel.  The stress
+//                       term is kept apart from the momentum (its coefficient
+//                       -4 V0 dt (p2g.rs:42-44) is applied without dt) because
+//                       the next dt is not known until every particle is done;
+//                    5. the CFL contribution (max speed, min material dt) of
+//                       the new state, reduced with integer atomics on the
+//                       f32 bit patterns (both are non-negative).
+// The particle state is read once and written once per substep.
+//
+// Projection (project_particle_state_to_admissible, step.rs:347-429) is a
+// pure function of the stored state.  A kernel that writes a state checks
+// whether projecting it would change anything; only then does the next reader
+// project (and count) on the fly and persist the projected fields.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -44,7 +55,6 @@ __device__ __forceinline__ void gatomic_add(float* addr, float v) {
     __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wave-level exclusive/inclusive scans and reductions (64 lanes).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const uint32_t lane = lane_id();
 #pragma unroll
@@ -52,6 +62,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
         uint32_t n = __shfl_up(v, o, 64);
         if (lane >= (uint32_t)o) v += n;
     }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 __device__ __forceinline__ float wave_max(float v) {
@@ -94,6 +109,25 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& tota
     return out;
 }
 
+// Block-wide (max, min) of a float pair; every thread gets the result.
+__device__ void block_maxmin(float& a, float& d, float* s_red) {
+    a = wave_max(a);
+    d = wave_min(d);
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_red[2 * wave] = a;
+        s_red[2 * wave + 1] = d;
+    }
+    __syncthreads();
+    a = s_red[0];
+    d = s_red[1];
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+        a = fmaxf(a, s_red[2 * w]);
+        d = fminf(d, s_red[2 * w + 1]);
+    }
+    __syncthreads();
+}
+
 // Arrival rank of `key` among the lanes and waves that add to counts[key];
 // same-key lanes of a wave share one atomic.  Every lane of the wave must call.
 __device__ uint32_t agg_rank(uint32_t* counts, uint32_t key, bool valid) {
@@ -133,40 +167,58 @@ __device__ void agg_count(uint32_t* counts, uint32_t key, bool valid) {
     }
 }
 
-__device__ __forceinline__ PState load_state(const PBuf& b, uint32_t i) {
+// ------------------------------------------------------------ particle I/O
+// Fields a kernel does not load keep values projection leaves unchanged.
+__device__ __forceinline__ PState neutral_state() {
     PState s;
-    float2 x = b.x[i], v = b.v[i];
-    s.x = x.x;
-    s.y = x.y;
-    s.vx = v.x;
-    s.vy = v.y;
-    s.c = from4(b.c[i]);
+    s.x = s.y = s.vx = s.vy = 0.0f;
+    s.c = M2{0.0f, 0.0f, 0.0f, 0.0f};
+    s.f = M2{1.0f, 0.0f, 0.0f, 1.0f};
+    s.mass = s.v0 = s.vol = s.dens = 1.0f;
+    s.jp = s.h = 1.0f;
+    s.q = s.lvs = 0.0f;
+    s.mat = 0;
+    return s;
+}
+// x is loaded by the caller; v and C only when `vc`.
+__device__ __forceinline__ void load_fields(PState& s, const PBuf& b, uint32_t i, uint32_t mask, bool vc) {
+    if (vc) {
+        const float2 v = b.v[i];
+        s.vx = v.x;
+        s.vy = v.y;
+        s.c = from4(b.c[i]);
+    }
     s.f = from4(b.f[i]);
     s.mass = b.mass[i];
     s.v0 = b.v0[i];
-    s.vol = b.vol[i];
-    s.dens = b.dens[i];
-    s.jp = b.jp[i];
-    s.h = b.h[i];
-    s.q = b.q[i];
-    s.lvs = b.lvs[i];
     s.mat = b.mat[i];
-    return s;
+    if (mask & kFVolDens) {
+        s.vol = b.vol[i];
+        s.dens = b.dens[i];
+    }
+    if (mask & kFJp) s.jp = b.jp[i];
+    if (mask & kFH) s.h = b.h[i];
+    if (mask & kFQ) s.q = b.q[i];
+    if (mask & kFLvs) s.lvs = b.lvs[i];
 }
-__device__ __forceinline__ void store_state(const PBuf& b, uint32_t i, const PState& s) {
+// x, v, C, F, volume and density always; the optional fields in `mask`;
+// mass, initial volume and material word when `meta`.
+__device__ __forceinline__ void store_fields(const PBuf& b, uint32_t i, const PState& s, uint32_t mask, bool meta) {
     b.x[i] = make_float2(s.x, s.y);
     b.v[i] = make_float2(s.vx, s.vy);
     b.c[i] = to4(s.c);
     b.f[i] = to4(s.f);
-    b.mass[i] = s.mass;
-    b.v0[i] = s.v0;
     b.vol[i] = s.vol;
     b.dens[i] = s.dens;
-    b.jp[i] = s.jp;
-    b.h[i] = s.h;
-    b.q[i] = s.q;
-    b.lvs[i] = s.lvs;
-    b.mat[i] = s.mat;
+    if (mask & kFJp) b.jp[i] = s.jp;
+    if (mask & kFH) b.h[i] = s.h;
+    if (mask & kFQ) b.q[i] = s.q;
+    if (mask & kFLvs) b.lvs[i] = s.lvs;
+    if (meta) {
+        b.mass[i] = s.mass;
+        b.v0[i] = s.v0;
+        b.mat[i] = s.mat;
+    }
 }
 
 // Position part of project_particle_state_to_admissible (step.rs:357-362).
@@ -181,179 +233,110 @@ __device__ __forceinline__ void project_x(float& x, float& y, const DevConfig& c
     }
 }
 
+// Would projecting `s` change it?  (`sp` receives the projected state.)
+__device__ __forceinline__ bool needs_projection(const PState& s, PState& sp, const DevConfig& cfg) {
+    sp = s;
+    if (!cfg.project) return false;
+    const bool reset = project_state(sp, cfg);
+    return reset || __float_as_uint(sp.x) != __float_as_uint(s.x) || __float_as_uint(sp.y) != __float_as_uint(s.y) ||
+           __float_as_uint(sp.dens) != __float_as_uint(s.dens);
+}
+
 __device__ __forceinline__ const DevMat& mat_of(const Pipeline& p, uint32_t word) {
     uint32_t m = word & kMatMask;  // MaterialRegistry::get falls back to slot 0 (registry.rs:66-76)
     return p.mats[m < p.n_mats ? m : 0u];
 }
 
-// Epilogue shared by the prologue and G2P: CFL contribution of the state as
-// it stands (choose_substep_dt reads it before projection, step.rs:445-466),
-// then the cell key of the projected position and its arrival rank.
-__device__ void finish_particle(const Pipeline& p, const PState& s, bool valid, const PBuf& out, uint32_t dest,
-                                float* s_red) {
+// CFL contribution of one particle as choose_substep_dt reads it, before
+// projection (step.rs:445-466): speed incl. affine term, material dt bound.
+template <int ONLY>
+__device__ __forceinline__ void cfl_of(const Pipeline& p, const PState& s, float& speed, float& mdt) {
+    float sp = len2(s.vx, s.vy);
+    if (p.cfg.cfl_affine) sp = sp + affine_cfl(s.c, p.cfg.cell);
+    speed = fmaxf(speed, sp);
+    const float b = timestep_bound<ONLY>(mat_of(p, s.mat), s.dens, s.h, p.cfg);
+    if (__builtin_isfinite(b) && b > 0.0f) mdt = fminf(mdt, b);
+}
+
+// The decision of control launch s: the loop condition of Simulation::step
+// (step.rs:34) on the state launch s - 1 left, then choose_substep_dt /
+// cfl_bound (step.rs:434-479) from the reduction of launch s - 1.
+__device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const DevConfig& cfg) {
+    CtrlState N = P;
+    if (!(P.remaining > kF32Eps && P.substeps < cfg.max_substeps)) {
+        N.active = 0;
+        return N;
+    }
+    const float a = __uint_as_float(Q.max_speed), d = __uint_as_float(Q.min_mdt);
+    const float remaining = P.remaining;
+    float dt;
+    if (!cfg.adaptive) {
+        dt = rmin(remaining, cfg.dt);
+    } else {
+        float min_mat_dt = rmin(remaining, d);
+        dt = remaining;
+        if (a > kF32Eps) dt = rmin(dt, cfg.cfl_coefficient * cfg.cell / a);
+        dt = rmin(dt, min_mat_dt);
+        dt = rclamp(dt, rmin(cfg.min_dt, remaining), remaining);
+    }
+    N.dt = dt;
+    N.last_dt = dt;
+    N.active = 1;
+    N.substeps = P.substeps + 1;
+    N.remaining = remaining - dt;
+    N.total_substeps = P.total_substeps + 1;
+    N.cur_grid = P.cur_grid == 2u ? 0u : P.cur_grid + 1u;
+    N.cur_buf = P.cur_buf ^ (P.reorder ? 1u : 0u);
+    N.reorder = 0;
+    return N;
+}
+__device__ __forceinline__ void reset_red(CtrlRed& r) {
+    r.max_speed = 0u;
+    r.min_mdt = kInfBits;
+    r.dirty = 0u;
+}
+__device__ __forceinline__ uint32_t seq_mod3(uint64_t s) { return (uint32_t)(s % 3ull); }
+
+// Grid velocity of one cell after Grid::update_velocities, gravity, the slip
+// walls and the speed clamp (grid/mod.rs:264-291, step.rs:140-165), from the
+// P2G accumulators of the current substep; (0, 0) for untouched cells
+// (velocity_at of a missing cell, grid/mod.rs:215-233).
+__device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set, float dt, int cx, int cy) {
     const DevConfig& cfg = p.cfg;
-    float speed = 0.0f, mdt = kInf;
-    uint32_t key = 0;
-    if (valid) {
-        float sp = len2(s.vx, s.vy);
-        if (cfg.cfl_affine) sp = sp + affine_cfl(s.c, cfg.cell);
-        speed = sp;
-        float b = timestep_bound(mat_of(p, s.mat), s.dens, s.h, cfg);
-        mdt = (__builtin_isfinite(b) && b > 0.0f) ? b : kInf;
-        float px = s.x, py = s.y;
-        if (cfg.project) project_x(px, py, cfg);
-        key = cell_key(sat_i32(floorf(px)), sat_i32(floorf(py)), cfg.grid_res, cfg.tiles_per_dim);
+    const int R = (int)cfg.grid_res;
+    if (cx < 0 || cy < 0 || cx >= R || cy >= R) return make_float2(0.0f, 0.0f);
+    const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
+    const float m = p.grid.acc[set][kAccM][gi];
+    if (!(m > 0.0f)) return make_float2(0.0f, 0.0f);
+    float vx = p.grid.acc[set][kAccPx][gi] + dt * p.grid.acc[set][kAccSx][gi];
+    float vy = p.grid.acc[set][kAccPy][gi] + dt * p.grid.acc[set][kAccSy][gi];
+    vx = vx / m;
+    vy = vy / m;
+    vx = vx + cfg.gx * dt;
+    vy = vy + cfg.gy * dt;
+    const uint32_t x = (uint32_t)cx, y = (uint32_t)cy;
+    for (uint32_t b = 0; b < cfg.n_slip; ++b) {
+        const uint32_t t = cfg.slip[b];
+        const uint32_t hi = (uint32_t)R - (t + 1);
+        if (x < t) vx = rmax(vx, 0.0f);
+        if (x > hi) vx = rmin(vx, 0.0f);
+        if (y < t) vy = rmax(vy, 0.0f);
+        if (y > hi) vy = rmin(vy, 0.0f);
     }
-    uint32_t rank = agg_rank(p.sort.cell_count, key, valid);
-    agg_count(p.sort.tile_count, key >> 8, valid);
-    if (valid) {
-        out.key[dest] = key;
-        out.rank[dest] = rank;
+    const float vel_limit = cfg.cell / dt;
+    const float spd = len2(vx, vy);
+    if (spd > vel_limit) {
+        const float k = vel_limit / spd;
+        vx = vx * k;
+        vy = vy * k;
     }
-    // block reduction of (max speed, min dt) -> one partial per block
-    float ws = wave_max(speed), wd = wave_min(mdt);
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        s_red[2 * wave] = ws;
-        s_red[2 * wave + 1] = wd;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float a = s_red[0], d = s_red[1];
-        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
-            a = fmaxf(a, s_red[2 * w]);
-            d = fminf(d, s_red[2 * w + 1]);
-        }
-        p.sort.cfl_partial[blockIdx.x] = make_float2(a, d);
-    }
-}
-
-// ------------------------------------------------------------------ kernels
-
-__global__ void k_zero_counters(const Pipeline p) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
-    if (i < p.cfg.n_bins) p.sort.cell_count[i] = 0;
-    if (i < p.cfg.tiles_per_dim * p.cfg.tiles_per_dim) p.sort.tile_count[i] = 0;
-    if (i < R2) {
-        for (int g = 0; g < 2; ++g) {
-            p.grid.m[g][i] = 0.0f;
-            p.grid.mx[g][i] = 0.0f;
-            p.grid.my[g][i] = 0.0f;
-        }
-        p.grid.vel[i] = make_float2(0.0f, 0.0f);
-        p.grid.dmass[i] = 0.0f;
-    }
-}
-
-// Keys, ranks and CFL partials of a freshly uploaded state.
-__global__ __launch_bounds__(kBlock) void k_prologue(const Pipeline p) {
-    __shared__ float s_red[2 * (kBlock / 64)];
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const PBuf b = sel_buf(p, p.ctrl->cur_buf);
-    bool valid = i < p.n;
-    PState s{};
-    if (valid) s = load_state(b, i);
-    finish_particle(p, s, valid, b, i, s_red);
-}
-
-__global__ void k_begin_step(const Pipeline p) {
-    Ctrl* c = p.ctrl;
-    c->remaining = p.cfg.dt;
-    c->substeps = 0;
-    c->active = 0;
-    c->clamp_count = 0;
-    c->proj_count = 0;
-}
-
-// One workgroup: substep loop control, CFL dt and the tile scan.
-__global__ __launch_bounds__(1024) void k_control(const Pipeline p) {
-    __shared__ float s_a[16], s_d[16];
-    __shared__ uint32_t s_scan[17];
-    Ctrl* c = p.ctrl;
-    const DevConfig& cfg = p.cfg;
-    const float remaining = c->remaining;
-    const uint32_t subs = c->substeps;
-    // while remaining > f32::EPSILON && substeps < max_substeps_per_step  (step.rs:34)
-    if (!(remaining > kF32Eps && subs < cfg.max_substeps)) {
-        if (threadIdx.x == 0) c->active = 0;
-        return;
-    }
-    float ms = 0.0f, md = kInf;
-    for (uint32_t i = threadIdx.x; i < c->n_cfl_blocks; i += blockDim.x) {
-        float2 v = p.sort.cfl_partial[i];
-        ms = fmaxf(ms, v.x);
-        md = fminf(md, v.y);
-    }
-    ms = wave_max(ms);
-    md = wave_min(md);
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) {
-        s_a[wave] = ms;
-        s_d[wave] = md;
-    }
-    // tile scan: each thread sums a contiguous run of tiles
-    const uint32_t n_tiles = cfg.tiles_per_dim * cfg.tiles_per_dim;
-    const uint32_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
-    const uint32_t beg = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (beg + k < n_tiles) sum += p.sort.tile_count[beg + k];
-    uint32_t total;
-    uint32_t run = block_excl_scan(sum, s_scan, total);  // contains __syncthreads
-    for (uint32_t k = 0; k < per; ++k) {
-        if (beg + k < n_tiles) {
-            p.sort.tile_start[beg + k] = run;
-            run += p.sort.tile_count[beg + k];
-        }
-    }
-    if (threadIdx.x == 0) {
-        p.sort.tile_start[n_tiles] = total;
-        float a = s_a[0], d = s_d[0];
-        for (uint32_t w = 1; w < nw; ++w) {
-            a = fmaxf(a, s_a[w]);
-            d = fminf(d, s_d[w]);
-        }
-        // choose_substep_dt / cfl_bound  step.rs:434-479
-        float dt;
-        if (!cfg.adaptive) {
-            dt = rmin(remaining, cfg.dt);
-        } else {
-            float min_mat_dt = rmin(remaining, d);
-            dt = remaining;
-            if (a > kF32Eps) dt = rmin(dt, cfg.cfl_coefficient * cfg.cell / a);
-            dt = rmin(dt, min_mat_dt);
-            dt = rclamp(dt, rmin(cfg.min_dt, remaining), remaining);
-        }
-        c->dt = dt;
-        c->active = 1;
-        c->substeps = subs + 1;
-        c->remaining = remaining - dt;
-        c->last_dt = dt;
-        c->total_substeps = c->total_substeps + 1;
-        c->cur_grid ^= 1u;
-        c->cur_buf ^= 1u;  // the state after this substep lives in the other buffer
-    }
-}
-
-// Cell-level exclusive scan inside each 16x16 tile.
-__global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
-    __shared__ uint32_t s_scan[17];
-    if (!p.ctrl->active) return;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t idx = tile * kTileCells + threadIdx.x;
-    uint32_t cnt = p.sort.cell_count[idx];
-    uint32_t total;
-    uint32_t ex = block_excl_scan(cnt, s_scan, total);
-    p.sort.cell_start[idx] = p.sort.tile_start[tile] + ex;
-    if (tile == gridDim.x - 1 && threadIdx.x == kTileCells - 1)
-        p.sort.cell_start[p.cfg.n_bins] = p.sort.tile_start[gridDim.x];
+    return make_float2(vx, vy);
 }
 
 // ---------------------------------------------------- wave segmented sums
 // Lanes of a wave hold consecutive particles of a cell-sorted chunk, so runs
-// of lanes share a base cell and with it all nine stencil cells.  The run's
-// 27 contributions are summed across its lanes with DPP row shifts and row
+// of lanes share a base cell and with it all nine stencil cells.  A run's
+// contributions are summed across its lanes with DPP row shifts and row
 // broadcasts (VALU work, no LDS traffic), leaving each run's total in its
 // last lane, which alone issues the atomics.
 struct SegMask {
@@ -427,26 +410,20 @@ __device__ __forceinline__ bool run_tail(const SegMask& m) {
     return m.lane == 63 || ((m.heads >> (m.lane + 1)) & 1ull);
 }
 
-// Chunk bounding box of base cells (positions projected) -> LDS box origin/size.
+// ------------------------------------------------------------- chunk box
+// LDS box of grid cells: [ox, ox + w) x [oy, oy + h), cell (cx, cy) at
+// (cx - ox) * h + (cy - oy).  `lds` is false when the box exceeds the budget
+// (sparse chunks); those chunks read and scatter the global grid directly.
 struct Box {
     int ox, oy, w, h;
     bool lds;
-};
-__device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool project, int* s_box) {
-    int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
-    for (int k = 0; k < kChunk / kBlock; ++k) {
-        uint32_t j = base + k * kBlock + threadIdx.x;
-        if (j < p.n) {
-            float2 x = a.x[j];
-            float px = x.x, py = x.y;
-            if (project) project_x(px, py, p.cfg);
-            int bx = sat_i32(floorf(px)), by = sat_i32(floorf(py));
-            mnx = min(mnx, bx);
-            mny = min(mny, by);
-            mxx = max(mxx, bx);
-            mxy = max(mxy, by);
-        }
+    __device__ __forceinline__ bool holds3x3(int bx, int by) const {  // cells bx-1..bx+1, by-1..by+1
+        const long long lx = (long long)bx - 1 - ox, ly = (long long)by - 1 - oy;
+        return lds && lx >= 0 && ly >= 0 && lx + 3 <= w && ly + 3 <= h;
     }
+    __device__ __forceinline__ int index(int cx, int cy) const { return (cx - ox) * h + (cy - oy); }
+};
+__device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_box) {
     mnx = wave_imin(mnx);
     mny = wave_imin(mny);
     mxx = wave_imax(mxx);
@@ -459,7 +436,7 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
         s_box[4 * wave + 3] = mxy;
     }
     __syncthreads();
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < kWaves; ++w) {
         mnx = min(mnx, s_box[4 * w + 0]);
         mny = min(mny, s_box[4 * w + 1]);
         mxx = max(mxx, s_box[4 * w + 2]);
@@ -467,346 +444,514 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
     }
     __syncthreads();
     Box b;
-    b.ox = mnx - 1;
-    b.oy = mny - 1;
-    // widths in 64-bit to stay safe for any position range
-    long long w = (long long)mxx - (long long)mnx + 3, h = (long long)mxy - (long long)mny + 3;
-    b.lds = (w > 0 && h > 0 && w * h <= (long long)kLdsCells);
+    const long long w = (long long)mxx - (long long)mnx + 1 + 2 * margin;
+    const long long h = (long long)mxy - (long long)mny + 1 + 2 * margin;
+    b.lds = (mnx <= mxx) && (long long)mnx - margin > INT_MIN / 2 && (long long)mxx + margin < INT_MAX / 2 &&
+            (long long)mny - margin > INT_MIN / 2 && (long long)mxy + margin < INT_MAX / 2 && w * h <= kLdsCells;
+    b.ox = b.lds ? mnx - margin : 0;
+    b.oy = b.lds ? mny - margin : 0;
     b.w = b.lds ? (int)w : 0;
     b.h = b.lds ? (int)h : 0;
     return b;
 }
 
-// Density re-estimate, scatter half: Σ w·m per cell (p2g.rs:230-243).
-__global__ __launch_bounds__(kBlock) void k_density_scatter(const Pipeline p) {
-    __shared__ float s_m[kLdsCells];
-    __shared__ int s_box[4 * (kBlock / 64)];
-    const Ctrl* c = p.ctrl;
-    if (!c->active) return;
-    const PBuf a = sel_buf(p, c->cur_buf ^ 1u);  // state at the start of this substep
-    const uint32_t base = blockIdx.x * kChunk;
-    const uint32_t R = p.cfg.grid_res;
-    Box bx = chunk_box(p, a, base, p.cfg.project != 0, s_box);
-    if (bx.lds) {
-        for (int i = threadIdx.x; i < bx.w * bx.h; i += kBlock) s_m[i] = 0.0f;
-        __syncthreads();
+
+// P2G contribution of one particle to the 3x3 stencil, one channel at a time
+// (p2g.rs:45-68): mass, momentum m (v + C d) per axis, and the stress moment
+// -4 V0 (tau d) per axis that the grid update multiplies by dt.
+struct P2GTerms {
+    Wts w;
+    float x, y, vx, vy, mass, coeff;
+    M2 c, tau;
+};
+template <int CH>
+__device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy) {
+            const float weight = t.w.wx[gx] * t.w.wy[gy];
+            const float dx = ((float)(t.w.bx + gx - 1) - t.x) + 0.5f;
+            const float dy = ((float)(t.w.by + gy - 1) - t.y) + 0.5f;
+            float r;
+            if (CH == kAccM) {
+                r = weight * t.mass;
+            } else if (CH == kAccPx) {
+                r = weight * (t.mass * (t.vx + (t.c.a * dx + t.c.c * dy)));
+            } else if (CH == kAccPy) {
+                r = weight * (t.mass * (t.vy + (t.c.b * dx + t.c.d * dy)));
+            } else if (CH == kAccSx) {
+                r = weight * (t.coeff * (t.tau.a * dx + t.tau.c * dy));
+            } else {
+                r = weight * (t.coeff * (t.tau.b * dx + t.tau.d * dy));
+            }
+            v[gx * 3 + gy] = r;
+        }
+}
+// Segmented wave sum of one channel, then the run tails add it to the LDS box
+// (or to the global grid when the stencil leaves the box).
+template <int CH>
+__device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, const SegMask& sm, const Box& box,
+                                                float* s_ch, float* g_ch, int R) {
+    float v[9];
+    if (valid) {
+        p2g_channel<CH>(t, v);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[q] = 0.0f;
     }
-    for (int k = 0; k < kChunk / kBlock; ++k) {
-        const uint32_t j = base + k * kBlock + threadIdx.x;
-        const bool valid = j < p.n;
-        float vals[9];
-        int bxc = INT_MIN + (int)lane_id(), byc = 0;
-        if (valid) {
-            PState s = load_state(a, j);
-            if (p.cfg.project) project_state(s, p.cfg);
-            Wts w = weights(s.x, s.y);
-            bxc = w.bx;
-            byc = w.by;
+    seg_scan(v, sm);
+    if (valid && run_tail(sm)) {
+        const int bx = t.w.bx, by = t.w.by;
+        if (box.holds3x3(bx, by)) {
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-                for (int gy = 0; gy < 3; ++gy) vals[gx * 3 + gy] = (w.wx[gx] * w.wy[gy]) * s.mass;
+                for (int gy = 0; gy < 3; ++gy) atomicAdd(&s_ch[box.index(bx + gx - 1, by + gy - 1)], v[gx * 3 + gy]);
         } else {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) vals[q] = 0.0f;
-        }
-        const SegMask sm = run_mask(bxc, byc);
-        seg_scan(vals, sm);
-        if (valid && run_tail(sm)) {
             for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
                 for (int gy = 0; gy < 3; ++gy) {
-                    const float wm = vals[gx * 3 + gy];
-                    const int cx = bxc + gx - 1, cy = byc + gy - 1;
-                    if (bx.lds) {
-                        atomicAdd(&s_m[(cx - bx.ox) * bx.h + (cy - bx.oy)], wm);
-                    } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
-                        gatomic_add(&p.grid.dmass[(uint32_t)cx * R + (uint32_t)cy], wm);
-                    }
+                    const int cx = bx + gx - 1, cy = by + gy - 1;
+                    if (cx >= 0 && cy >= 0 && cx < R && cy < R)
+                        gatomic_add(&g_ch[(uint32_t)cx * (uint32_t)R + (uint32_t)cy], v[gx * 3 + gy]);
                 }
         }
     }
-    if (bx.lds) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < bx.w * bx.h; i += kBlock) {
-            float m = s_m[i];
-            if (m == 0.0f) continue;
-            int cx = bx.ox + i / bx.h, cy = bx.oy + i % bx.h;
-            if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) gatomic_add(&p.grid.dmass[(uint32_t)cx * R + cy], m);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+// run of chunks so that neighbouring chunks share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Block reduction of the CFL contribution, one atomic pair per workgroup.
+// f32::max / f32::min ignore NaN (as cfl_bound's fold does, step.rs:445-466);
+// a NaN block result means every lane was NaN and contributes nothing.
+__device__ void reduce_cfl(CtrlRed* red, float speed, float mdt, float* s_red) {
+    block_maxmin(speed, mdt, s_red);
+    if (threadIdx.x == 0) {
+        if (!__builtin_isnan(speed) && speed > 0.0f) atomicMax(&red->max_speed, __float_as_uint(speed));
+        if (!__builtin_isnan(mdt) && mdt < kInf) atomicMin(&red->min_mdt, __float_as_uint(mdt));
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_zero_all(const Pipeline p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
+    if (i < p.cfg.n_bins) p.sort.cell_count[i] = 0;
+    if (i < p.cfg.tiles_per_dim * p.cfg.tiles_per_dim) p.sort.tile_count[i] = 0;
+    if (i < R2) {
+        for (int s = 0; s < 3; ++s) {
+            for (int ch = 0; ch < kAccChannels; ++ch) p.grid.acc[s][ch][i] = 0.0f;
+            p.grid.dm[s][i] = 0.0f;
         }
     }
 }
 
-// Density re-estimate, gather half (density.rs:225-246), in place; this pass
-// also persists the substep's projected state, so later kernels of the
-// substep do not project again.
-__global__ __launch_bounds__(kBlock) void k_density_gather(const Pipeline p) {
+// Control launch `seq` that starts a step: remaining = config.dt, no
+// substeps yet (step.rs:24-33); the CFL reduction carries over.
+__global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
+    Ctrl* c = p.ctrl;
+    CtrlState N = c->st[seq & 1u];
+    N.remaining = p.cfg.dt;
+    N.substeps = 0;
+    N.active = 0;
+    N.reorder = reorder;
+    c->st[(seq + 1u) & 1u] = N;
+    c->red[seq_mod3(seq)] = c->red[seq_mod3(seq + 2u)];
+    reset_red(c->red[seq_mod3(seq + 1u)]);
+    c->clamp_count = 0;
+    c->proj_count = 0;
+}
+
+// Cell key of the stored (projected) position of every particle, its arrival
+// rank inside the key's bin, and the per-tile counts.  `seq` is the control
+// launch that will read the keys.
+__global__ __launch_bounds__(kBlock) void k_keys(const Pipeline p, uint64_t seq) {
     const Ctrl* c = p.ctrl;
-    if (!c->active) return;
-    const PBuf a = sel_buf(p, c->cur_buf ^ 1u);
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= p.n) return;
-    const uint32_t R = p.cfg.grid_res;
-    PState s = load_state(a, j);
-    bool projected = false;
-    if (p.cfg.project) {
-        projected = project_state(s, p.cfg);
-        if (projected) atomicAdd(&p.ctrl->proj_count, 1u);
+    const DevConfig& cfg = p.cfg;
+    const PBuf a = sel_buf(p, c->st[seq & 1u].cur_buf);
+    const bool project = cfg.project && c->red[seq_mod3(seq + 2u)].dirty;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = j < p.n;
+    uint32_t key = 0;
+    if (valid) {
+        float2 x = a.x[j];
+        if (project) project_x(x.x, x.y, cfg);
+        key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), cfg.grid_res, cfg.tiles_per_dim);
     }
-    Wts w = weights(s.x, s.y);
-    float density = 0.0f;
-    for (int gx = 0; gx < 3; ++gx)
-        for (int gy = 0; gy < 3; ++gy) {
-            float weight = w.wx[gx] * w.wy[gy];
-            int cx = w.bx + gx - 1, cy = w.by + gy - 1;
-            float m = 0.0f;
-            if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) m = p.grid.dmass[(uint32_t)cx * R + cy];
-            density = density + m * weight;
-        }
-    if (density > kF32Eps) {
-        s.dens = density;
-        s.vol = s.mass / density;
-    }
-    if (projected) {
-        store_state(a, j, s);
-    } else {
-        a.x[j] = make_float2(s.x, s.y);
-        a.dens[j] = s.dens;
-        a.vol[j] = s.vol;
+    const uint32_t rank = agg_rank(p.sort.cell_count, key, valid);
+    agg_count(p.sort.tile_count, key >> 8, valid);
+    if (valid) {
+        p.sort.key[j] = key;
+        p.sort.rank[j] = rank;
     }
 }
 
-// P2G.  PROJECT: apply the state projection on the fly (no density pass ran).
-template <bool PROJECT>
-__global__ __launch_bounds__(kBlock) void k_p2g(const Pipeline p) {
-    __shared__ float s_m[kLdsCells], s_mx[kLdsCells], s_my[kLdsCells];
-    __shared__ int s_box[4 * (kBlock / 64)];
-    const Ctrl* c = p.ctrl;
-    if (!c->active) return;
-    const PBuf a = sel_buf(p, c->cur_buf ^ 1u);
-    const float dt = c->dt;
-    const uint32_t g = c->cur_grid;
-    float* gm = g ? p.grid.m[1] : p.grid.m[0];
-    float* gmx = g ? p.grid.mx[1] : p.grid.mx[0];
-    float* gmy = g ? p.grid.my[1] : p.grid.my[0];
-    const uint32_t base = blockIdx.x * kChunk;
-    const uint32_t R = p.cfg.grid_res;
-
-    // this substep's counting-sort scatter: slot = bin start + arrival rank
-    for (int k = 0; k < kChunk / kBlock; ++k) {
-        uint32_t j = base + k * kBlock + threadIdx.x;
-        if (j < p.n) p.sort.perm[p.sort.cell_start[a.key[j]] + a.rank[j]] = j;
+// Exclusive scan of the tile counts (one workgroup); clears the counts.
+__global__ __launch_bounds__(1024) void k_tile_scan(const Pipeline p) {
+    __shared__ uint32_t s_scan[17];
+    const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
+    const uint32_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (beg + k < n_tiles) sum += p.sort.tile_count[beg + k];
+    uint32_t total;
+    uint32_t run = block_excl_scan(sum, s_scan, total);
+    for (uint32_t k = 0; k < per; ++k) {
+        if (beg + k < n_tiles) {
+            const uint32_t cnt = p.sort.tile_count[beg + k];
+            p.sort.tile_start[beg + k] = run;
+            p.sort.tile_count[beg + k] = 0;
+            run += cnt;
+        }
     }
+}
 
-    Box bx = chunk_box(p, a, base, PROJECT && p.cfg.project, s_box);
-    if (bx.lds) {
-        for (int i = threadIdx.x; i < bx.w * bx.h; i += kBlock) {
-            s_m[i] = 0.0f;
-            s_mx[i] = 0.0f;
-            s_my[i] = 0.0f;
+// Start slot of every cell-key bin inside each 16x16 tile; clears the counts.
+__global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
+    __shared__ uint32_t s_scan[17];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t idx = tile * kTileCells + threadIdx.x;
+    const uint32_t cnt = p.sort.cell_count[idx];
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(cnt, s_scan, total);
+    p.sort.cell_start[idx] = p.sort.tile_start[tile] + ex;
+    p.sort.cell_count[idx] = 0;
+}
+
+constexpr int kModeG2P = 1;   // gather + update the stored state (a substep)
+constexpr int kModeP2G = 2;   // scatter mass, momentum and stress moment
+constexpr int kModeMass = 4;  // scatter mass only (density re-estimate)
+
+// Bounding box of the chunk's base cells (positions projected when the state
+// is flagged), widened by `margin` cells.
+__device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool project, int margin, int* s_box) {
+    int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t j = base + k * kBlock + threadIdx.x;
+        if (j < p.n) {
+            float2 x = a.x[j];
+            if (project) project_x(x.x, x.y, p.cfg);
+            const int bx = sat_i32(floorf(x.x)), by = sat_i32(floorf(x.y));
+            mnx = min(mnx, bx);
+            mny = min(mny, by);
+            mxx = max(mxx, bx);
+            mxy = max(mxy, by);
+        }
+    }
+    return block_box(mnx, mny, mxx, mxy, margin, s_box);
+}
+
+// P2G terms of a (projected) particle state (p2g.rs:24-72; the stress
+// coefficient -4 V0 without dt, transfer.rs:77-104 for tau).
+template <int ONLY>
+__device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s) {
+    P2GTerms t;
+    t.w = weights(s.x, s.y);
+    t.x = s.x;
+    t.y = s.y;
+    t.vx = s.vx;
+    t.vy = s.vy;
+    t.mass = s.mass;
+    t.c = s.c;
+    const DevMat& mt = mat_of(p, s.mat);
+    t.tau = kirchhoff<ONLY>(mt, s);
+    t.coeff = (-stress_volume<ONLY>(mt, s)) * kDInv;
+    return t;
+}
+
+// All five channels, one at a time (keeps 9 partial sums live, not 45).
+__device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
+                                            float (*s_acc)[kLdsCells], float* const* g, int R) {
+    const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
+    scatter_channel<kAccM>(t, valid, sm, box, s_acc[kAccM], g[kAccM], R);
+    scatter_channel<kAccPx>(t, valid, sm, box, s_acc[kAccPx], g[kAccPx], R);
+    scatter_channel<kAccPy>(t, valid, sm, box, s_acc[kAccPy], g[kAccPy], R);
+    scatter_channel<kAccSx>(t, valid, sm, box, s_acc[kAccSx], g[kAccSx], R);
+    scatter_channel<kAccSy>(t, valid, sm, box, s_acc[kAccSy], g[kAccSy], R);
+}
+
+// Add the LDS box into the global grid: one atomic per touched cell and channel.
+template <int NCH>
+__device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCells], float* const* g, int R) {
+    if (!box.lds) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
+        const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
+        if (cx < 0 || cy < 0 || cx >= R || cy >= R) continue;
+        const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const float v = s_acc[ch][i];
+            if (v != 0.0f) gatomic_add(&g[ch][gi], v);
+        }
+    }
+}
+
+// The substep kernel (MODE = G2P | P2G or G2P | Mass) and the prologue of a
+// freshly uploaded state (MODE = P2G or Mass: no gather, no store).  ONLY:
+// the one constitutive model every particle uses, or -1 for any mix.
+template <int MODE, int ONLY>
+__global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t seq) {
+    constexpr bool G2P = (MODE & kModeG2P) != 0;
+    constexpr bool P2G = (MODE & kModeP2G) != 0;
+    constexpr int NCH = P2G ? kAccChannels : 1;
+    __shared__ float s_acc[NCH][kLdsCells];
+    __shared__ float2 s_vel[G2P ? kLdsCells : 1];
+    __shared__ int s_box[4 * kWaves];
+    __shared__ float s_red[2 * kWaves];
+
+    // 0. this launch's substep, as every workgroup derives it
+    Ctrl* c = p.ctrl;
+    const DevConfig& cfg = p.cfg;
+    const CtrlState P = c->st[seq & 1u];
+    const CtrlRed Q = c->red[seq_mod3(seq + 2u)];
+    const CtrlState N = G2P ? decide_substep(P, Q, cfg) : P;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CtrlState W = N;
+        if (!G2P) W.active = 0;
+        c->st[(seq + 1u) & 1u] = W;
+        if (G2P && !N.active) c->red[seq_mod3(seq)] = Q;  // nothing ran: the reduction carries over
+        reset_red(c->red[seq_mod3(seq + 1u)]);
+    }
+    if (G2P && !N.active) return;
+    CtrlRed* red = &c->red[seq_mod3(seq)];
+    const int R = (int)cfg.grid_res;
+    const uint32_t cur = N.cur_grid;
+    const uint32_t nxt = cur == 2u ? 0u : cur + 1u;
+    const uint32_t old = nxt == 2u ? 0u : nxt + 1u;
+    const float dt = G2P ? N.dt : 0.0f;
+    // the prologue projects the uploaded state in registers; a substep
+    // projects its input only if the writer flagged it (and no density pass
+    // ran in between, which would have projected and persisted it)
+    const bool project_in = cfg.project && (G2P ? (Q.dirty && !cfg.density_pass) : true);
+    const bool reorder = G2P && P.reorder;
+    const bool full = !G2P || project_in || reorder;
+    const uint32_t mask = full ? kFAll : cfg.fields;
+    const PBuf a = sel_buf(p, P.cur_buf);
+    const PBuf o = reorder ? sel_buf(p, P.cur_buf ^ 1u) : a;
+    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t base = chunk * kChunk;
+
+    // 1. the chunk's box; a substep's output base cell moves at most one cell
+    //    (the G2P speed clamp bounds |v dt| by one cell), so the box is
+    //    widened by two for G2P and one for a scatter-only pass
+    const Box box = chunk_box(p, a, base, project_in, G2P ? 2 : 1, s_box);
+
+    // 2. clear the LDS accumulators; stage the updated grid velocity
+    if (box.lds) {
+        for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0f;
+            if (G2P) s_vel[i] = cell_velocity(p, cur, dt, box.ox + i / box.h, box.oy + i % box.h);
         }
         __syncthreads();
     }
-    for (int k = 0; k < kChunk / kBlock; ++k) {
+
+    float* gdst[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) gdst[ch] = P2G ? p.grid.acc[nxt][ch] : p.grid.dm[nxt];
+
+    float speed = 0.0f, mdt = kInf;
+    uint32_t n_clamp = 0, n_proj = 0;
+    bool any_dirty = false;
+#pragma unroll 1
+    for (int k = 0; k < kPer; ++k) {
         const uint32_t j = base + k * kBlock + threadIdx.x;
         const bool valid = j < p.n;
-        // contributions to the 3x3 stencil: mass, momentum x, momentum y
-        float vals[27];
-        int bxc = INT_MIN + (int)lane_id(), byc = 0;
+        PState sp = neutral_state();
         if (valid) {
-            PState s = load_state(a, j);
-            if (PROJECT && p.cfg.project) project_state(s, p.cfg);
-            const DevMat& mt = mat_of(p, s.mat);
-            const M2 tau = kirchhoff(mt, s);
-            const float coeff = ((-stress_volume(mt, s)) * kDInv) * dt;
+            PState s = neutral_state();
+            const float2 x = a.x[j];
+            s.x = x.x;
+            s.y = x.y;
+            load_fields(s, a, j, mask, !G2P || project_in);
+            if (G2P) {
+                if (project_in && project_state(s, cfg)) ++n_proj;
+                // 3. gather_grid_to_particles phase 1 (g2p.rs:203-325)
+                const Wts w = weights(s.x, s.y);
+                const bool lds = box.holds3x3(w.bx, w.by);
+                float nvx = 0.0f, nvy = 0.0f;
+                M2 b{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+                    for (int gy = 0; gy < 3; ++gy) {
+                        const float weight = w.wx[gx] * w.wy[gy];
+                        const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
+                        const float dx = ((float)cx - s.x) + 0.5f;
+                        const float dy = ((float)cy - s.y) + 0.5f;
+                        const float2 nv = lds ? s_vel[box.index(cx, cy)] : cell_velocity(p, cur, dt, cx, cy);
+                        const float wvx = nv.x * weight, wvy = nv.y * weight;
+                        b.a = b.a + wvx * dx;
+                        b.b = b.b + wvy * dx;
+                        b.c = b.c + wvx * dy;
+                        b.d = b.d + wvy * dy;
+                        nvx = nvx + wvx;
+                        nvy = nvy + wvy;
+                    }
+                if (s.mat & kPinnedBit) {
+                    s.vx = 0.0f;
+                    s.vy = 0.0f;
+                    s.c = M2{0.0f, 0.0f, 0.0f, 0.0f};
+                } else {
+                    const float vel_limit = cfg.cell / dt;
+                    const float spd = len2(nvx, nvy);
+                    if (spd > vel_limit) {
+                        const float kk = vel_limit / spd;
+                        nvx = nvx * kk;
+                        nvy = nvy * kk;
+                        ++n_clamp;
+                    }
+                    float nx = s.x + nvx * dt, ny = s.y + nvy * dt;
+                    for (uint32_t q = 0; q < cfg.n_slip; ++q) {
+                        nx = rmin(rmax(nx, cfg.slip_lo[q]), cfg.slip_hi[q]);
+                        ny = rmin(rmax(ny, cfg.slip_lo[q]), cfg.slip_hi[q]);
+                    }
+                    s.vx = nvx;
+                    s.vy = nvy;
+                    s.c = M2{(b.a * kDInv) * cfg.apic_blend, (b.b * kDInv) * cfg.apic_blend,
+                             (b.c * kDInv) * cfg.apic_blend, (b.d * kDInv) * cfg.apic_blend};
+                    s.x = nx;
+                    s.y = ny;
+                }
+                // phase 2: update_particle (g2p.rs:333-340)
+                update_particle<ONLY>(mat_of(p, s.mat), s, dt);
+            }
+            // 4. the state the next substep starts from: CFL, projection check
+            cfl_of<ONLY>(p, s, speed, mdt);
+            any_dirty |= needs_projection(s, sp, cfg);
+            if (G2P) {
+                const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
+                store_fields(o, dest, s, mask, full);
+                if (reorder) o.id[dest] = a.id[j];
+            }
+        }
+        // 5. P2G of the projected state into the next accumulator set
+        if (P2G) {
+            const P2GTerms t = p2g_terms<ONLY>(p, sp);
+            scatter_p2g(t, valid, box, s_acc, gdst, R);
+        } else {
+            P2GTerms t;
+            t.w = weights(sp.x, sp.y);
+            t.mass = sp.mass;
+            const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
+            scatter_channel<kAccM>(t, valid, sm, box, s_acc[0], gdst[0], R);
+        }
+    }
+
+    // 6. flush the LDS box into the next accumulator set
+    flush_box<NCH>(box, s_acc, gdst, R);
+    // 7. clear this workgroup's slice of the set the substep after next uses
+    if (G2P) {
+        const uint32_t R2 = (uint32_t)R * (uint32_t)R;
+        const uint32_t per = (R2 + gridDim.x - 1) / gridDim.x;
+        const uint32_t lo = blockIdx.x * per, hi = min(R2, lo + per);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+#pragma unroll
+            for (int ch = 0; ch < kAccChannels; ++ch) p.grid.acc[old][ch][i] = 0.0f;
+            if (cfg.density_pass) p.grid.dm[old][i] = 0.0f;
+        }
+    }
+    // 8. counters and the CFL reduction the next launch decides from
+    const uint32_t wc = wave_sum(n_clamp), wp = wave_sum(n_proj);
+    const bool wd = __ballot(any_dirty) != 0ull;
+    if (lane_id() == 0) {
+        if (wc) atomicAdd(&c->clamp_count, wc);
+        if (wp) atomicAdd(&c->proj_count, wp);
+        if (wd) atomicOr(&red->dirty, 1u);
+    }
+    reduce_cfl(red, speed, mdt, s_red);
+}
+
+// Density re-estimate (density.rs:216-247) from the mass grid the previous
+// substep scattered, then this substep's P2G.  This kernel is the first to
+// read the substep's state, so it also projects (and persists) it if flagged.
+__global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64_t seq) {
+    __shared__ float s_acc[kAccChannels][kLdsCells];
+    __shared__ float s_dm[kLdsCells];
+    __shared__ int s_box[4 * kWaves];
+    Ctrl* c = p.ctrl;
+    const DevConfig& cfg = p.cfg;
+    // same launch number as the substep kernel that follows: same decision
+    const CtrlState P = c->st[seq & 1u];
+    const CtrlRed Q = c->red[seq_mod3(seq + 2u)];
+    const CtrlState N = decide_substep(P, Q, cfg);
+    if (!N.active) return;
+    const int R = (int)cfg.grid_res;
+    const uint32_t cur = N.cur_grid;
+    const bool project_in = cfg.project && Q.dirty;
+    const uint32_t mask = project_in ? kFAll : (cfg.fields | kFVolDens);
+    const PBuf a = sel_buf(p, P.cur_buf);
+    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t base = chunk * kChunk;
+    const float* dm = p.grid.dm[cur];
+    float* gdst[kAccChannels];
+#pragma unroll
+    for (int ch = 0; ch < kAccChannels; ++ch) gdst[ch] = p.grid.acc[cur][ch];
+
+    const Box box = chunk_box(p, a, base, project_in, 1, s_box);
+    if (box.lds) {
+        for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
+#pragma unroll
+            for (int ch = 0; ch < kAccChannels; ++ch) s_acc[ch][i] = 0.0f;
+            const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
+            s_dm[i] = (cx >= 0 && cy >= 0 && cx < R && cy < R) ? dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] : 0.0f;
+        }
+        __syncthreads();
+    }
+    uint32_t n_proj = 0;
+#pragma unroll 1
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t j = base + k * kBlock + threadIdx.x;
+        const bool valid = j < p.n;
+        PState s = neutral_state();
+        if (valid) {
+            const float2 x = a.x[j];
+            s.x = x.x;
+            s.y = x.y;
+            load_fields(s, a, j, mask, true);
+            if (project_in && project_state(s, cfg)) ++n_proj;
             const Wts w = weights(s.x, s.y);
-            bxc = w.bx;
-            byc = w.by;
-#pragma unroll
+            const bool lds = box.holds3x3(w.bx, w.by);
+            float density = 0.0f;
             for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
                 for (int gy = 0; gy < 3; ++gy) {
                     const float weight = w.wx[gx] * w.wy[gy];
-                    const float dx = ((float)(w.bx + gx - 1) - s.x) + 0.5f;
-                    const float dy = ((float)(w.by + gy - 1) - s.y) + 0.5f;
-                    const float vcx = s.vx + (s.c.a * dx + s.c.c * dy);
-                    const float vcy = s.vy + (s.c.b * dx + s.c.d * dy);
-                    const float sdx = tau.a * dx + tau.c * dy;
-                    const float sdy = tau.b * dx + tau.d * dy;
-                    vals[gx * 3 + gy] = weight * s.mass;
-                    vals[9 + gx * 3 + gy] = weight * (s.mass * vcx + coeff * sdx);
-                    vals[18 + gx * 3 + gy] = weight * (s.mass * vcy + coeff * sdy);
+                    const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
+                    float m = 0.0f;
+                    if (lds)
+                        m = s_dm[box.index(cx, cy)];
+                    else if (cx >= 0 && cy >= 0 && cx < R && cy < R)
+                        m = dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy];
+                    density = density + m * weight;
                 }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 27; ++q) vals[q] = 0.0f;
-        }
-        const SegMask sm = run_mask(bxc, byc);
-        seg_scan(vals, sm);
-        if (valid && run_tail(sm)) {
-            for (int gx = 0; gx < 3; ++gx)
-                for (int gy = 0; gy < 3; ++gy) {
-                    const int cx = bxc + gx - 1, cy = byc + gy - 1;
-                    const float mm = vals[gx * 3 + gy], mx = vals[9 + gx * 3 + gy], my = vals[18 + gx * 3 + gy];
-                    if (bx.lds) {
-                        const int li = (cx - bx.ox) * bx.h + (cy - bx.oy);
-                        atomicAdd(&s_m[li], mm);
-                        atomicAdd(&s_mx[li], mx);
-                        atomicAdd(&s_my[li], my);
-                    } else if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) {
-                        const uint32_t gi = (uint32_t)cx * R + (uint32_t)cy;
-                        gatomic_add(&gm[gi], mm);
-                        gatomic_add(&gmx[gi], mx);
-                        gatomic_add(&gmy[gi], my);
-                    }
-                }
-        }
-    }
-    if (bx.lds) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < bx.w * bx.h; i += kBlock) {
-            float m = s_m[i], mx = s_mx[i], my = s_my[i];
-            if (m == 0.0f && mx == 0.0f && my == 0.0f) continue;
-            int cx = bx.ox + i / bx.h, cy = bx.oy + i % bx.h;
-            if (cx < 0 || cy < 0 || cx >= (int)R || cy >= (int)R) continue;
-            uint32_t gi = (uint32_t)cx * R + (uint32_t)cy;
-            if (m != 0.0f) gatomic_add(&gm[gi], m);
-            if (mx != 0.0f) gatomic_add(&gmx[gi], mx);
-            if (my != 0.0f) gatomic_add(&gmy[gi], my);
-        }
-    }
-}
-
-// Grid update (normalise, gravity, slip walls, speed clamp) and clears.
-__global__ __launch_bounds__(kBlock) void k_grid_update(const Pipeline p) {
-    const Ctrl* c = p.ctrl;
-    if (!c->active) return;
-    const DevConfig& cfg = p.cfg;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t R = cfg.grid_res, R2 = R * R;
-    if (i < cfg.n_bins) p.sort.cell_count[i] = 0;
-    if (i < cfg.tiles_per_dim * cfg.tiles_per_dim) p.sort.tile_count[i] = 0;
-    if (i >= R2) return;
-    const uint32_t g = c->cur_grid;
-    const float dt = c->dt;
-    float* gm = g ? p.grid.m[1] : p.grid.m[0];
-    float* gmx = g ? p.grid.mx[1] : p.grid.mx[0];
-    float* gmy = g ? p.grid.my[1] : p.grid.my[0];
-    float m = gm[i], vx = gmx[i], vy = gmy[i];
-    if (m > 0.0f) {
-        vx = vx / m;
-        vy = vy / m;
-        vx = vx + cfg.gx * dt;
-        vy = vy + cfg.gy * dt;
-        const uint32_t x = i / R, y = i % R;
-        for (uint32_t b = 0; b < cfg.n_slip; ++b) {
-            const uint32_t t = cfg.slip[b];
-            const uint32_t hi = R - (t + 1);
-            if (x < t) vx = rmax(vx, 0.0f);
-            if (x > hi) vx = rmin(vx, 0.0f);
-            if (y < t) vy = rmax(vy, 0.0f);
-            if (y > hi) vy = rmin(vy, 0.0f);
-        }
-        const float vel_limit = cfg.cell / dt;
-        float spd = len2(vx, vy);
-        if (spd > vel_limit) {
-            float k = vel_limit / spd;
-            vx = vx * k;
-            vy = vy * k;
-        }
-    }
-    p.grid.vel[i] = make_float2(vx, vy);
-    // clear the accumulators the next substep's P2G (and density pass) will use
-    float* om = g ? p.grid.m[0] : p.grid.m[1];
-    float* omx = g ? p.grid.mx[0] : p.grid.mx[1];
-    float* omy = g ? p.grid.my[0] : p.grid.my[1];
-    om[i] = 0.0f;
-    omx[i] = 0.0f;
-    omy[i] = 0.0f;
-    if (cfg.density_pass) p.grid.dmass[i] = 0.0f;
-}
-
-// G2P + constitutive update for the particle at each sorted slot.
-template <bool PROJECT>
-__global__ __launch_bounds__(kBlock) void k_g2p(const Pipeline p) {
-    __shared__ float s_red[2 * (kBlock / 64)];
-    Ctrl* c = p.ctrl;
-    if (!c->active) return;
-    const DevConfig& cfg = p.cfg;
-    const uint32_t wb = c->cur_buf;  // output buffer (state after this substep)
-    const PBuf a = sel_buf(p, wb ^ 1u);
-    const PBuf o = sel_buf(p, wb);
-    const float dt = c->dt;
-    const uint32_t R = cfg.grid_res;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < p.n;
-    PState s{};
-    uint32_t dest = 0;
-    if (valid) {
-        // stable position of this slot's particle inside its bin: the bin is
-        // ordered by the particles' positions in the input buffer
-        const uint32_t e = p.sort.perm[t];
-        const uint32_t key = a.key[e];
-        const uint32_t start = p.sort.cell_start[key];
-        const uint32_t cnt = p.sort.cell_start[key + 1] - start;
-        uint32_t rank = 0;
-        for (uint32_t u = 0; u < cnt; ++u) rank += (p.sort.perm[start + u] < e) ? 1u : 0u;
-        dest = start + rank;
-
-        s = load_state(a, e);
-        if (PROJECT && cfg.project) {
-            if (project_state(s, cfg)) atomicAdd(&c->proj_count, 1u);
-        }
-        // gather_grid_to_particles phase 1 (g2p.rs:203-325)
-        Wts w = weights(s.x, s.y);
-        float nvx = 0.0f, nvy = 0.0f;
-        M2 b{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int gx = 0; gx < 3; ++gx)
-            for (int gy = 0; gy < 3; ++gy) {
-                float weight = w.wx[gx] * w.wy[gy];
-                int cx = w.bx + gx - 1, cy = w.by + gy - 1;
-                float dx = ((float)cx - s.x) + 0.5f;
-                float dy = ((float)cy - s.y) + 0.5f;
-                float2 nv = make_float2(0.0f, 0.0f);
-                if (cx >= 0 && cy >= 0 && cx < (int)R && cy < (int)R) nv = p.grid.vel[(uint32_t)cx * R + cy];
-                float wvx = nv.x * weight, wvy = nv.y * weight;
-                b.a = b.a + wvx * dx;
-                b.b = b.b + wvy * dx;
-                b.c = b.c + wvx * dy;
-                b.d = b.d + wvy * dy;
-                nvx = nvx + wvx;
-                nvy = nvy + wvy;
+            if (density > kF32Eps) {
+                s.dens = density;
+                s.vol = s.mass / density;
             }
-        if (s.mat & kPinnedBit) {
-            s.vx = 0.0f;
-            s.vy = 0.0f;
-            s.c = M2{0.0f, 0.0f, 0.0f, 0.0f};
-        } else {
-            const float vel_limit = cfg.cell / dt;
-            float spd = len2(nvx, nvy);
-            if (spd > vel_limit) {
-                float k = vel_limit / spd;
-                nvx = nvx * k;
-                nvy = nvy * k;
-                atomicAdd(&c->clamp_count, 1u);
+            if (project_in) {
+                store_fields(a, j, s, kFAll, true);  // persist the projection
+            } else {
+                a.dens[j] = s.dens;
+                a.vol[j] = s.vol;
             }
-            float nx = s.x + nvx * dt, ny = s.y + nvy * dt;
-            for (uint32_t q = 0; q < cfg.n_slip; ++q) {
-                nx = rmin(rmax(nx, cfg.slip_lo[q]), cfg.slip_hi[q]);
-                ny = rmin(rmax(ny, cfg.slip_lo[q]), cfg.slip_hi[q]);
-            }
-            s.vx = nvx;
-            s.vy = nvy;
-            s.c = M2{(b.a * kDInv) * cfg.apic_blend, (b.b * kDInv) * cfg.apic_blend, (b.c * kDInv) * cfg.apic_blend,
-                     (b.d * kDInv) * cfg.apic_blend};
-            s.x = nx;
-            s.y = ny;
         }
-        // phase 2: update_particle (g2p.rs:333-340)
-        update_particle(mat_of(p, s.mat), s, dt);
-        store_state(o, dest, s);
-        o.id[dest] = a.id[e];
+        const P2GTerms t = p2g_terms<-1>(p, s);
+        scatter_p2g(t, valid, box, s_acc, gdst, R);
     }
-    finish_particle(p, s, valid, o, dest, s_red);
+    flush_box<kAccChannels>(box, s_acc, gdst, R);
+    const uint32_t wp = wave_sum(n_proj);
+    if (lane_id() == 0 && wp) atomicAdd(&c->proj_count, wp);
 }
 
 // ------------------------------------------------- stand-alone cell sort
@@ -858,6 +1003,7 @@ __global__ void k_sort_scatter(uint32_t n, const uint32_t* keys, const uint32_t*
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) perm_unordered[cell_start[keys[i]] + rank[i]] = i;
 }
+// Stable order inside a bin: rank of each element among its bin by input index.
 __global__ void k_sort_finalize(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
                                 const uint32_t* perm_unordered, uint32_t* perm_sorted) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -876,72 +1022,82 @@ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------------ launchers
 
-hipError_t launch_zero_counters(const Pipeline& p, hipStream_t stream) {
-    uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
-    uint32_t n = R2 > p.cfg.n_bins ? R2 : p.cfg.n_bins;
-    k_zero_counters<<<div_up(n, kBlock), kBlock, 0, stream>>>(p);
-    return hipGetLastError();
-}
+const char* const kKernelNames[kKernelCount] = {"k_begin_step", "k_keys",        "k_tile_scan",
+                                                  "k_cell_scan",  "k_density_p2g", "k_substep"};
 
-hipError_t launch_prologue(const Pipeline& p, hipStream_t stream) {
-    if (p.n == 0) return hipSuccess;
-    k_prologue<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p);
-    return hipGetLastError();
-}
-
-hipError_t launch_begin_step(const Pipeline& p, hipStream_t stream) {
-    k_begin_step<<<1, 1, 0, stream>>>(p);
-    return hipGetLastError();
-}
-
-const char* const kKernelNames[kKernelCount] = {"k_control",        "k_cell_scan", "k_density_scatter",
-                                                  "k_density_gather", "k_p2g",       "k_grid_update",
-                                                  "k_g2p"};
-
-hipError_t launch_substep(const Pipeline& p, hipStream_t stream, SlotEvents* events) {
-    if (p.n == 0) return hipSuccess;
-    const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
+hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream) {
     const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
-    const uint32_t cells = R2 > p.cfg.n_bins ? R2 : p.cfg.n_bins;
-    const bool dens = p.cfg.density_pass != 0;
-    auto mark = [&](int id, int which) {
-        if (!events) return;
-        (void)hipEventRecord(events->ev[id][which], stream);
-        events->used[id] = true;
-    };
-    if (events)
-        for (int k = 0; k < kKernelCount; ++k) events->used[k] = false;
-    mark(kKControl, 0);
-    k_control<<<1, 1024, 0, stream>>>(p);
-    mark(kKControl, 1);
-    mark(kKCellScan, 0);
-    k_cell_scan<<<n_tiles, kTileCells, 0, stream>>>(p);
-    mark(kKCellScan, 1);
-    if (dens) {
-        mark(kKDensityScatter, 0);
-        k_density_scatter<<<div_up(p.n, kChunk), kBlock, 0, stream>>>(p);
-        mark(kKDensityScatter, 1);
-        mark(kKDensityGather, 0);
-        k_density_gather<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p);
-        mark(kKDensityGather, 1);
-        mark(kKP2G, 0);
-        k_p2g<false><<<div_up(p.n, kChunk), kBlock, 0, stream>>>(p);
-        mark(kKP2G, 1);
-    } else {
-        mark(kKP2G, 0);
-        k_p2g<true><<<div_up(p.n, kChunk), kBlock, 0, stream>>>(p);
-        mark(kKP2G, 1);
+    const uint32_t n = R2 > p.cfg.n_bins ? R2 : p.cfg.n_bins;
+    k_zero_all<<<div_up(n, kBlock), kBlock, 0, stream>>>(p);
+    return hipGetLastError();
+}
+
+hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream) {
+    const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
+    if (p.cfg.density_pass)
+        k_substep<kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    else
+        k_substep<kModeP2G, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    return hipGetLastError();
+}
+
+namespace {
+struct Marker {
+    SlotEvents* ev;
+    hipStream_t stream;
+    void operator()(int id, int which) const {
+        if (!ev) return;
+        (void)hipEventRecord(ev->ev[id][which], stream);
+        ev->used[id] = true;
     }
-    mark(kKGridUpdate, 0);
-    k_grid_update<<<div_up(cells, kBlock), kBlock, 0, stream>>>(p);
-    mark(kKGridUpdate, 1);
-    mark(kKG2P, 0);
-    if (dens) {
-        k_g2p<false><<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p);
-    } else {
-        k_g2p<true><<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p);
+};
+}  // namespace
+
+hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream,
+                             SlotEvents* events) {
+    Marker mark{events, stream};
+    reorder = reorder && p.n > 0;
+    mark(kKBegin, 0);
+    k_begin_step<<<1, 1, 0, stream>>>(p, seq, reorder ? 1u : 0u);
+    mark(kKBegin, 1);
+    if (reorder) {
+        const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
+        mark(kKKeys, 0);
+        k_keys<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p, seq + 1u);
+        mark(kKKeys, 1);
+        mark(kKTileScan, 0);
+        k_tile_scan<<<1, 1024, 0, stream>>>(p);
+        mark(kKTileScan, 1);
+        mark(kKCellScan, 0);
+        k_cell_scan<<<n_tiles, kTileCells, 0, stream>>>(p);
+        mark(kKCellScan, 1);
     }
-    mark(kKG2P, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, SlotEvents* events) {
+    Marker mark{events, stream};
+    const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
+    if (p.cfg.density_pass) {
+        mark(kKDensityP2G, 0);
+        k_density_p2g<<<nb, kBlock, 0, stream>>>(p, seq);
+        mark(kKDensityP2G, 1);
+        mark(kKSubstep, 0);
+        k_substep<kModeG2P | kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+        mark(kKSubstep, 1);
+    } else {
+        constexpr int kFused = kModeG2P | kModeP2G;
+        mark(kKSubstep, 0);
+        switch (p.cfg.only_model) {
+            case kModelNH: k_substep<kFused, kModelNH><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            case kModelCorot: k_substep<kFused, kModelCorot><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            case kModelSnow: k_substep<kFused, kModelSnow><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            case kModelDP: k_substep<kFused, kModelDP><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            case kModelNacc: k_substep<kFused, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            default: k_substep<kFused, -1><<<nb, kBlock, 0, stream>>>(p, seq); break;
+        }
+        mark(kKSubstep, 1);
+    }
     return hipGetLastError();
 }
 

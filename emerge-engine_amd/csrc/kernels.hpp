@@ -9,13 +9,15 @@
 namespace emh {
 
 constexpr int kBlock = 256;        // threads per workgroup for particle kernels (4 waves of 64)
-constexpr int kChunk = 1024;       // particles per P2G workgroup (4 per thread)
-constexpr int kLdsCells = 3072;    // P2G bounding-box budget in LDS cells (3 floats each: 36 KiB)
+constexpr int kWaves = kBlock / 64;
+constexpr int kChunk = 1024;       // particles per substep workgroup (4 per thread)
+constexpr int kPer = kChunk / kBlock;
+constexpr int kLdsCells = 1024;    // LDS box budget in grid cells (7 floats each: 28 KiB)
 constexpr int kTileCells = 256;    // 16 x 16 cells per sort tile
+constexpr uint32_t kReorderSubsteps = 8;  // physical cell-key reorder at least this often
 
-// Particle state, structure of arrays.  Two of these ping-pong: a substep
-// reads one (in the order the previous substep left it) and writes the other
-// in cell-key order.
+// Particle state, structure of arrays.  Two of these ping-pong only when the
+// particles are physically reordered; other substeps update in place.
 struct PBuf {
     float2* x;
     float2* v;
@@ -29,45 +31,64 @@ struct PBuf {
     float* h;    // hardening_scale
     float* q;    // friction_hardening
     float* lvs;  // log_volume_strain
-    uint32_t* mat;   // material_id | pinned bit
-    uint32_t* id;    // index of the particle in the caller's order
-    uint32_t* key;   // cell key of the (projected) position, for the next sort
-    uint32_t* rank;  // arrival rank of the particle inside its key bin
+    uint32_t* mat;  // material_id | pinned bit
+    uint32_t* id;   // index of the particle in the caller's order
 };
 
+// Dense grid, x-major (cell (x, y) at x * R + y).  Three rotating sets of P2G
+// accumulators: the substep being gathered reads one, the particle kernel
+// scatters the next substep's P2G into the second and clears the third.
+constexpr int kAccM = 0, kAccPx = 1, kAccPy = 2, kAccSx = 3, kAccSy = 4, kAccChannels = 5;
 struct GridBuf {
-    float* m[2];  // P2G mass, ping-pong by substep parity
-    float* mx[2];
-    float* my[2];
-    float2* vel;  // grid velocity after the update (what G2P gathers)
-    float* dmass; // mass of the density re-estimate pass
+    // mass, momentum (m v + m C d), and the stress moment (-4 V0 tau d) that
+    // the grid update scales by the substep dt once the CFL reduction is done
+    float* acc[3][kAccChannels];
+    float* dm[3];  // mass grid of the density re-estimate pass
 };
 
 struct SortBuf {
+    uint32_t* key;         // N: cell key of the particle in each slot
+    uint32_t* rank;        // N: arrival rank inside its key bin
     uint32_t* cell_count;  // n_bins
-    uint32_t* cell_start;  // n_bins + 1
+    uint32_t* cell_start;  // n_bins
     uint32_t* tile_count;  // n_tiles
-    uint32_t* tile_start;  // n_tiles + 1
-    uint32_t* perm;        // N: particle at each slot, unordered inside a bin
-    float2* cfl_partial;   // per G2P block: (max speed, min material dt)
+    uint32_t* tile_start;  // n_tiles
 };
 
-// Device-resident control block: the adaptive substep loop runs on the
-// device so that no substep needs a host round trip.
-struct Ctrl {
-    float remaining;      // simulated time left in the current step
-    float dt;             // current substep dt
-    uint32_t substeps;    // substeps taken in the current step
-    uint32_t active;      // 1 while the current substep slot is live
-    uint32_t clamp_count; // G2P velocity clamps in the current step
-    uint32_t proj_count;  // state projections in the current step
-    uint32_t total_substeps;
+// Device-resident control: the adaptive substep loop runs on the device so
+// that no substep needs a host round trip, and no kernel synchronises its
+// workgroups with each other.  Every launch that takes part in the loop has a
+// sequence number s (counted by the host).  Launch s reads st[s & 1] (written
+// by launch s - 1) and the reduction red[(s + 2) % 3] of launch s - 1; every
+// workgroup derives the same decision for its substep from those, workgroup 0
+// alone writes st[(s + 1) & 1] and clears red[(s + 1) % 3], and all
+// workgroups reduce into red[s % 3] with atomics.
+struct CtrlState {
+    float remaining;   // simulated time left in the current step
+    float dt;          // dt of the last decided substep
     float last_dt;
-    uint32_t cur_buf;     // which PBuf holds the current state
-    uint32_t cur_grid;    // which grid accumulator the current substep uses
-    uint32_t n_cfl_blocks;
-    uint32_t pad[5];
+    uint32_t substeps;  // substeps taken in the current step
+    uint32_t active;    // the launch that wrote this state ran a substep
+    uint32_t total_substeps;
+    uint32_t cur_buf;   // which PBuf holds the state
+    uint32_t cur_grid;  // accumulator set the last substep gathered from (0..2)
+    uint32_t reorder;   // the next substep writes the particles in cell-key order
+    uint32_t pad[3];
 };
+struct CtrlRed {
+    uint32_t max_speed;  // f32 bits (non-negative: ordered as integers)
+    uint32_t min_mdt;    // f32 bits (positive or +inf)
+    uint32_t dirty;      // some stored particle is changed by projection
+    uint32_t pad;
+};
+struct Ctrl {
+    CtrlState st[2];
+    CtrlRed red[3];
+    uint32_t clamp_count;  // G2P velocity clamps in the current step
+    uint32_t proj_count;   // state projections in the current step
+    uint32_t pad[2];
+};
+constexpr uint32_t kInfBits = 0x7f800000u;
 
 struct Pipeline {
     DevConfig cfg;
@@ -80,15 +101,14 @@ struct Pipeline {
     Ctrl* ctrl;
 };
 
-// Kernels of one substep, in launch order (profiling identifiers).
+// Kernels, in launch order (profiling identifiers).
 enum KernelId : int {
-    kKControl = 0,
+    kKBegin = 0,
+    kKKeys,
+    kKTileScan,
     kKCellScan,
-    kKDensityScatter,
-    kKDensityGather,
-    kKP2G,
-    kKGridUpdate,
-    kKG2P,
+    kKDensityP2G,
+    kKSubstep,
     kKernelCount
 };
 extern const char* const kKernelNames[kKernelCount];
@@ -99,13 +119,15 @@ struct SlotEvents {
     bool used[kKernelCount];
 };
 
-// Launch wrappers (all asynchronous on `stream`).
-hipError_t launch_zero_counters(const Pipeline& p, hipStream_t stream);
-hipError_t launch_prologue(const Pipeline& p, hipStream_t stream);
-hipError_t launch_begin_step(const Pipeline& p, hipStream_t stream);
-hipError_t launch_substep(const Pipeline& p, hipStream_t stream, SlotEvents* events = nullptr);
+// Launch wrappers (all asynchronous on `stream`); `seq` is the launch's
+// sequence number in the device-side loop (see Ctrl).
+hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream);
+hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream);
+hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream,
+                             SlotEvents* events = nullptr);
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, SlotEvents* events = nullptr);
 
-// Stand-alone cell-key sort (same kernels the substep uses).
+// Stand-alone cell-key sort (bit-exact stable counting sort).
 hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys,
                                  uint32_t* perm_sorted, uint32_t* scratch_rank, uint32_t* cell_count,
                                  uint32_t* cell_start, uint32_t* tile_count, uint32_t* tile_start,

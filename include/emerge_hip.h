@@ -298,8 +298,10 @@ int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint3
  * No CPU counterpart in the reference (its WGSL sort is a coarse 16x16-block
  * counting sort with a non-deterministic order inside a block,
  * src/systems/gpu/shaders/particle_sort.wgsl:94-245).  This is the device
- * sort the substep uses: key = emh_cell_key(floor(x), floor(y)), stable by
- * input index.  keys_out[i] is the key of input i; perm_out[j] is the input
+ * cell key the substep reorders particles by: key = emh_cell_key(floor(x),
+ * floor(y)) -- 16x16-cell tiles numbered column by column with odd columns
+ * reversed (serpentine), then 16*lx + ly inside the tile -- stable by input
+ * index.  keys_out[i] is the key of input i; perm_out[j] is the input
  * index at sorted position j. */
 uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,

@@ -110,6 +110,31 @@ def test_cell_key_matches_oracle(oracle):
         assert eh.cell_key(cx, cy, r) == oracle.cell_key(cx, cy, r)
 
 
+@pytest.mark.parametrize("r", [16, 64, 100])
+def test_cell_key_layout(r):
+    """Properties that fix the key without restating it: a bijection from the
+    R x R cells into [0, 256 * ceil(R/16)^2); 256 consecutive keys per 16x16
+    tile; consecutive tiles share an edge (serpentine), so a run of keys is a
+    compact patch of cells."""
+    keys = np.array([[eh.cell_key(x, y, r) for y in range(r)] for x in range(r)])
+    tpd = (r + 15) // 16
+    assert len(np.unique(keys)) == r * r and keys.min() >= 0 and keys.max() < 256 * tpd * tpd
+    tiles = keys // 256
+    for tx in range(tpd):
+        for ty in range(tpd):
+            blk = tiles[16 * tx:16 * tx + 16, 16 * ty:16 * ty + 16]
+            assert np.all(blk == blk.flat[0])
+    origin = {}
+    for tx in range(tpd):
+        for ty in range(tpd):
+            origin[int(tiles[16 * tx, 16 * ty])] = (tx, ty)
+    order = [origin[t] for t in sorted(origin)]
+    for (ax, ay), (bx, by) in zip(order, order[1:]):
+        assert abs(ax - bx) + abs(ay - by) == 1
+    # clamped outside the grid
+    assert eh.cell_key(-3, 5, r) == eh.cell_key(0, 5, r) and eh.cell_key(r + 9, r, r) == eh.cell_key(r - 1, r - 1, r)
+
+
 def _no_gpu():
     return eh.device_count() == 0
 
