@@ -576,6 +576,7 @@ struct emh_sim {
         ctrl.st[0].total_substeps = stats.total_substeps;
         ctrl.st[0].last_dt = stats.last_step_dt;
         ctrl.st[0].dt = stats.last_step_dt;
+        ctrl.st[0].free_dt = cfg.dt;
         for (auto& r : ctrl.red) {
             r.max_speed = 0u;
             r.min_mdt = kInfBits;
@@ -1153,7 +1154,7 @@ int emh_sim_get_grid(emh_sim* s, float* mass, float* momentum, size_t n_cells) {
     HIP_TRY(hipMemcpyAsync(&ctrl, s->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     // the accumulators the last substep's grid update read; momentum as that
-    // update formed it: m v + m C d plus dt times the stress moment
+    // update formed it: A + (dt - dt') S (see GridBuf)
     const CtrlState st = ctrl.st[s->seq & 1u];
     const uint32_t g = st.cur_grid % 3u;
     if (mass) HIP_TRY(hipMemcpy(mass, s->pipe.grid.acc[g][kAccM], sizeof(float) * R2, hipMemcpyDeviceToHost));
@@ -1163,10 +1164,10 @@ int emh_sim_get_grid(emh_sim* s, float* mass, float* momentum, size_t n_cells) {
             ch[k].resize(R2);
             HIP_TRY(hipMemcpy(ch[k].data(), s->pipe.grid.acc[g][kAccPx + k], sizeof(float) * R2, hipMemcpyDeviceToHost));
         }
-        const float dt = st.last_dt;
+        const float ddt = st.last_ddt;
         for (size_t i = 0; i < R2; ++i) {
-            momentum[2 * i] = ch[0][i] + dt * ch[2][i];
-            momentum[2 * i + 1] = ch[1][i] + dt * ch[3][i];
+            momentum[2 * i] = ch[0][i] + ddt * ch[2][i];
+            momentum[2 * i + 1] = ch[1][i] + ddt * ch[3][i];
         }
     }
     return EMH_OK;

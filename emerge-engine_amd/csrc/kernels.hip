@@ -24,10 +24,11 @@
 //                    4. the NEXT substep's P2G (transfer/p2g.rs:24-72) from the
 //                       new state held in registers, accumulated in LDS with
 //                       wave-level segmented sums and flushed with one global
-//                       f32 atomic per touched cell and channel.  The stress
-//                       term is kept apart from the momentum (its coefficient
-//                       -4 V0 dt (p2g.rs:42-44) is applied without dt) because
-//                       the next dt is not known until every particle is done;
+//                       f32 atomic per touched cell and channel.  The next dt
+//                       is not known until every particle is done, so the
+//                       stress coefficient -4 V0 dt (p2g.rs:42-44) uses a
+//                       predicted dt' and the stress moment is kept beside the
+//                       momentum for the grid update's (dt - dt') correction;
 //                    5. the CFL contribution (max speed, min material dt) of
 //                       the new state, reduced with integer atomics on the
 //                       f32 bit patterns (both are non-negative).
@@ -269,7 +270,7 @@ __device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const 
     }
     const float a = __uint_as_float(Q.max_speed), d = __uint_as_float(Q.min_mdt);
     const float remaining = P.remaining;
-    float dt;
+    float dt, free = cfg.dt;
     if (!cfg.adaptive) {
         dt = rmin(remaining, cfg.dt);
     } else {
@@ -278,8 +279,12 @@ __device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const 
         if (a > kF32Eps) dt = rmin(dt, cfg.cfl_coefficient * cfg.cell / a);
         dt = rmin(dt, min_mat_dt);
         dt = rclamp(dt, rmin(cfg.min_dt, remaining), remaining);
+        free = d;
+        if (a > kF32Eps) free = rmin(free, cfg.cfl_coefficient * cfg.cell / a);
+        free = rmax(free, cfg.min_dt);
     }
     N.dt = dt;
+    N.free_dt = free;
     N.last_dt = dt;
     N.active = 1;
     N.substeps = P.substeps + 1;
@@ -289,6 +294,16 @@ __device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const 
     N.cur_buf = P.cur_buf ^ (P.reorder ? 1u : 0u);
     N.reorder = 0;
     return N;
+}
+// Predicted dt of the substep after the one state N describes: the same
+// clamps as decide_substep with the CFL bound of N's own decision (the CFL
+// reduction of the new state is not known yet).  Exact for fixed dt and for
+// a step's last, time-limited substep.
+__device__ __forceinline__ float predict_dt(const CtrlState& N, const DevConfig& cfg) {
+    const float rem = (N.remaining > kF32Eps && N.substeps < cfg.max_substeps) ? N.remaining : cfg.dt;
+    if (!cfg.adaptive) return rmin(rem, cfg.dt);
+    float dt = rmin(rem, N.free_dt);
+    return rclamp(dt, rmin(cfg.min_dt, rem), rem);
 }
 __device__ __forceinline__ void reset_red(CtrlRed& r) {
     r.max_speed = 0u;
@@ -301,15 +316,16 @@ __device__ __forceinline__ uint32_t seq_mod3(uint64_t s) { return (uint32_t)(s %
 // walls and the speed clamp (grid/mod.rs:264-291, step.rs:140-165), from the
 // P2G accumulators of the current substep; (0, 0) for untouched cells
 // (velocity_at of a missing cell, grid/mod.rs:215-233).
-__device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set, float dt, int cx, int cy) {
+__device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set, float dt, float ddt, int cx, int cy) {
     const DevConfig& cfg = p.cfg;
     const int R = (int)cfg.grid_res;
     if (cx < 0 || cy < 0 || cx >= R || cy >= R) return make_float2(0.0f, 0.0f);
     const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
     const float m = p.grid.acc[set][kAccM][gi];
     if (!(m > 0.0f)) return make_float2(0.0f, 0.0f);
-    float vx = p.grid.acc[set][kAccPx][gi] + dt * p.grid.acc[set][kAccSx][gi];
-    float vy = p.grid.acc[set][kAccPy][gi] + dt * p.grid.acc[set][kAccSy][gi];
+    // momentum = A + (dt - dt') S (ddt = dt - dt' is 0 when the prediction held)
+    float vx = p.grid.acc[set][kAccPx][gi] + ddt * p.grid.acc[set][kAccSx][gi];
+    float vy = p.grid.acc[set][kAccPy][gi] + ddt * p.grid.acc[set][kAccSy][gi];
     vx = vx / m;
     vy = vy / m;
     vx = vx + cfg.gx * dt;
@@ -336,9 +352,14 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
 // ---------------------------------------------------- wave segmented sums
 // Lanes of a wave hold consecutive particles of a cell-sorted chunk, so runs
 // of lanes share a base cell and with it all nine stencil cells.  A run's
-// contributions are summed across its lanes with DPP row shifts and row
-// broadcasts (VALU work, no LDS traffic), leaving each run's total in its
-// last lane, which alone issues the atomics.
+// contributions are summed across its lanes with a segmented inclusive scan
+// of six DPP steps (row shifts 1/2/4/8 inside 16-lane rows, then row
+// broadcasts 15 and 31; VALU work, no LDS traffic), leaving each run's total
+// in its last lane, which alone issues the atomics.  A step adds the shifted
+// partial sum only if the source lane is in the same run: as fma(t, p, v)
+// with p = 1 or 0, which is exactly v + t or v when t is finite (2
+// instructions), or as a select (3 instructions) for waves whose terms are
+// not all small enough for every partial sum to stay finite.
 struct SegMask {
     uint64_t heads;  // bit L: lane L starts a run
     uint32_t lane;
@@ -354,46 +375,40 @@ template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_take(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, true));
 }
-// Six-step segmented inclusive scan over the wave (row_shr 1/2/4/8 inside
-// 16-lane rows, then row_bcast 15 and 31 across rows).
-template <int N>
-__device__ __forceinline__ void seg_scan(float (&v)[N], const SegMask& m) {
+struct ScanPred {
+    bool p[6];   // per step: the source lane is in this lane's run
+    float f[6];  // the same as 1.0f / 0.0f
+};
+__device__ __forceinline__ ScanPred scan_pred(const SegMask& m) {
+    ScanPred q;
     const int l = (int)m.lane, r = l & 15;
-    const bool p1 = r >= 1 && same_run(m, l - 1);
-    const bool p2 = r >= 2 && same_run(m, l - 2);
-    const bool p4 = r >= 4 && same_run(m, l - 4);
-    const bool p8 = r >= 8 && same_run(m, l - 8);
-    const bool pb15 = (l >= 16) && same_run(m, (l & ~15) - 1) && ((l >> 4) == 1 || (l >> 4) == 3);
-    const bool pb31 = (l >= 32) && same_run(m, 31);
+    q.p[0] = r >= 1 && same_run(m, l - 1);
+    q.p[1] = r >= 2 && same_run(m, l - 2);
+    q.p[2] = r >= 4 && same_run(m, l - 4);
+    q.p[3] = r >= 8 && same_run(m, l - 8);
+    q.p[4] = (l >= 16) && same_run(m, (l & ~15) - 1) && ((l >> 4) == 1 || (l >> 4) == 3);
+    q.p[5] = (l >= 32) && same_run(m, 31);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.f[k] = q.p[k] ? 1.0f : 0.0f;
+    return q;
+}
+template <bool FMA, int CTRL, int ROWS>
+__device__ __forceinline__ float scan_step(float v, bool p, float f) {
+    const float t = dpp_take<CTRL, ROWS>(v);
+    return FMA ? __builtin_fmaf(t, f, v) : v + (p ? t : 0.0f);
+}
+template <bool FMA, int N>
+__device__ __forceinline__ void seg_scan(float (&v)[N], const ScanPred& q) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x111, 0xf>(v[i]);  // row_shr:1
-        v[i] = v[i] + (p1 ? t : 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x112, 0xf>(v[i]);  // row_shr:2
-        v[i] = v[i] + (p2 ? t : 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x114, 0xf>(v[i]);  // row_shr:4
-        v[i] = v[i] + (p4 ? t : 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x118, 0xf>(v[i]);  // row_shr:8
-        v[i] = v[i] + (p8 ? t : 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x142, 0xa>(v[i]);  // row_bcast:15 into rows 1 and 3
-        v[i] = v[i] + (pb15 ? t : 0.0f);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        float t = dpp_take<0x143, 0xc>(v[i]);  // row_bcast:31 into rows 2 and 3
-        v[i] = v[i] + (pb31 ? t : 0.0f);
+        float x = v[i];
+        x = scan_step<FMA, 0x111, 0xf>(x, q.p[0], q.f[0]);  // row_shr:1
+        x = scan_step<FMA, 0x112, 0xf>(x, q.p[1], q.f[1]);  // row_shr:2
+        x = scan_step<FMA, 0x114, 0xf>(x, q.p[2], q.f[2]);  // row_shr:4
+        x = scan_step<FMA, 0x118, 0xf>(x, q.p[3], q.f[3]);  // row_shr:8
+        x = scan_step<FMA, 0x142, 0xa>(x, q.p[4], q.f[4]);  // row_bcast:15 into rows 1 and 3
+        x = scan_step<FMA, 0x143, 0xc>(x, q.p[5], q.f[5]);  // row_bcast:31 into rows 2 and 3
+        v[i] = x;
     }
 }
 // Run structure of the base cells (bx, by) across the wave; every lane must
@@ -461,7 +476,7 @@ __device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_
 // -4 V0 (tau d) per axis that the grid update multiplies by dt.
 struct P2GTerms {
     Wts w;
-    float x, y, vx, vy, mass, coeff;
+    float x, y, vx, vy, mass, coeff, coeff_dt;  // coeff = -4 V0, coeff_dt = coeff dt'
     M2 c, tau;
 };
 template <int CH>
@@ -477,9 +492,9 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
             if (CH == kAccM) {
                 r = weight * t.mass;
             } else if (CH == kAccPx) {
-                r = weight * (t.mass * (t.vx + (t.c.a * dx + t.c.c * dy)));
+                r = weight * (t.mass * (t.vx + (t.c.a * dx + t.c.c * dy)) + t.coeff_dt * (t.tau.a * dx + t.tau.c * dy));
             } else if (CH == kAccPy) {
-                r = weight * (t.mass * (t.vy + (t.c.b * dx + t.c.d * dy)));
+                r = weight * (t.mass * (t.vy + (t.c.b * dx + t.c.d * dy)) + t.coeff_dt * (t.tau.b * dx + t.tau.d * dy));
             } else if (CH == kAccSx) {
                 r = weight * (t.coeff * (t.tau.a * dx + t.tau.c * dy));
             } else {
@@ -491,8 +506,8 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
 // Segmented wave sum of one channel, then the run tails add it to the LDS box
 // (or to the global grid when the stencil leaves the box).
 template <int CH>
-__device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, const SegMask& sm, const Box& box,
-                                                float* s_ch, float* g_ch, int R) {
+__device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, const SegMask& sm, const ScanPred& sq,
+                                                bool fma_ok, const Box& box, float* s_ch, float* g_ch, int R) {
     float v[9];
     if (valid) {
         p2g_channel<CH>(t, v);
@@ -500,7 +515,10 @@ __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, c
 #pragma unroll
         for (int q = 0; q < 9; ++q) v[q] = 0.0f;
     }
-    seg_scan(v, sm);
+    if (fma_ok)
+        seg_scan<true>(v, sq);
+    else
+        seg_scan<false>(v, sq);
     if (valid && run_tail(sm)) {
         const int bx = t.w.bx, by = t.w.by;
         if (box.holds3x3(bx, by)) {
@@ -651,10 +669,11 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
     return block_box(mnx, mny, mxx, mxy, margin, s_box);
 }
 
-// P2G terms of a (projected) particle state (p2g.rs:24-72; the stress
-// coefficient -4 V0 without dt, transfer.rs:77-104 for tau).
+// P2G terms of a (projected) particle state (p2g.rs:24-72, tau from
+// transfer.rs:77-104); the stress coefficient as p2g.rs:42-44 forms it,
+// ((-V) * 4) * dt', and without dt' for the correction moment.
 template <int ONLY>
-__device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s) {
+__device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s, float pred_dt) {
     P2GTerms t;
     t.w = weights(s.x, s.y);
     t.x = s.x;
@@ -666,18 +685,27 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
     const DevMat& mt = mat_of(p, s.mat);
     t.tau = kirchhoff<ONLY>(mt, s);
     t.coeff = (-stress_volume<ONLY>(mt, s)) * kDInv;
+    t.coeff_dt = t.coeff * pred_dt;
     return t;
 }
 
 // All five channels, one at a time (keeps 9 partial sums live, not 45).
+// Terms below 1e18 in magnitude keep every stencil value below 1e37 and every
+// partial sum of 64 of them finite, so the fma form of the scan is exact.
 __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
                                             float (*s_acc)[kLdsCells], float* const* g, int R) {
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
-    scatter_channel<kAccM>(t, valid, sm, box, s_acc[kAccM], g[kAccM], R);
-    scatter_channel<kAccPx>(t, valid, sm, box, s_acc[kAccPx], g[kAccPx], R);
-    scatter_channel<kAccPy>(t, valid, sm, box, s_acc[kAccPy], g[kAccPy], R);
-    scatter_channel<kAccSx>(t, valid, sm, box, s_acc[kAccSx], g[kAccSx], R);
-    scatter_channel<kAccSy>(t, valid, sm, box, s_acc[kAccSy], g[kAccSy], R);
+    const ScanPred sq = scan_pred(sm);
+    float big = fmaxf(fmaxf(fabsf(t.mass), fmaxf(fabsf(t.coeff), fabsf(t.coeff_dt))), fmaxf(fabsf(t.vx), fabsf(t.vy)));
+    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.c.a), fabsf(t.c.b)), fmaxf(fabsf(t.c.c), fabsf(t.c.d))));
+    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.tau.a), fabsf(t.tau.b)), fmaxf(fabsf(t.tau.c), fabsf(t.tau.d))));
+    const bool small = !valid || (big < 1e18f && !__builtin_isnan(t.mass + t.coeff + t.vx + t.vy));
+    const bool fma_ok = __ballot(!small) == 0ull;
+    scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, s_acc[kAccM], g[kAccM], R);
+    scatter_channel<kAccPx>(t, valid, sm, sq, fma_ok, box, s_acc[kAccPx], g[kAccPx], R);
+    scatter_channel<kAccPy>(t, valid, sm, sq, fma_ok, box, s_acc[kAccPy], g[kAccPy], R);
+    scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, s_acc[kAccSx], g[kAccSx], R);
+    scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, s_acc[kAccSy], g[kAccSy], R);
 }
 
 // Add the LDS box into the global grid: one atomic per touched cell and channel.
@@ -701,7 +729,7 @@ __device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCel
 // freshly uploaded state (MODE = P2G or Mass: no gather, no store).  ONLY:
 // the one constitutive model every particle uses, or -1 for any mix.
 template <int MODE, int ONLY>
-__global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t seq) {
+__global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_t seq) {
     constexpr bool G2P = (MODE & kModeG2P) != 0;
     constexpr bool P2G = (MODE & kModeP2G) != 0;
     constexpr int NCH = P2G ? kAccChannels : 1;
@@ -716,9 +744,15 @@ __global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t s
     const CtrlState P = c->st[seq & 1u];
     const CtrlRed Q = c->red[seq_mod3(seq + 2u)];
     const CtrlState N = G2P ? decide_substep(P, Q, cfg) : P;
+    // dt' of this launch's P2G, and dt - dt' of the accumulators it gathers
+    // (a density pass scatters with the exact dt)
+    const float pred = predict_dt(N, cfg);
+    const float ddt = (G2P && !cfg.density_pass) ? N.dt - P.pred_dt : 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         CtrlState W = N;
         if (!G2P) W.active = 0;
+        if (P2G) W.pred_dt = pred;
+        if (G2P && N.active) W.last_ddt = ddt;
         c->st[(seq + 1u) & 1u] = W;
         if (G2P && !N.active) c->red[seq_mod3(seq)] = Q;  // nothing ran: the reduction carries over
         reset_red(c->red[seq_mod3(seq + 1u)]);
@@ -752,7 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t s
         for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0f;
-            if (G2P) s_vel[i] = cell_velocity(p, cur, dt, box.ox + i / box.h, box.oy + i % box.h);
+            if (G2P) s_vel[i] = cell_velocity(p, cur, dt, ddt, box.ox + i / box.h, box.oy + i % box.h);
         }
         __syncthreads();
     }
@@ -790,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t s
                         const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
                         const float dx = ((float)cx - s.x) + 0.5f;
                         const float dy = ((float)cy - s.y) + 0.5f;
-                        const float2 nv = lds ? s_vel[box.index(cx, cy)] : cell_velocity(p, cur, dt, cx, cy);
+                        const float2 nv = lds ? s_vel[box.index(cx, cy)] : cell_velocity(p, cur, dt, ddt, cx, cy);
                         const float wvx = nv.x * weight, wvy = nv.y * weight;
                         b.a = b.a + wvx * dx;
                         b.b = b.b + wvy * dx;
@@ -838,14 +872,15 @@ __global__ __launch_bounds__(kBlock) void k_substep(const Pipeline p, uint64_t s
         }
         // 5. P2G of the projected state into the next accumulator set
         if (P2G) {
-            const P2GTerms t = p2g_terms<ONLY>(p, sp);
+            const P2GTerms t = p2g_terms<ONLY>(p, sp, pred);
             scatter_p2g(t, valid, box, s_acc, gdst, R);
         } else {
             P2GTerms t;
             t.w = weights(sp.x, sp.y);
             t.mass = sp.mass;
             const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
-            scatter_channel<kAccM>(t, valid, sm, box, s_acc[0], gdst[0], R);
+            const bool fma_ok = __ballot(valid && !(fabsf(t.mass) < 1e18f)) == 0ull;
+            scatter_channel<kAccM>(t, valid, sm, scan_pred(sm), fma_ok, box, s_acc[0], gdst[0], R);
         }
     }
 
@@ -946,7 +981,7 @@ __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64
                 a.vol[j] = s.vol;
             }
         }
-        const P2GTerms t = p2g_terms<-1>(p, s);
+        const P2GTerms t = p2g_terms<-1>(p, s, N.dt);
         scatter_p2g(t, valid, box, s_acc, gdst, R);
     }
     flush_box<kAccChannels>(box, s_acc, gdst, R);

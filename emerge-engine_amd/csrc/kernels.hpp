@@ -38,11 +38,13 @@ struct PBuf {
 // Dense grid, x-major (cell (x, y) at x * R + y).  Three rotating sets of P2G
 // accumulators: the substep being gathered reads one, the particle kernel
 // scatters the next substep's P2G into the second and clears the third.
+// P2G runs before the CFL reduction has fixed the substep's dt, so it uses a
+// predicted dt' and also keeps the stress moment S = sum w (-4 V0) (tau d):
+// the grid update forms momentum = A + (dt - dt') S, which is the reference's
+// sum w (m (v + C d) + (-4 V0 dt) (tau d)) exactly when dt' = dt.
 constexpr int kAccM = 0, kAccPx = 1, kAccPy = 2, kAccSx = 3, kAccSy = 4, kAccChannels = 5;
 struct GridBuf {
-    // mass, momentum (m v + m C d), and the stress moment (-4 V0 tau d) that
-    // the grid update scales by the substep dt once the CFL reduction is done
-    float* acc[3][kAccChannels];
+    float* acc[3][kAccChannels];  // mass, A (x, y), S (x, y)
     float* dm[3];  // mass grid of the density re-estimate pass
 };
 
@@ -73,7 +75,9 @@ struct CtrlState {
     uint32_t cur_buf;   // which PBuf holds the state
     uint32_t cur_grid;  // accumulator set the last substep gathered from (0..2)
     uint32_t reorder;   // the next substep writes the particles in cell-key order
-    uint32_t pad[3];
+    float pred_dt;      // dt' the accumulators of the next substep were scattered with
+    float free_dt;      // the last decision's dt before the clamp to the time left
+    float last_ddt;     // dt - dt' the last substep's grid update applied
 };
 struct CtrlRed {
     uint32_t max_speed;  // f32 bits (non-negative: ordered as integers)
