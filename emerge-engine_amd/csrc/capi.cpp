@@ -292,11 +292,15 @@ struct emh_sim {
     uint32_t substeps_since_reorder = 0;
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
 
-    // per-kernel event timing
+    // Live timing: three events per step on the library stream -- before
+    // the step's set-up launches (k_begin_step and, on reorder steps, the
+    // key / scan kernels), before its first substep launch and after its
+    // last.  The substep launches run back to back, so their span divided by
+    // the substeps taken is the substep kernel's average duration.
     bool profiling = false;
-    std::vector<SlotEvents> slot_events;
-    double prof_ms[kKernelCount] = {};
-    uint64_t prof_launches[kKernelCount] = {};
+    hipEvent_t prof_ev[3] = {nullptr, nullptr, nullptr};
+    double prof_ms[2] = {0.0, 0.0};       // set-up, substep kernels
+    uint64_t prof_count[2] = {0, 0};      // steps, substeps
 
     // device
     int device = 0;
@@ -320,31 +324,14 @@ struct emh_sim {
         cap = 0;
     }
     void release_events() {
-        for (auto& se : slot_events)
-            for (int k = 0; k < kKernelCount; ++k)
-                for (int w = 0; w < 2; ++w) (void)hipEventDestroy(se.ev[k][w]);
-        slot_events.clear();
-    }
-    int ensure_slot_events(uint32_t n) {
-        while (slot_events.size() < n) {
-            SlotEvents se;
-            std::memset(&se, 0, sizeof(se));
-            for (int k = 0; k < kKernelCount; ++k)
-                for (int w = 0; w < 2; ++w) HIP_TRY(hipEventCreate(&se.ev[k][w]));
-            slot_events.push_back(se);
+        for (auto& e : prof_ev) {
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
         }
-        return EMH_OK;
     }
-    SlotEvents* slot_ev(uint32_t k) { return profiling ? &slot_events[k] : nullptr; }
-    int accumulate_profile(uint32_t active_slots) {
-        for (uint32_t k = 0; k < active_slots && k < slot_events.size(); ++k)
-            for (int id = 0; id < kKernelCount; ++id) {
-                if (!slot_events[k].used[id]) continue;
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, slot_events[k].ev[id][0], slot_events[k].ev[id][1]));
-                prof_ms[id] += ms;
-                prof_launches[id] += 1;
-            }
+    int ensure_events() {
+        for (auto& e : prof_ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
         return EMH_OK;
     }
 
@@ -680,19 +667,19 @@ struct emh_sim {
             return EMH_OK;
         }
         if (profiling) {
-            int rc = ensure_slot_events(cfg.max_substeps_per_step + 2);
+            int rc = ensure_events();
             if (rc) return rc;
         }
         for (uint32_t s = 0; s < n_steps; ++s) {
             const bool reorder = substeps_since_reorder >= kReorderSubsteps;
-            if (profiling)
-                for (auto& se : slot_events)
-                    for (int k = 0; k < kKernelCount; ++k) se.used[k] = false;
-            HIP_TRY(launch_begin_step(pipe, seq++, reorder, stream, slot_ev(0)));
+            if (profiling) HIP_TRY(hipEventRecord(prof_ev[0], stream));
+            HIP_TRY(launch_begin_step(pipe, seq++, reorder, stream));
+            if (profiling) HIP_TRY(hipEventRecord(prof_ev[1], stream));
             if (reorder) substeps_since_reorder = 0;
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, slot_ev(k)));
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream));
+            if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
             Ctrl c;
             CtrlState ctrl;
             uint32_t launched = slots;
@@ -704,12 +691,19 @@ struct emh_sim {
                 if (!(ctrl.remaining > kF32Eps && ctrl.substeps < cfg.max_substeps_per_step)) break;
                 if (launched > cfg.max_substeps_per_step + 1)
                     return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
-                HIP_TRY(launch_substep(pipe, seq++, stream, slot_ev(launched)));
+                HIP_TRY(launch_substep(pipe, seq++, stream));
+                if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
                 ++launched;
             }
             if (profiling) {
-                int rc = accumulate_profile(ctrl.substeps);
-                if (rc) return rc;
+                float a = 0.0f, b = 0.0f;
+                HIP_TRY(hipEventSynchronize(prof_ev[2]));
+                HIP_TRY(hipEventElapsedTime(&a, prof_ev[0], prof_ev[1]));
+                HIP_TRY(hipEventElapsedTime(&b, prof_ev[1], prof_ev[2]));
+                prof_ms[0] += a;
+                prof_count[0] += 1;
+                prof_ms[1] += b;
+                prof_count[1] += ctrl.substeps;
             }
             predicted_substeps = ctrl.substeps;
             substeps_since_reorder += ctrl.substeps;
@@ -1182,21 +1176,22 @@ int emh_sim_synchronize(emh_sim* s) {
 int emh_sim_set_profiling(emh_sim* s, int enable) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     s->profiling = enable != 0;
-    for (int k = 0; k < kKernelCount; ++k) {
+    for (int k = 0; k < 2; ++k) {
         s->prof_ms[k] = 0.0;
-        s->prof_launches[k] = 0;
+        s->prof_count[k] = 0;
     }
     return EMH_OK;
 }
 
 int emh_sim_kernel_times(emh_sim* s, emh_kernel_time* out, uint32_t cap, uint32_t* n_out) {
     if (!s || (!out && cap)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    const char* names[2] = {"step_setup", s->pipe.cfg.density_pass ? "k_density_p2g+k_substep" : "k_substep"};
     uint32_t n = 0;
-    for (int k = 0; k < kKernelCount; ++k) {
+    for (int k = 0; k < 2; ++k) {
         if (n < cap) {
             std::memset(&out[n], 0, sizeof(out[n]));
-            std::snprintf(out[n].name, sizeof(out[n].name), "%s", kKernelNames[k]);
-            out[n].launches = s->prof_launches[k];
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", names[k]);
+            out[n].launches = s->prof_count[k];
             out[n].total_ms = s->prof_ms[k];
         }
         ++n;

@@ -1057,9 +1057,6 @@ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------------ launchers
 
-const char* const kKernelNames[kKernelCount] = {"k_begin_step", "k_keys",        "k_tile_scan",
-                                                  "k_cell_scan",  "k_density_p2g", "k_substep"};
-
 hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream) {
     const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
     const uint32_t n = R2 > p.cfg.n_bins ? R2 : p.cfg.n_bins;
@@ -1076,53 +1073,25 @@ hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream) 
     return hipGetLastError();
 }
 
-namespace {
-struct Marker {
-    SlotEvents* ev;
-    hipStream_t stream;
-    void operator()(int id, int which) const {
-        if (!ev) return;
-        (void)hipEventRecord(ev->ev[id][which], stream);
-        ev->used[id] = true;
-    }
-};
-}  // namespace
-
-hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream,
-                             SlotEvents* events) {
-    Marker mark{events, stream};
+hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream) {
     reorder = reorder && p.n > 0;
-    mark(kKBegin, 0);
     k_begin_step<<<1, 1, 0, stream>>>(p, seq, reorder ? 1u : 0u);
-    mark(kKBegin, 1);
     if (reorder) {
         const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
-        mark(kKKeys, 0);
         k_keys<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p, seq + 1u);
-        mark(kKKeys, 1);
-        mark(kKTileScan, 0);
         k_tile_scan<<<1, 1024, 0, stream>>>(p);
-        mark(kKTileScan, 1);
-        mark(kKCellScan, 0);
         k_cell_scan<<<n_tiles, kTileCells, 0, stream>>>(p);
-        mark(kKCellScan, 1);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, SlotEvents* events) {
-    Marker mark{events, stream};
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream) {
     const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
     if (p.cfg.density_pass) {
-        mark(kKDensityP2G, 0);
         k_density_p2g<<<nb, kBlock, 0, stream>>>(p, seq);
-        mark(kKDensityP2G, 1);
-        mark(kKSubstep, 0);
         k_substep<kModeG2P | kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
-        mark(kKSubstep, 1);
     } else {
         constexpr int kFused = kModeG2P | kModeP2G;
-        mark(kKSubstep, 0);
         switch (p.cfg.only_model) {
             case kModelNH: k_substep<kFused, kModelNH><<<nb, kBlock, 0, stream>>>(p, seq); break;
             case kModelCorot: k_substep<kFused, kModelCorot><<<nb, kBlock, 0, stream>>>(p, seq); break;
@@ -1131,7 +1100,6 @@ hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, S
             case kModelNacc: k_substep<kFused, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq); break;
             default: k_substep<kFused, -1><<<nb, kBlock, 0, stream>>>(p, seq); break;
         }
-        mark(kKSubstep, 1);
     }
     return hipGetLastError();
 }

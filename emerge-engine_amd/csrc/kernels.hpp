@@ -105,31 +105,12 @@ struct Pipeline {
     Ctrl* ctrl;
 };
 
-// Kernels, in launch order (profiling identifiers).
-enum KernelId : int {
-    kKBegin = 0,
-    kKKeys,
-    kKTileScan,
-    kKCellScan,
-    kKDensityP2G,
-    kKSubstep,
-    kKernelCount
-};
-extern const char* const kKernelNames[kKernelCount];
-
-// Optional per-kernel event pairs of one substep slot (null: no timing).
-struct SlotEvents {
-    hipEvent_t ev[kKernelCount][2];
-    bool used[kKernelCount];
-};
-
 // Launch wrappers (all asynchronous on `stream`); `seq` is the launch's
 // sequence number in the device-side loop (see Ctrl).
 hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream);
 hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream);
-hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream,
-                             SlotEvents* events = nullptr);
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, SlotEvents* events = nullptr);
+hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream);
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream);
 
 // Stand-alone cell-key sort (bit-exact stable counting sort).
 hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys,

@@ -624,11 +624,11 @@ class Simulation:
         return self.stats().last_substeps
 
     def set_profiling(self, enable: bool) -> None:
-        """Per-kernel HIP-event timing (resets the totals)."""
+        """Step-level HIP-event timing on the library stream (resets the totals)."""
         _check(load_library().emh_sim_set_profiling(self._h, int(bool(enable))))
 
     def kernel_times(self) -> dict:
-        """{kernel name: (launches, total ms)} over the substeps that ran."""
+        """{"step_setup": (steps, ms), "k_substep": (substeps, ms)} since profiling was enabled."""
         arr = (CKernelTime * 16)()
         n = ctypes.c_uint32()
         _check(load_library().emh_sim_kernel_times(self._h, arr, 16, ctypes.byref(n)))

@@ -282,8 +282,11 @@ int emh_sim_get_grid(emh_sim* sim, float* mass, float* momentum, size_t n_cells)
 int emh_sim_synchronize(emh_sim* sim);
 
 /* ---- per-kernel timing ---------------------------------------------------
- * HIP events around every substep kernel on the library's stream, summed over
- * the substeps that ran (slots the device loop skipped are not counted).  The
+ * HIP events on the library's stream, three per step: two entries, summed
+ * over the steps taken since enabling -- "step_setup" (launches = steps): the
+ * step's start kernel and, on reorder steps, the cell-key and scan kernels;
+ * "k_substep" ("k_density_p2g+k_substep" when a density pass runs; launches =
+ * substeps taken): the back-to-back substep launches of each step.  The
  * counterpart of GpuSimulation::enable_profiling / last_pass_timings_ns
  * (src/systems/gpu/solver/profiling.rs:9-75).  Enabling resets the totals. */
 typedef struct emh_kernel_time {

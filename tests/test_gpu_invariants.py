@@ -237,3 +237,18 @@ def test_projection_repairs_invalid_state(oracle):
         assert ok, (f, err, tol)
     ok, err, tol = _rel_ulp_close(pg["density"], po["density"])
     assert ok, ("density", err, tol)
+
+
+@pytest.mark.gpu
+def test_profiling_counts_substeps():
+    """Step-level timing counts every step and every substep taken."""
+    sim = S.quickstart("gpu")
+    sim.step_n(2)
+    sim.set_profiling(True)
+    s0 = sim.stats().total_substeps
+    sim.step_n(3)
+    t = sim.kernel_times()
+    sim.set_profiling(False)
+    assert t["step_setup"][0] == 3
+    assert t["k_density_p2g+k_substep"][0] == sim.stats().total_substeps - s0
+    assert all(ms > 0.0 for _, ms in t.values())
