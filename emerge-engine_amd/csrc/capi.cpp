@@ -640,7 +640,65 @@ struct emh_sim {
 
     int host_current() { return download(); }
 
-    // Simulation::step x n  (step.rs:23-60), one device sync per step.
+    // Running per the loop condition of step.rs:34.
+    bool running(const CtrlState& st) const {
+        return st.in_step && st.remaining > kF32Eps && st.substeps < cfg.max_substeps_per_step;
+    }
+    // Wait until every queued step has finished, launching further substep
+    // slots while a step is still running or queued; then publish the last
+    // step's records (step.rs:51-59).
+    int drain() {
+        Ctrl c;
+        CtrlState st;
+        uint32_t extra = 0;
+        const uint32_t limit = (cfg.max_substeps_per_step + 1) * (kStepRing + 1);
+        while (true) {
+            if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
+            HIP_TRY(hipMemcpyAsync(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            st = c.st[seq & 1u];  // the state the next launch would decide from
+            if (!running(st) && st.pending == 0) break;
+            uint32_t more = (running(st) ? 1u : 0u) + st.pending * std::max<uint32_t>(predicted_substeps, 1u);
+            more = std::min<uint32_t>(std::max<uint32_t>(more, 1u), 64u);
+            if (extra + more > limit) return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
+            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream));
+            extra += more;
+        }
+        // the last step: finished but not yet recorded if no launch followed it
+        uint32_t subs, step;
+        float last_dt, dropped;
+        if (st.in_step) {
+            subs = st.substeps;
+            step = st.step;
+            last_dt = st.last_dt;
+            dropped = std::fmax(st.remaining, 0.0f);
+        } else {
+            subs = st.done_substeps;
+            step = st.done_step;
+            last_dt = st.done_dt;
+            dropped = st.done_dropped;
+        }
+        if (profiling) {
+            float a = 0.0f, b = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&a, prof_ev[0], prof_ev[1]));
+            HIP_TRY(hipEventElapsedTime(&b, prof_ev[1], prof_ev[2]));
+            prof_ms[0] += a;
+            prof_count[0] += 1;
+            prof_ms[1] += b;
+            prof_count[1] += subs;
+        }
+        predicted_substeps = subs;
+        stats.last_substeps = subs;
+        stats.last_step_dt = last_dt;
+        stats.last_sim_time_dropped = dropped;
+        stats.last_vel_clamp_count = c.clamp_count[step % kStepRing];
+        stats.last_j_projection_count = c.proj_count[step % kStepRing];
+        stats.total_substeps = st.total_substeps;
+        return EMH_OK;
+    }
+
+    // Simulation::step x n  (step.rs:23-60): queued on the device, drained
+    // at the end.
     int step(uint32_t n_steps) {
         if (host_dirty) {
             int rc = upload();
@@ -670,6 +728,11 @@ struct emh_sim {
             int rc = ensure_events();
             if (rc) return rc;
         }
+        // Queue every step without waiting: the device loop starts each queued
+        // step once the previous one has finished.  Drain (wait and, if some
+        // step needed more substep launches than queued, launch more) every
+        // kDrainSteps steps, at the end, and after every step when profiling.
+        constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
             const bool reorder = substeps_since_reorder >= kReorderSubsteps;
             if (profiling) HIP_TRY(hipEventRecord(prof_ev[0], stream));
@@ -679,42 +742,13 @@ struct emh_sim {
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
             for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream));
-            if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
-            Ctrl c;
-            CtrlState ctrl;
-            uint32_t launched = slots;
-            while (true) {
-                HIP_TRY(hipMemcpyAsync(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
-                HIP_TRY(hipStreamSynchronize(stream));
-                ctrl = c.st[seq & 1u];  // the state the next launch would decide from
-                // while remaining > f32::EPSILON && substeps < max_substeps_per_step  (step.rs:34)
-                if (!(ctrl.remaining > kF32Eps && ctrl.substeps < cfg.max_substeps_per_step)) break;
-                if (launched > cfg.max_substeps_per_step + 1)
-                    return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
-                HIP_TRY(launch_substep(pipe, seq++, stream));
-                if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
-                ++launched;
+            substeps_since_reorder += slots;
+            if (profiling || (s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
+                int rc = drain();
+                if (rc) return rc;
             }
-            if (profiling) {
-                float a = 0.0f, b = 0.0f;
-                HIP_TRY(hipEventSynchronize(prof_ev[2]));
-                HIP_TRY(hipEventElapsedTime(&a, prof_ev[0], prof_ev[1]));
-                HIP_TRY(hipEventElapsedTime(&b, prof_ev[1], prof_ev[2]));
-                prof_ms[0] += a;
-                prof_count[0] += 1;
-                prof_ms[1] += b;
-                prof_count[1] += ctrl.substeps;
-            }
-            predicted_substeps = ctrl.substeps;
-            substeps_since_reorder += ctrl.substeps;
-            stats.last_substeps = ctrl.substeps;
-            stats.last_step_dt = ctrl.last_dt;
-            stats.last_sim_time_dropped = std::fmax(ctrl.remaining, 0.0f);
-            stats.last_vel_clamp_count = c.clamp_count;
-            stats.last_j_projection_count = c.proj_count;
-            stats.total_substeps = ctrl.total_substeps;
-            stats.frame_index++;
         }
+        stats.frame_index += n_steps;
         device_ahead = true;
         return EMH_OK;
     }

@@ -1,7 +1,7 @@
 // kernels.hip -- the MLS-MPM substep as gfx950 HIP kernels.
 //
 // Simulation::step (solver/step.rs:23-60) runs on the device as:
-//   k_begin_step     1 lane: starts the step's substep loop (step.rs:34).
+//   k_begin_step     1 lane: queues the start of a step (step.rs:24-33).
 //  [k_keys]          on reorder steps: cell key and arrival rank of every
 //                    particle, per-cell and per-tile counts.
 //  [k_tile_scan]     on reorder steps: start slot of every 16x16 tile.
@@ -259,17 +259,33 @@ __device__ __forceinline__ void cfl_of(const Pipeline& p, const PState& s, float
     if (__builtin_isfinite(b) && b > 0.0f) mdt = fminf(mdt, b);
 }
 
-// The decision of control launch s: the loop condition of Simulation::step
-// (step.rs:34) on the state launch s - 1 left, then choose_substep_dt /
-// cfl_bound (step.rs:434-479) from the reduction of launch s - 1.
+// The decision of control launch s from the state launch s - 1 left and its
+// CFL reduction: the loop condition of Simulation::step (step.rs:34) --
+// finishing the step and starting the next queued one when it fails -- then
+// choose_substep_dt / cfl_bound (step.rs:434-479).
 __device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const DevConfig& cfg) {
     CtrlState N = P;
-    if (!(P.remaining > kF32Eps && P.substeps < cfg.max_substeps)) {
-        N.active = 0;
-        return N;
+    N.active = 0;
+    bool running = P.in_step && P.remaining > kF32Eps && P.substeps < cfg.max_substeps;
+    if (!running) {
+        if (P.in_step) {  // step.rs:51-59: the step's records
+            N.in_step = 0;
+            N.done_steps = P.done_steps + 1;
+            N.done_step = P.step;
+            N.done_substeps = P.substeps;
+            N.done_dt = P.last_dt;
+            N.done_dropped = rmax(P.remaining, 0.0f);
+        }
+        if (N.pending == 0) return N;
+        N.pending -= 1;  // step.rs:24-33
+        N.in_step = 1;
+        N.step = P.step + 1;
+        N.remaining = cfg.dt;
+        N.substeps = 0;
+        if (!(N.remaining > kF32Eps)) return N;  // a step too short for any substep ends next launch
     }
     const float a = __uint_as_float(Q.max_speed), d = __uint_as_float(Q.min_mdt);
-    const float remaining = P.remaining;
+    const float remaining = N.remaining;
     float dt, free = cfg.dt;
     if (!cfg.adaptive) {
         dt = rmin(remaining, cfg.dt);
@@ -287,7 +303,7 @@ __device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const 
     N.free_dt = free;
     N.last_dt = dt;
     N.active = 1;
-    N.substeps = P.substeps + 1;
+    N.substeps = N.substeps + 1;
     N.remaining = remaining - dt;
     N.total_substeps = P.total_substeps + 1;
     N.cur_grid = P.cur_grid == 2u ? 0u : P.cur_grid + 1u;
@@ -572,20 +588,20 @@ __global__ void k_zero_all(const Pipeline p) {
     }
 }
 
-// Control launch `seq` that starts a step: remaining = config.dt, no
-// substeps yet (step.rs:24-33); the CFL reduction carries over.
+// Control launch `seq` that queues a step start (remaining = config.dt, no
+// substeps yet: step.rs:24-33, applied by the substep launch that begins it)
+// and clears that step's counters; the CFL reduction carries over.
 __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
     Ctrl* c = p.ctrl;
     CtrlState N = c->st[seq & 1u];
-    N.remaining = p.cfg.dt;
-    N.substeps = 0;
-    N.active = 0;
-    N.reorder = reorder;
+    const uint32_t step = N.step + N.pending + 1u;  // the step this marker starts
+    N.pending += 1u;
+    N.reorder |= reorder;
     c->st[(seq + 1u) & 1u] = N;
     c->red[seq_mod3(seq)] = c->red[seq_mod3(seq + 2u)];
     reset_red(c->red[seq_mod3(seq + 1u)]);
-    c->clamp_count = 0;
-    c->proj_count = 0;
+    c->clamp_count[step % kStepRing] = 0;
+    c->proj_count[step % kStepRing] = 0;
 }
 
 // Cell key of the stored (projected) position of every particle, its arrival
@@ -901,8 +917,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
     const uint32_t wc = wave_sum(n_clamp), wp = wave_sum(n_proj);
     const bool wd = __ballot(any_dirty) != 0ull;
     if (lane_id() == 0) {
-        if (wc) atomicAdd(&c->clamp_count, wc);
-        if (wp) atomicAdd(&c->proj_count, wp);
+        if (wc) atomicAdd(&c->clamp_count[N.step % kStepRing], wc);
+        if (wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
         if (wd) atomicOr(&red->dirty, 1u);
     }
     reduce_cfl(red, speed, mdt, s_red);
@@ -986,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64
     }
     flush_box<kAccChannels>(box, s_acc, gdst, R);
     const uint32_t wp = wave_sum(n_proj);
-    if (lane_id() == 0 && wp) atomicAdd(&c->proj_count, wp);
+    if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
 
 // ------------------------------------------------- stand-alone cell sort

@@ -58,8 +58,12 @@ struct SortBuf {
 };
 
 // Device-resident control: the adaptive substep loop runs on the device so
-// that no substep needs a host round trip, and no kernel synchronises its
-// workgroups with each other.  Every launch that takes part in the loop has a
+// that neither a substep nor a step needs a host round trip, and no kernel
+// synchronises its workgroups with each other.  The host queues step starts
+// (k_begin_step adds one to `pending`) and substep launches; a substep launch
+// that finds the current step finished (step.rs:34) starts the next pending
+// step, so launches left over by a short step serve the next one and the
+// host only checks at the end that every queued step has finished.  Every launch that takes part in the loop has a
 // sequence number s (counted by the host).  Launch s reads st[s & 1] (written
 // by launch s - 1) and the reduction red[(s + 2) % 3] of launch s - 1; every
 // workgroup derives the same decision for its substep from those, workgroup 0
@@ -78,6 +82,14 @@ struct CtrlState {
     float pred_dt;      // dt' the accumulators of the next substep were scattered with
     float free_dt;      // the last decision's dt before the clamp to the time left
     float last_ddt;     // dt - dt' the last substep's grid update applied
+    uint32_t in_step;   // a step is running
+    uint32_t pending;   // queued step starts
+    uint32_t step;      // number of the running (or last) step, from 1
+    uint32_t done_steps;     // finished steps
+    uint32_t done_step;      // number of the last finished step
+    uint32_t done_substeps;  // its substeps
+    float done_dt;           // its last substep's dt
+    float done_dropped;      // its unsimulated time, max(remaining, 0)
 };
 struct CtrlRed {
     uint32_t max_speed;  // f32 bits (non-negative: ordered as integers)
@@ -85,12 +97,12 @@ struct CtrlRed {
     uint32_t dirty;      // some stored particle is changed by projection
     uint32_t pad;
 };
+constexpr uint32_t kStepRing = 64;  // per-step counters, indexed by step % kStepRing
 struct Ctrl {
     CtrlState st[2];
     CtrlRed red[3];
-    uint32_t clamp_count;  // G2P velocity clamps in the current step
-    uint32_t proj_count;   // state projections in the current step
-    uint32_t pad[2];
+    uint32_t clamp_count[kStepRing];  // G2P velocity clamps of each step
+    uint32_t proj_count[kStepRing];   // state projections of each step
 };
 constexpr uint32_t kInfBits = 0x7f800000u;
 
