@@ -40,6 +40,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # launch moves 136 B per particle algorithmically.
 SUBSTEP_BYTES = 136
 ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
+# HBM bytes per k_substep launch on this workload, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (one run each, mean over launches) and scaled by
+# the same counters' reading for a 256 MiB device-to-device copy
+# (profiles/k_substep_sand_1m_512.md): 26861 KiB x 2.000 + 78440 KiB x 1.000.
+MEASURED_TRAFFIC = {"k_substep": (26861.0 * 2.000 + 78440.0 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu"):
@@ -85,6 +90,9 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the bench line, run three device-to-device copies of this size (a known-size "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the substep has no multi-GPU "
@@ -148,7 +156,7 @@ def main() -> int:
         "substeps_timed": substeps,
         "substeps_per_step": substeps / args.steps,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(dominant),
                      "algorithmic_bytes_per_launch": algo, "avg_launch_us": avg_s * 1e6},
         "substep_roofline": {"achieved": substep_gbs, "frac": substep_gbs / HBM_PEAK_GBS,
                              "bytes_per_particle_substep": SUBSTEP_BYTES},
@@ -160,8 +168,30 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, first_dt)
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib << 20)
     return 0
+
+
+def copy_calibration(nbytes: int) -> None:
+    """Three device-to-device copies of `nbytes` through the HIP runtime."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)):
+            raise RuntimeError("hipMemset failed")
+        for _ in range(3):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize():
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 if __name__ == "__main__":
