@@ -109,12 +109,17 @@ int check_supported_material(const emh_material_params& p) {
         case EMH_MODEL_COROTATED:
         case EMH_MODEL_SNOW:
         case EMH_MODEL_DRUCKER_PRAGER:
+        case EMH_MODEL_VON_MISES:
+        case EMH_MODEL_VISCOELASTIC:
         case EMH_MODEL_NACC:
             break;
         default:
             return fail(EMH_ERR_UNSUPPORTED, "constitutive model %u is not implemented on the GPU path", p.model);
     }
-    if (p.active_stress_coeff != 0.0f && (p.model == EMH_MODEL_NEOHOOKEAN || p.model == EMH_MODEL_COROTATED))
+    if (p.model == EMH_MODEL_VISCOELASTIC && !(p.volume_ratio_min > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "viscoelastic j_min (volume_ratio_min) must be positive");
+    if (p.active_stress_coeff != 0.0f &&
+        (p.model == EMH_MODEL_NEOHOOKEAN || p.model == EMH_MODEL_COROTATED || p.model == EMH_MODEL_VISCOELASTIC))
         return fail(EMH_ERR_UNSUPPORTED, "active stress (active_stress_coeff != 0) is not supported");
     if (p.thermal_expansion != 0.0f && (p.model == EMH_MODEL_NEOHOOKEAN || p.model == EMH_MODEL_COROTATED))
         return fail(EMH_ERR_UNSUPPORTED, "thermal_expansion != 0 is not supported");
@@ -172,6 +177,14 @@ DevMat make_dev_mat(const emh_material_params& p, const emh_material_ext& e) {
             d.sn_jmin = p.volume_ratio_min;
             d.sn_jmax = p.volume_ratio_max;
             d.sn_coh = p.cohesion_coeff;
+            break;
+        case EMH_MODEL_VON_MISES:  // von_mises.rs:143-152: yield stress in hardening_exponent
+            d.vm_yield = p.hardening_exponent;
+            d.vm_hmod = p.hardening_modulus;
+            break;
+        case EMH_MODEL_VISCOELASTIC:  // viscoelastic.rs:149-159
+            d.ve_visc = p.dynamic_viscosity;
+            d.ve_jmin = p.volume_ratio_min;
             break;
         case EMH_MODEL_NACC: {
             float kappa = e.nacc_kappa != 0.0f ? e.nacc_kappa : p.lambda + p.mu;
@@ -314,6 +327,8 @@ struct emh_sim {
     uint32_t* d_tile_count = nullptr;
     uint32_t* d_tile_start = nullptr;
     Ctrl* d_ctrl = nullptr;
+    DevSnapshot* d_snap = nullptr;     // diagnostics scratch, allocated on first use
+    uint32_t* d_snap_cells = nullptr;  // 3 R^2 per-cell material ranges and counts
     std::vector<void*> particle_allocs;
 
     ~emh_sim() { release(); }
@@ -346,6 +361,10 @@ struct emh_sim {
         if (d_tile_count) (void)hipFree(d_tile_count);
         if (d_tile_start) (void)hipFree(d_tile_start);
         if (d_ctrl) (void)hipFree(d_ctrl);
+        if (d_snap) (void)hipFree(d_snap);
+        if (d_snap_cells) (void)hipFree(d_snap_cells);
+        d_snap = nullptr;
+        d_snap_cells = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         d_mats = nullptr;
         grid_block = nullptr;
@@ -485,6 +504,8 @@ struct emh_sim {
             switch (mats[i].model) {
                 case EMH_MODEL_NEOHOOKEAN: fields |= kFQ; break;
                 case EMH_MODEL_DRUCKER_PRAGER: fields |= kFQ | kFLvs; break;
+                case EMH_MODEL_VON_MISES: fields |= kFQ; break;
+                case EMH_MODEL_VISCOELASTIC: break;
                 case EMH_MODEL_COROTATED: fields |= kFH; break;
                 case EMH_MODEL_SNOW: fields |= kFJp | kFH; break;
                 case EMH_MODEL_NACC: fields |= kFLvs | kFH | kFVolDens; break;
@@ -991,6 +1012,33 @@ int emh_material_snow_from_young(float young, float nu, emh_material_params* p, 
     return emh_material_snow(l, m, 10.0f, 0.025f, 0.0075f, 0.6f, 20.0f, p, e);
 }
 
+int emh_material_von_mises(float lambda, float mu, float yield_stress, float hardening_modulus,
+                           emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    if (!(hardening_modulus >= 0.0f)) return fail(EMH_ERR_INVALID_ARG, "hardening_modulus must be >= 0");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_VON_MISES;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->hardening_exponent = yield_stress;
+    p->hardening_modulus = hardening_modulus;
+    return EMH_OK;
+}
+
+int emh_material_viscoelastic(float lambda, float mu, float viscosity, emh_material_params* p,
+                              emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_VISCOELASTIC;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->dynamic_viscosity = viscosity;
+    p->volume_ratio_min = 0.01f;
+    return EMH_OK;
+}
+
 int emh_material_nacc(float mu, float kappa, float friction, float cohesion, float xi, emh_material_params* p,
                       emh_material_ext* e) {
     if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
@@ -1198,6 +1246,96 @@ int emh_sim_get_grid(emh_sim* s, float* mass, float* momentum, size_t n_cells) {
             momentum[2 * i + 1] = ch[1][i] + ddt * ch[3][i];
         }
     }
+    return EMH_OK;
+}
+
+int emh_sim_snapshot(emh_sim* s, emh_snapshot* out) {  // snapshot.rs:227-372, queries.rs:15-30
+    if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (s->host_dirty) {
+        int rc = s->upload();
+        if (rc) return rc;
+    }
+    const size_t R2 = (size_t)s->res() * s->res();
+    if (!s->d_snap) {
+        s->d_snap = dalloc<DevSnapshot>(1);
+        s->d_snap_cells = dalloc<uint32_t>(3 * R2);
+        if (!s->d_snap || !s->d_snap_cells) return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+        HIP_TRY(hipMemsetAsync(s->d_snap_cells, 0xff, sizeof(uint32_t) * R2, s->stream));
+        HIP_TRY(hipMemsetAsync(s->d_snap_cells + R2, 0, sizeof(uint32_t) * 2 * R2, s->stream));
+    }
+    Ctrl ctrl;
+    HIP_TRY(hipMemcpyAsync(&ctrl, s->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const CtrlState st = ctrl.st[s->seq & 1u];
+    DevSnapshot z;
+    std::memset(&z, 0, sizeof(z));
+    z.min_j = z.min_jp = 0xffffffffu;
+    HIP_TRY(hipMemcpyAsync(s->d_snap, &z, sizeof(z), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(launch_snapshot(s->pipe, st.cur_buf & 1u, st.cur_grid % 3u, st.last_dt, st.last_ddt, s->d_snap,
+                            s->d_snap_cells, s->d_snap_cells + R2, s->d_snap_cells + 2 * R2, s->stream));
+    DevSnapshot d;
+    HIP_TRY(hipMemcpyAsync(&d, s->d_snap, sizeof(d), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const emh_config& cfg = s->cfg;
+    const uint32_t n = s->pipe.n;
+    emh_snapshot o;
+    std::memset(&o, 0, sizeof(o));
+    o.frame_index = s->stats.frame_index;
+    o.configured_dt = cfg.dt;
+    o.effective_dt = s->stats.last_step_dt;
+    o.substeps_last_step = s->stats.last_substeps;
+    o.particle_count = n;
+    o.valid_particle_count = d.valid;
+    o.active_grid_cells = d.active_cells;
+    if (d.active_cells > 0) o.particles_per_active_cell = (float)n / (float)d.active_cells;
+    if (d.touched_cells > 0) o.mixed_material_cell_ratio = (float)d.mixed_cells / (float)d.touched_cells;
+    if (n > 0) o.mixed_material_particle_ratio = (float)d.particles_in_mixed / (float)n;
+    o.total_particle_mass = (float)d.mass;
+    o.total_grid_mass = (float)d.grid_mass;
+    if (o.total_particle_mass > kF32Eps)
+        o.relative_mass_error = std::fabs(o.total_grid_mass - o.total_particle_mass) / o.total_particle_mass;
+    o.total_particle_momentum[0] = (float)d.mom_x;
+    o.total_particle_momentum[1] = (float)d.mom_y;
+    o.total_grid_momentum[0] = (float)d.grid_mom_x;
+    o.total_grid_momentum[1] = (float)d.grid_mom_y;
+    const float pm = std::hypot(o.total_particle_momentum[0], o.total_particle_momentum[1]);
+    const float gm = std::hypot(o.total_grid_momentum[0], o.total_grid_momentum[1]);
+    const float scale = std::fmax(std::fmax(pm, gm), o.total_particle_mass * 1.0e-3f);
+    if (scale > 0.0f)
+        o.relative_momentum_error = std::hypot(o.total_grid_momentum[0] - o.total_particle_momentum[0],
+                                               o.total_grid_momentum[1] - o.total_particle_momentum[1]) / scale;
+    float bits2f;
+    std::memcpy(&bits2f, &d.max_speed, 4);
+    o.max_particle_speed = bits2f;
+    std::memcpy(&bits2f, &d.max_grid_speed, 4);
+    o.max_grid_speed = bits2f;
+    std::memcpy(&bits2f, &d.max_pinned_speed, 4);
+    o.max_pinned_particle_speed = bits2f;
+    o.min_deformation_j = d.min_j == 0xffffffffu ? std::numeric_limits<float>::infinity() : order_key_inverse(d.min_j);
+    o.max_deformation_j = d.max_j == 0u ? -std::numeric_limits<float>::infinity() : order_key_inverse(d.max_j);
+    o.out_of_bounds_particles = d.out_of_bounds;
+    o.invalid_physical_particle_values = d.invalid;
+    o.non_finite_particle_values = d.non_finite;
+    o.non_finite_grid_values = d.non_finite_grid;
+    o.avg_plastic_jacobian = 1.0f;
+    o.avg_elastic_hardening = 1.0f;
+    o.min_plastic_jacobian = std::numeric_limits<float>::infinity();
+    if (n > 0) {
+        o.avg_plastic_jacobian = (float)(d.jp_sum / (double)n);
+        o.avg_elastic_hardening = (float)(d.h_sum / (double)n);
+        o.min_plastic_jacobian = d.jp_finite ? order_key_inverse(d.min_jp) : 1.0f;
+    }
+    const float cell = std::fmax(cfg.grid_cell_size, kF32Eps);
+    o.cfl_number = o.max_particle_speed * o.effective_dt / cell;
+    o.recommended_max_dt_from_velocity_cfl =
+        o.max_particle_speed > kF32Eps ? cell / o.max_particle_speed : std::numeric_limits<float>::infinity();
+    o.active_count = n;
+    o.sleeping_count = 0;
+    o.vel_clamp_count = s->stats.last_vel_clamp_count;
+    o.j_projection_count = s->stats.last_j_projection_count;
+    o.sim_time_dropped = s->stats.last_sim_time_dropped;
+    o.total_kinetic_energy = (float)d.kinetic;
+    *out = o;
     return EMH_OK;
 }
 

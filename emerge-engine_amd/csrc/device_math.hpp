@@ -92,13 +92,18 @@ struct DevMat {
     float sn_xi, sn_lo, sn_hi, sn_jmin, sn_jmax, sn_coh;
     // NACC
     float na_kappa, na_m, na_beta, na_xi;
+    // Von Mises
+    float vm_yield, vm_hmod;
+    // Viscoelastic
+    float ve_visc, ve_jmin;
     // CFL modulus helpers
     float wave_lambda;  // lambda used by elastic_wave_dt (NACC: kappa - mu)
     float pad[3];
 };
 
 constexpr uint32_t kModelFallback = 0, kModelFluid = 1, kModelNH = 2, kModelCorot = 3, kModelSnow = 4,
-                   kModelDP = 5, kModelNacc = 10, kModelMixed = 0xffffffffu;
+                   kModelDP = 5, kModelVonMises = 6, kModelViscoelastic = 9, kModelNacc = 10,
+                   kModelMixed = 0xffffffffu;
 
 // ------------------------------------------------------------ small algebra
 struct M2 {
@@ -303,7 +308,8 @@ EMH_DEV bool project_state(PState& s, const DevConfig& cfg) {
 // the material's model id at run time; a model id compiles that model alone
 // (used when every particle resolves to materials of one model).
 
-// stress_volume: materials/mod.rs:118-120, fluid.rs:193-195
+// stress_volume: materials/mod.rs:118-120, fluid.rs:193-195 (von_mises.rs:104-106 and
+// viscoelastic.rs:141-143 restate the default)
 template <int ONLY = -1>
 EMH_DEV float stress_volume(const DevMat& m, const PState& s) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
@@ -318,7 +324,8 @@ EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
 }
 
 // Kirchhoff stress: elastic.rs:180-255, corotated.rs:62-78, sand.rs:220-231,
-// snow.rs:109-136, fluid.rs:135-191, nacc.rs:236-249, Fallback mod.rs:113-115.
+// von_mises.rs:94-102, viscoelastic.rs:103-130, snow.rs:109-136, fluid.rs:135-191,
+// nacc.rs:236-249, Fallback mod.rs:113-115.
 template <int ONLY = -1>
 EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
     const M2 zero{0.0f, 0.0f, 0.0f, 0.0f};
@@ -354,10 +361,25 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
             if (j <= kMinJ) return zero;
             return corot_tau(s.f, j, m.mu * s.h * 1.0f, m.lambda * s.h * 1.0f);
         }
-        case kModelDP: {
+        case kModelDP:
+        case kModelVonMises: {
             float j = mdet(s.f);
             if (j <= kMinJ) return zero;
             return corot_tau(s.f, j, m.mu, m.lambda);
+        }
+        case kModelViscoelastic: {
+            // neo-Hookean spring mu (F F^T - I) + lambda ln(J) I with J clamped to
+            // [j_min, 1/j_min], in parallel with the dashpot visc dev(D); the
+            // thermal scale is 1 (thermal_expansion is not part of MaterialParams)
+            float j = rmin(rmax(s.f.a * s.f.d - s.f.c * s.f.b, m.ve_jmin), 1.0f / m.ve_jmin);
+            M2 b = mmul(s.f, mtr(s.f));
+            M2 el = mscale(M2{b.a - 1.0f, b.b - 0.0f, b.c - 0.0f, b.d - 1.0f}, m.mu);
+            float lj = m.lambda * logf(j);
+            el = madd(el, M2{lj, 0.0f, 0.0f, lj});
+            M2 d = mscale(madd(s.c, mtr(s.c)), 0.5f);
+            float ht = (d.a + d.d) * 0.5f;
+            M2 ddev = M2{d.a - ht, d.b - 0.0f, d.c - 0.0f, d.d - ht};
+            return madd(el, mscale(ddev, m.ve_visc));
         }
         case kModelSnow: {
             float j = mdet(s.f);
@@ -404,7 +426,8 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
 }
 
 // timestep_bound: elastic.rs:290-327, corotated.rs:108-125, fluid.rs:228-259,
-// sand.rs:345-362, snow.rs:193-211, nacc.rs:279-298 (Fallback: +inf).
+// sand.rs:345-362, von_mises.rs:154-178, viscoelastic.rs:161-190, snow.rs:193-211,
+// nacc.rs:279-298 (Fallback: +inf).
 template <int ONLY = -1>
 EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevConfig& cfg) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
@@ -423,7 +446,18 @@ EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevC
         case kModelSnow:
             return elastic_wave_dt(m.lambda, m.mu, h, density, kMinJ, cfg.cell, cfg.material_cfl);
         case kModelDP:
+        case kModelVonMises:
             return elastic_wave_dt(m.lambda, m.mu, 1.0f, density, kMinJ, cfg.cell, cfg.material_cfl);
+        case kModelViscoelastic: {
+            float e = elastic_wave_dt(m.lambda, m.mu, 1.0f, density, 1.0e-6f, cfg.cell, cfg.material_cfl);
+            float v = kInf;
+            if (m.ve_visc > 0.0f) {
+                float d = rmax(density, 1.0e-6f);
+                float kin = m.ve_visc / d;
+                if (kin > kF32Eps) v = cfg.viscous_cfl * cfg.cell * cfg.cell / kin;
+            }
+            return rmin(e, v);
+        }
         case kModelNacc:
             return elastic_wave_dt(m.wave_lambda, m.mu, h, density, m.min_density, cfg.cell, cfg.material_cfl);
         case kModelFluid: {
@@ -559,14 +593,16 @@ EMH_DEV void sync_volume(PState& s) {
     s.dens = s.mass / v;
 }
 
-// update_particle: elastic.rs:262-269, corotated.rs:84-91, sand.rs:248-319,
-// snow.rs:142-176, fluid.rs:197-207, nacc.rs:255-269; Fallback: no-op.
+// update_particle: elastic.rs:262-269, corotated.rs:84-91, viscoelastic.rs:132-139,
+// sand.rs:248-319, von_mises.rs:108-141, snow.rs:142-176, fluid.rs:197-207,
+// nacc.rs:255-269; Fallback: no-op.
 template <int ONLY = -1>
 EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
     switch (model) {
         case kModelNH:
-        case kModelCorot: {
+        case kModelCorot:
+        case kModelViscoelastic: {
             M2 fp = madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt));
             s.f = mmul(fp, s.f);
             sync_volume(s);
@@ -598,6 +634,34 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
                 float k = sqrtf(m.dp_min_vj / rmax(jn, 1e-6f));
                 n1 = n1 * k;
                 n2 = n2 * k;
+            }
+            s.f = usv(u, n1, n2, vt);
+            sync_volume(s);
+            break;
+        }
+        case kModelVonMises: {
+            // J2 return map on the Hencky strain: the deviatoric part is scaled
+            // back onto 2 mu |dev| = yield + H kappa, the trace is kept
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            M2 u, vt;
+            float2 sg;
+            svd2(ft, u, sg, vt);
+            float e1 = logf(rmax(fabsf(sg.x), kLogClamp)), e2 = logf(rmax(fabsf(sg.y), kLogClamp));
+            float tr = e1 + e2;
+            float ht = tr * 0.5f;
+            float d1 = e1 - ht, d2 = e2 - ht;
+            float dn = len2(d1, d2);
+            float kappa = s.q;
+            float ey = m.vm_yield + m.vm_hmod * kappa;
+            float el = (2.0f * m.mu) * dn;
+            float n1 = sg.x, n2 = sg.y;
+            if (el > ey && dn > kLogClamp) {
+                float den = 2.0f * m.mu + m.vm_hmod;
+                float gamma = den > kF32Eps ? (el - ey) / den : 0.0f;
+                s.q = kappa + gamma;
+                float k = ey / el;
+                n1 = expf(d1 * k + ht);
+                n2 = expf(d2 * k + ht);
             }
             s.f = usv(u, n1, n2, vt);
             sync_volume(s);

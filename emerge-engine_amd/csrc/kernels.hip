@@ -41,6 +41,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstring>
 
 #include "kernels.hpp"
 
@@ -1005,6 +1006,158 @@ __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
 
+// ------------------------------------------------------ diagnostics
+// SimSnapshot (systems/diagnostics/snapshot.rs:227-372) over the device state.
+__device__ __forceinline__ uint32_t dev_order_key(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__global__ __launch_bounds__(kBlock) void k_snap_particles(const Pipeline p, uint32_t cur_buf, DevSnapshot* out,
+                                                           uint32_t* cell_lo, uint32_t* cell_hi, uint32_t* cell_count) {
+    const PBuf b = sel_buf(p, cur_buf);
+    const DevConfig& cfg = p.cfg;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < p.n;
+    double mass = 0.0, mx = 0.0, my = 0.0, ke = 0.0, jp_sum = 0.0, h_sum = 0.0;
+    float speed = 0.0f, pinned_speed = 0.0f;
+    uint32_t min_j = 0xffffffffu, max_j = 0u, min_jp = 0xffffffffu;
+    uint32_t n_valid = 0, n_nonfinite = 0, n_invalid = 0, n_oob = 0, n_jp = 0;
+    if (valid) {
+        const float2 x = b.x[i], v = b.v[i];
+        const float4 c = b.c[i], f = b.f[i];
+        const float m = b.mass[i], v0 = b.v0[i], vol = b.vol[i], dens = b.dens[i], jp = b.jp[i], h = b.h[i];
+        const uint32_t word = b.mat[i];
+        const float j = f.x * f.w - f.z * f.y;  // Mat2::determinant, column-major
+        mass = m;
+        mx = (double)(m * v.x);
+        my = (double)(m * v.y);
+        const float sp = len2(v.x, v.y);
+        speed = sp;
+        ke = (double)((0.5f * m) * (v.x * v.x + v.y * v.y));
+        if (word & kPinnedBit) pinned_speed = sp;
+        if (__builtin_isfinite(j)) {
+            min_j = max_j = dev_order_key(j);
+        }
+        const float lo = (float)(cfg.bt > 0 ? cfg.bt - 1 : 0), hi = (float)(cfg.grid_res > cfg.bt ? cfg.grid_res - cfg.bt : 0);
+        if (x.x < lo || x.x > hi || x.y < lo || x.y > hi) n_oob = 1;
+        if (__builtin_isfinite(x.x) && __builtin_isfinite(x.y)) {  // particle_cell_index
+            const int ix = sat_i32(floorf(x.x)), iy = sat_i32(floorf(x.y));
+            if (ix >= 0 && iy >= 0 && ix < (int)cfg.grid_res && iy < (int)cfg.grid_res) {
+                const uint32_t ci = (uint32_t)ix * cfg.grid_res + (uint32_t)iy;
+                const uint32_t id = word & kMatMask;
+                atomicMin(&cell_lo[ci], id);
+                atomicMax(&cell_hi[ci], id);
+                atomicAdd(&cell_count[ci], 1u);
+            }
+        }
+        if (__builtin_isfinite(jp)) {
+            jp_sum = jp;
+            min_jp = dev_order_key(jp);
+            n_jp = 1;
+        }
+        if (__builtin_isfinite(h)) h_sum = h;
+        const float vals[17] = {x.x, x.y, v.x, v.y, c.x, c.y, c.z, c.w, f.x, f.y, f.z, f.w, m, v0, vol, dens, jp};
+        for (int k = 0; k < 17; ++k) n_nonfinite += __builtin_isfinite(vals[k]) ? 0u : 1u;
+        n_valid = n_nonfinite == 0 ? 1u : 0u;
+        n_invalid = (m <= 0.0f) + (vol <= 0.0f) + (v0 <= 0.0f) + (dens <= 0.0f) + (j <= 0.0f) + (jp <= 0.0f);
+    }
+    mass = wave_sum_d(mass);
+    mx = wave_sum_d(mx);
+    my = wave_sum_d(my);
+    ke = wave_sum_d(ke);
+    jp_sum = wave_sum_d(jp_sum);
+    h_sum = wave_sum_d(h_sum);
+    speed = wave_max(speed);
+    pinned_speed = wave_max(pinned_speed);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        min_j = min(min_j, (uint32_t)__shfl_xor((int)min_j, o, 64));
+        max_j = max(max_j, (uint32_t)__shfl_xor((int)max_j, o, 64));
+        min_jp = min(min_jp, (uint32_t)__shfl_xor((int)min_jp, o, 64));
+    }
+    n_valid = wave_sum(n_valid);
+    n_nonfinite = wave_sum(n_nonfinite);
+    n_invalid = wave_sum(n_invalid);
+    n_oob = wave_sum(n_oob);
+    n_jp = wave_sum(n_jp);
+    if (lane_id() == 0) {
+        atomicAdd(&out->mass, mass);
+        atomicAdd(&out->mom_x, mx);
+        atomicAdd(&out->mom_y, my);
+        atomicAdd(&out->kinetic, ke);
+        atomicAdd(&out->jp_sum, jp_sum);
+        atomicAdd(&out->h_sum, h_sum);
+        if (speed > 0.0f) atomicMax(&out->max_speed, __float_as_uint(speed));
+        if (pinned_speed > 0.0f) atomicMax(&out->max_pinned_speed, __float_as_uint(pinned_speed));
+        atomicMin(&out->min_j, min_j);
+        atomicMax(&out->max_j, max_j);
+        atomicMin(&out->min_jp, min_jp);
+        if (n_valid) atomicAdd(&out->valid, n_valid);
+        if (n_nonfinite) atomicAdd(&out->non_finite, n_nonfinite);
+        if (n_invalid) atomicAdd(&out->invalid, n_invalid);
+        if (n_oob) atomicAdd(&out->out_of_bounds, n_oob);
+        if (n_jp) atomicAdd(&out->jp_finite, n_jp);
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_snap_grid(const Pipeline p, uint32_t set, float dt, float ddt,
+                                                      DevSnapshot* out, uint32_t* cell_lo, uint32_t* cell_hi,
+                                                      uint32_t* cell_count) {
+    const uint32_t R = p.cfg.grid_res;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double gm = 0.0, gx = 0.0, gy = 0.0;
+    float gspeed = 0.0f;
+    uint32_t active = 0, nonfinite = 0, touched = 0, mixed = 0, in_mixed = 0;
+    if (i < R * R) {
+        const float m = p.grid.acc[set][kAccM][i];
+        if (m > 0.0f) {
+            const float2 v = cell_velocity(p, set, dt, ddt, (int)(i / R), (int)(i % R));
+            active = 1;
+            gm = m;
+            gx = (double)(m * v.x);
+            gy = (double)(m * v.y);
+            gspeed = len2(v.x, v.y);
+            nonfinite = (__builtin_isfinite(m) ? 0u : 1u) + (__builtin_isfinite(v.x) ? 0u : 1u) +
+                        (__builtin_isfinite(v.y) ? 0u : 1u);
+        }
+        const uint32_t cnt = cell_count[i];
+        if (cnt) {
+            touched = 1;
+            if (cell_lo[i] != cell_hi[i]) {
+                mixed = 1;
+                in_mixed = cnt;
+            }
+            cell_lo[i] = 0xffffffffu;
+            cell_hi[i] = 0u;
+            cell_count[i] = 0u;
+        }
+    }
+    gm = wave_sum_d(gm);
+    gx = wave_sum_d(gx);
+    gy = wave_sum_d(gy);
+    gspeed = wave_max(gspeed);
+    active = wave_sum(active);
+    nonfinite = wave_sum(nonfinite);
+    touched = wave_sum(touched);
+    mixed = wave_sum(mixed);
+    in_mixed = wave_sum(in_mixed);
+    if (lane_id() == 0) {
+        atomicAdd(&out->grid_mass, gm);
+        atomicAdd(&out->grid_mom_x, gx);
+        atomicAdd(&out->grid_mom_y, gy);
+        if (gspeed > 0.0f) atomicMax(&out->max_grid_speed, __float_as_uint(gspeed));
+        if (active) atomicAdd(&out->active_cells, active);
+        if (nonfinite) atomicAdd(&out->non_finite_grid, nonfinite);
+        if (touched) atomicAdd(&out->touched_cells, touched);
+        if (mixed) atomicAdd(&out->mixed_cells, mixed);
+        if (in_mixed) atomicAdd(&out->particles_in_mixed, in_mixed);
+    }
+}
+
 // ------------------------------------------------- stand-alone cell sort
 __global__ void k_sort_keys(const float2* xy, uint32_t n, uint32_t R, uint32_t tpd, uint32_t* keys, uint32_t* rank,
                             uint32_t* cell_count, uint32_t* tile_count) {
@@ -1117,6 +1270,28 @@ hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream) {
             default: k_substep<kFused, -1><<<nb, kBlock, 0, stream>>>(p, seq); break;
         }
     }
+    return hipGetLastError();
+}
+
+uint32_t order_key(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+float order_key_inverse(uint32_t k) {
+    const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+hipError_t launch_snapshot(const Pipeline& p, uint32_t cur_buf, uint32_t grid_set, float dt, float ddt,
+                           DevSnapshot* out, uint32_t* cell_lo, uint32_t* cell_hi, uint32_t* cell_count,
+                           hipStream_t stream) {
+    if (p.n > 0)
+        k_snap_particles<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p, cur_buf, out, cell_lo, cell_hi, cell_count);
+    const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
+    k_snap_grid<<<div_up(R2, kBlock), kBlock, 0, stream>>>(p, grid_set, dt, ddt, out, cell_lo, cell_hi, cell_count);
     return hipGetLastError();
 }
 

@@ -124,6 +124,24 @@ hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream);
 hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream);
 hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream);
 
+// Device accumulator of the diagnostics snapshot (see launch_snapshot).
+struct DevSnapshot {
+    double mass, mom_x, mom_y, kinetic, jp_sum, h_sum;
+    double grid_mass, grid_mom_x, grid_mom_y;
+    uint32_t max_speed, max_pinned_speed, max_grid_speed;  // f32 bits (non-negative)
+    uint32_t min_j, max_j, min_jp;                        // order-preserving f32 keys
+    uint32_t valid, non_finite, invalid, out_of_bounds, jp_finite;
+    uint32_t active_cells, non_finite_grid, touched_cells, mixed_cells, particles_in_mixed;
+};
+// Diagnostics over the stored particles and the last substep's grid: set
+// `grid_set` updated with `dt` / `ddt`; `cell_lo/hi/count` are R^2 scratch
+// arrays (lo = UINT_MAX, hi = count = 0 between calls; left that way).
+hipError_t launch_snapshot(const Pipeline& p, uint32_t cur_buf, uint32_t grid_set, float dt, float ddt,
+                           DevSnapshot* out, uint32_t* cell_lo, uint32_t* cell_hi, uint32_t* cell_count,
+                           hipStream_t stream);
+uint32_t order_key(float f);       // the key launch_snapshot uses for signed min / max
+float order_key_inverse(uint32_t k);
+
 // Stand-alone cell-key sort (bit-exact stable counting sort).
 hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys,
                                  uint32_t* perm_sorted, uint32_t* scratch_rank, uint32_t* cell_count,

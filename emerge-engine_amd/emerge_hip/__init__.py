@@ -108,6 +108,28 @@ class CStepStats(ctypes.Structure):
                 ("frame_index", ctypes.c_uint64)]
 
 
+class CSnapshot(ctypes.Structure):
+    """SimSnapshot mirror (systems/diagnostics/snapshot.rs:8-80, without timing)."""
+    _fields_ = [("frame_index", ctypes.c_uint64), ("configured_dt", ctypes.c_float), ("effective_dt", ctypes.c_float),
+                ("substeps_last_step", ctypes.c_uint32), ("particle_count", ctypes.c_uint32),
+                ("valid_particle_count", ctypes.c_uint32), ("active_grid_cells", ctypes.c_uint32),
+                ("particles_per_active_cell", ctypes.c_float), ("mixed_material_cell_ratio", ctypes.c_float),
+                ("mixed_material_particle_ratio", ctypes.c_float), ("total_particle_mass", ctypes.c_float),
+                ("total_grid_mass", ctypes.c_float), ("relative_mass_error", ctypes.c_float),
+                ("total_particle_momentum", ctypes.c_float * 2), ("total_grid_momentum", ctypes.c_float * 2),
+                ("relative_momentum_error", ctypes.c_float), ("max_particle_speed", ctypes.c_float),
+                ("max_grid_speed", ctypes.c_float), ("cfl_number", ctypes.c_float),
+                ("min_deformation_j", ctypes.c_float), ("max_deformation_j", ctypes.c_float),
+                ("out_of_bounds_particles", ctypes.c_uint32), ("invalid_physical_particle_values", ctypes.c_uint32),
+                ("non_finite_particle_values", ctypes.c_uint32), ("non_finite_grid_values", ctypes.c_uint32),
+                ("recommended_max_dt_from_velocity_cfl", ctypes.c_float), ("avg_plastic_jacobian", ctypes.c_float),
+                ("min_plastic_jacobian", ctypes.c_float), ("avg_elastic_hardening", ctypes.c_float),
+                ("active_count", ctypes.c_uint32), ("sleeping_count", ctypes.c_uint32),
+                ("vel_clamp_count", ctypes.c_uint32), ("j_projection_count", ctypes.c_uint32),
+                ("sim_time_dropped", ctypes.c_float), ("total_kinetic_energy", ctypes.c_float),
+                ("max_pinned_particle_speed", ctypes.c_float), ("_pad", ctypes.c_uint32)]
+
+
 assert ctypes.sizeof(CConfig) == 116
 assert ctypes.sizeof(CMaterialParams) == 96
 assert ctypes.sizeof(CMaterialExt) == 32
@@ -173,6 +195,8 @@ def load_library() -> ctypes.CDLL:
         "emh_material_drucker_prager": (ctypes.c_int, [ctypes.c_float] * 2 + [P(CMaterialParams),
                                                                               P(CMaterialExt)]),
         "emh_material_snow": (ctypes.c_int, [ctypes.c_float] * 7 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_von_mises": (ctypes.c_int, [ctypes.c_float] * 4 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_viscoelastic": (ctypes.c_int, [ctypes.c_float] * 3 + [P(CMaterialParams), P(CMaterialExt)]),
         "emh_material_snow_from_young": (ctypes.c_int, [ctypes.c_float] * 2 + [P(CMaterialParams),
                                                                                P(CMaterialExt)]),
         "emh_material_nacc": (ctypes.c_int, [ctypes.c_float] * 5 + [P(CMaterialParams), P(CMaterialExt)]),
@@ -194,6 +218,7 @@ def load_library() -> ctypes.CDLL:
         "emh_sim_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "emh_sim_stats": (ctypes.c_int, [ctypes.c_void_p, P(CStepStats)]),
         "emh_sim_get_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "emh_sim_snapshot": (ctypes.c_int, [ctypes.c_void_p, P(CSnapshot)]),
         "emh_sim_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "emh_sim_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
@@ -473,6 +498,45 @@ class DruckerPragerMaterial(Material):
         return self
 
 
+class VonMisesMaterial(Material):
+    """VonMisesMaterial (matter/materials/von_mises.rs): J2 plasticity on the
+    Hencky strain, yield stress + H * friction_hardening."""
+
+    @classmethod
+    def new(cls, lam: float, mu: float, yield_stress: float) -> "VonMisesMaterial":
+        return cls.with_hardening(lam, mu, yield_stress, 0.0)
+
+    @classmethod
+    def with_hardening(cls, lam: float, mu: float, yield_stress: float, hardening_modulus: float) -> "VonMisesMaterial":
+        return cls(*_build("emh_material_von_mises", lam, mu, yield_stress, hardening_modulus))
+
+    @classmethod
+    def from_young_modulus(cls, e: float, nu: float, yield_stress: float) -> "VonMisesMaterial":
+        return cls.new(*lame_from_young(e, nu), yield_stress)
+
+    @classmethod
+    def soft_ductile(cls, e: float, nu: float) -> "VonMisesMaterial":
+        """von_mises.rs:71-74: yield stress E / 200."""
+        return cls.from_young_modulus(e, nu, e * 0.005)
+
+
+class ViscoelasticMaterial(Material):
+    """ViscoelasticMaterial (matter/materials/viscoelastic.rs): Kelvin-Voigt,
+    neo-Hookean spring in parallel with a deviatoric dashpot."""
+
+    @classmethod
+    def new(cls, lam: float, mu: float, viscosity: float) -> "ViscoelasticMaterial":
+        return cls(*_build("emh_material_viscoelastic", lam, mu, viscosity))
+
+    @classmethod
+    def from_young_modulus(cls, e: float, nu: float, viscosity: float) -> "ViscoelasticMaterial":
+        return cls.new(*lame_from_young(e, nu), viscosity)
+
+    @classmethod
+    def near_incompressible(cls, e: float, viscosity: float) -> "ViscoelasticMaterial":
+        return cls.from_young_modulus(e, 0.45, viscosity)
+
+
 class StomakhinMaterial(Material):
     """StomakhinMaterial (matter/materials/snow.rs)."""
 
@@ -633,6 +697,18 @@ class Simulation:
         n = ctypes.c_uint32()
         _check(load_library().emh_sim_kernel_times(self._h, arr, 16, ctypes.byref(n)))
         return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(min(n.value, 16))}
+
+    def diagnostics_snapshot(self) -> dict:
+        """Simulation::diagnostics_snapshot (solver/queries.rs:15-30) as a dict."""
+        out = CSnapshot()
+        _check(load_library().emh_sim_snapshot(self._h, ctypes.byref(out)))
+        d = {}
+        for name, _ in CSnapshot._fields_:
+            if name == "_pad":
+                continue
+            v = getattr(out, name)
+            d[name] = tuple(v) if isinstance(v, ctypes.Array) else v
+        return d
 
     def grid(self) -> Tuple[np.ndarray, np.ndarray]:
         """(mass[R*R], momentum[R*R, 2]) of the last substep's P2G, flat index x*R+y."""

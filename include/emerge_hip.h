@@ -50,6 +50,8 @@ extern "C" {
 #define EMH_MODEL_COROTATED       3u
 #define EMH_MODEL_SNOW            4u
 #define EMH_MODEL_DRUCKER_PRAGER  5u
+#define EMH_MODEL_VON_MISES       6u
+#define EMH_MODEL_VISCOELASTIC    9u
 #define EMH_MODEL_NACC           10u
 
 #define EMH_MAX_MATERIALS  64u   /* MAX_MATERIAL_SLOTS, materials/registry.rs:6 */
@@ -91,7 +93,8 @@ typedef struct emh_config {
  * Field meaning is per model, exactly as each model's `params()` fills it:
  *   NeoHookean  elastic.rs:275-288   Corotated corotated.rs:97-106
  *   Fluid       fluid.rs:209-226     DruckerPrager sand.rs:321-343
- *   Snow        snow.rs:178-191
+ *   Snow        snow.rs:178-191      VonMises von_mises.rs:143-152
+ *   Viscoelastic viscoelastic.rs:149-159 (active_stress_coeff must be 0)
  *   Nacc        nacc.rs:300-314 uploads model=NeoHookean; this library needs
  *               model=EMH_MODEL_NACC (10) instead, with the same field use
  *               (lambda = kappa - mu, hardening_exponent = xi,
@@ -193,6 +196,52 @@ typedef struct emh_step_stats {
     uint64_t frame_index;             /* step.rs:59 */
 } emh_step_stats;
 
+/* ---- diagnostics: SimSnapshot (src/systems/diagnostics/snapshot.rs:8-80),
+ * the fields Simulation::diagnostics_snapshot fills
+ * (src/spacetime/solver/queries.rs:15-30), without the wall-clock timing.
+ * Particle sums are accumulated in f64 (the reference sums in f32 in
+ * particle order); the grid fields are those of the last substep's updated
+ * grid.  Sleeping particles do not exist on this path (sleeping_count = 0). */
+typedef struct emh_snapshot {
+    uint64_t frame_index;
+    float    configured_dt;
+    float    effective_dt;
+    uint32_t substeps_last_step;
+    uint32_t particle_count;
+    uint32_t valid_particle_count;
+    uint32_t active_grid_cells;
+    float    particles_per_active_cell;
+    float    mixed_material_cell_ratio;
+    float    mixed_material_particle_ratio;
+    float    total_particle_mass;
+    float    total_grid_mass;
+    float    relative_mass_error;
+    float    total_particle_momentum[2];
+    float    total_grid_momentum[2];
+    float    relative_momentum_error;
+    float    max_particle_speed;
+    float    max_grid_speed;
+    float    cfl_number;
+    float    min_deformation_j;
+    float    max_deformation_j;
+    uint32_t out_of_bounds_particles;
+    uint32_t invalid_physical_particle_values;
+    uint32_t non_finite_particle_values;
+    uint32_t non_finite_grid_values;
+    float    recommended_max_dt_from_velocity_cfl;
+    float    avg_plastic_jacobian;
+    float    min_plastic_jacobian;
+    float    avg_elastic_hardening;
+    uint32_t active_count;
+    uint32_t sleeping_count;
+    uint32_t vel_clamp_count;
+    uint32_t j_projection_count;
+    float    sim_time_dropped;
+    float    total_kinetic_energy;
+    float    max_pinned_particle_speed;
+    uint32_t _pad;
+} emh_snapshot;
+
 typedef struct emh_sim emh_sim;
 
 /* ---- library ------------------------------------------------------------ */
@@ -233,6 +282,14 @@ int emh_material_snow(float lambda, float mu, float hardening_exponent, float co
 /* StomakhinMaterial::from_young_modulus  snow.rs:68-71 */
 int emh_material_snow_from_young(float young_modulus, float poisson_ratio,
                                  emh_material_params* p, emh_material_ext* e);
+/* VonMisesMaterial::with_hardening(lambda, mu, yield_stress, H)  von_mises.rs:60-69;
+ * ::new is H = 0 (von_mises.rs:49-56).  H < 0 is EMH_ERR_INVALID_ARG (the
+ * reference asserts). */
+int emh_material_von_mises(float lambda, float mu, float yield_stress, float hardening_modulus,
+                           emh_material_params* p, emh_material_ext* e);
+/* ViscoelasticMaterial::new(lambda, mu, viscosity)  viscoelastic.rs:53-62 (j_min 0.01) */
+int emh_material_viscoelastic(float lambda, float mu, float viscosity, emh_material_params* p,
+                              emh_material_ext* e);
 /* NaccMaterial::new(mu, kappa, friction M, cohesion beta, hardening xi)  nacc.rs:63-73 */
 int emh_material_nacc(float mu, float kappa, float friction, float cohesion, float hardening_factor,
                       emh_material_params* p, emh_material_ext* e);
@@ -278,6 +335,9 @@ int emh_sim_stats(const emh_sim* sim, emh_step_stats* out);
  * x * grid_res + y as Grid uses (grid/mod.rs:89-99).  mass: grid_res^2 floats,
  * momentum: 2 * grid_res^2 floats (x, y interleaved).  Either may be NULL. */
 int emh_sim_get_grid(emh_sim* sim, float* mass, float* momentum, size_t n_cells);
+/* Simulation::diagnostics_snapshot (src/spacetime/solver/queries.rs:15-30),
+ * computed on the device. */
+int emh_sim_snapshot(emh_sim* sim, emh_snapshot* out);
 /* Wait for all queued device work of this handle. */
 int emh_sim_synchronize(emh_sim* sim);
 

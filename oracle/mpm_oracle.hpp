@@ -312,6 +312,8 @@ template <class T> struct Material {
       max_plastic_jacobian = 0, cohesion_coeff = 0;
     // NACC (nacc.rs:34-60)
     T kappa = 0, friction_m = 0, cohesion_beta = 0, hardening_factor = 0;
+    // VonMises (von_mises.rs:29-39), Viscoelastic (viscoelastic.rs:9-16)
+    T yield_stress = 0, hardening_modulus = 0, ve_viscosity = 0, ve_j_min = 0;
     bool hardening_enabled = false;
     bool nacc_advance_f = false;
 
@@ -364,6 +366,15 @@ template <class T> struct Material {
                 m.min_plastic_jacobian = T(p.volume_ratio_min);
                 m.max_plastic_jacobian = T(p.volume_ratio_max);
                 m.cohesion_coeff = T(p.cohesion_coeff);
+                break;
+            case EMH_MODEL_VON_MISES:  // von_mises.rs:150-159
+                m.yield_stress = T(p.hardening_exponent);
+                m.hardening_modulus = T(p.hardening_modulus);
+                break;
+            case EMH_MODEL_VISCOELASTIC:  // viscoelastic.rs:149-159
+                m.ve_viscosity = T(p.dynamic_viscosity);
+                m.active_stress_coeff = T(p.active_stress_coeff);
+                m.ve_j_min = T(p.volume_ratio_min);
                 break;
             case EMH_MODEL_NACC: {
                 m.min_density = T(min_density);
@@ -452,13 +463,28 @@ template <class T> struct Material {
                 M2<T> f_t = transpose(f);
                 return ((T(2) * mu_eff) * (f - r)) * f_t + ((lambda_eff * (j - T(1))) * j) * m2_identity<T>();
             }
-            case EMH_MODEL_DRUCKER_PRAGER: {  // sand.rs:220-231
+            case EMH_MODEL_DRUCKER_PRAGER:  // sand.rs:220-231
+            case EMH_MODEL_VON_MISES: {     // von_mises.rs:96-104 (the same corotated form)
                 M2<T> f = P.f[i];
                 T j = det(f);
                 if (j <= T(MIN_J)) return m2_zero<T>();
                 M2<T> r = polar_decomposition_2d(f);
                 M2<T> f_t = transpose(f);
                 return ((T(2) * mu) * (f - r)) * f_t + ((lambda * (j - T(1))) * j) * m2_identity<T>();
+            }
+            case EMH_MODEL_VISCOELASTIC: {  // viscoelastic.rs:103-130 (temperature scale 1: no thermal term)
+                M2<T> f = P.f[i];
+                T j = rmin(rmax(f.m00 * f.m11 - f.m10 * f.m01, ve_j_min), T(1) / ve_j_min);
+                const T t_scale = T(1);
+                T mu_t = mu * t_scale, lambda_t = lambda * t_scale;
+                M2<T> b = f * transpose(f);
+                M2<T> elastic = mu_t * (b - m2_identity<T>()) + m2_diag(lambda_t * o_ln(j), lambda_t * o_ln(j));
+                M2<T> c = P.c[i];
+                M2<T> d = (c + transpose(c)) * T(0.5);
+                T trace = d.m00 + d.m11;
+                M2<T> d_dev = d - m2_diag(trace * T(0.5), trace * T(0.5));
+                M2<T> viscous = (ve_viscosity * t_scale) * d_dev;
+                return elastic + viscous;
             }
             case EMH_MODEL_SNOW: {  // snow.rs:109-136
                 M2<T> f = P.f[i];
@@ -516,7 +542,9 @@ template <class T> struct Material {
     }
 
     T activation_scale() const {
-        if (model == EMH_MODEL_NEOHOOKEAN || model == EMH_MODEL_COROTATED) return active_stress_coeff;
+        // elastic.rs, corotated.rs, viscoelastic.rs:145-147 report their coefficient
+        if (model == EMH_MODEL_NEOHOOKEAN || model == EMH_MODEL_COROTATED || model == EMH_MODEL_VISCOELASTIC)
+            return active_stress_coeff;
         return T(0);
     }
 
@@ -539,7 +567,18 @@ template <class T> struct Material {
             case EMH_MODEL_SNOW:
                 return elastic_wave_dt(lambda, mu, hardening_scale, density, T(MIN_J), cell_width, material_cfl);
             case EMH_MODEL_DRUCKER_PRAGER:
+            case EMH_MODEL_VON_MISES:  // von_mises.rs:161-178
                 return elastic_wave_dt(lambda, mu, T(1), density, T(MIN_J), cell_width, material_cfl);
+            case EMH_MODEL_VISCOELASTIC: {  // viscoelastic.rs:161-190
+                T elastic_dt = elastic_wave_dt(lambda, mu, T(1), density, T(1.0e-6f), cell_width, material_cfl);
+                T viscous_dt = inf;
+                if (ve_viscosity > T(0)) {
+                    T d = rmax(density, T(1.0e-6f));
+                    T kinematic = ve_viscosity / d;
+                    if (kinematic > T(F32_EPSILON)) viscous_dt = viscous_cfl * cell_width * cell_width / kinematic;
+                }
+                return rmin(elastic_dt, viscous_dt);
+            }
             case EMH_MODEL_NACC:
                 return elastic_wave_dt(kappa - mu, mu, hardening_scale, density, min_density, cell_width,
                                        material_cfl);
@@ -658,7 +697,8 @@ template <class T> struct Material {
     void update_particle(Particles<T>& P, size_t i, T dt) const {
         switch (model) {
             case EMH_MODEL_NEOHOOKEAN:
-            case EMH_MODEL_COROTATED: {  // elastic.rs:262-269, corotated.rs:84-91
+            case EMH_MODEL_COROTATED:      // elastic.rs:262-269, corotated.rs:84-91
+            case EMH_MODEL_VISCOELASTIC: {  // viscoelastic.rs:132-139
                 M2<T> fp_new = m2_identity<T>() + dt * P.c[i];
                 P.f[i] = fp_new * P.f[i];
                 T j = rmax(det(P.f[i]), T(MIN_J));
@@ -695,6 +735,35 @@ template <class T> struct Material {
                     new_sigma = new_sigma * s;
                 }
                 P.f[i] = (u * m2_diag(new_sigma.x, new_sigma.y)) * vt;
+                T j = rmax(det(P.f[i]), T(MIN_J));
+                T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
+                P.volume[i] = v;
+                P.density[i] = P.mass[i] / v;
+                break;
+            }
+            case EMH_MODEL_VON_MISES: {  // von_mises.rs:113-147
+                M2<T> f_trial = (m2_identity<T>() + dt * P.c[i]) * P.f[i];
+                M2<T> u, vt;
+                V2<T> sigma;
+                svd2(f_trial, u, sigma, vt);
+                // hencky_strains  utils.rs:58-61
+                V2<T> eps{o_ln(rmax(std::fabs(sigma.x), T(LOG_CLAMP))), o_ln(rmax(std::fabs(sigma.y), T(LOG_CLAMP)))};
+                T tr = eps.x + eps.y;
+                V2<T> dev{eps.x - tr * T(0.5), eps.y - tr * T(0.5)};
+                T dev_norm = length(dev);
+                T kappa = P.friction_hardening[i];
+                T effective_yield = yield_stress + hardening_modulus * kappa;
+                T elastic_dev = (T(2) * mu) * dev_norm;
+                V2<T> sigma_new = sigma;
+                if (elastic_dev > effective_yield && dev_norm > T(LOG_CLAMP)) {
+                    T denom = T(2) * mu + hardening_modulus;
+                    T gamma = denom > T(F32_EPSILON) ? (elastic_dev - effective_yield) / denom : T(0);
+                    P.friction_hardening[i] = kappa + gamma;
+                    T k = effective_yield / elastic_dev;
+                    V2<T> eps_proj{dev.x * k + tr * T(0.5), dev.y * k + tr * T(0.5)};
+                    sigma_new = {o_exp(eps_proj.x), o_exp(eps_proj.y)};
+                }
+                P.f[i] = (u * m2_diag(sigma_new.x, sigma_new.y)) * vt;
                 T j = rmax(det(P.f[i]), T(MIN_J));
                 T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
                 P.volume[i] = v;
@@ -1136,7 +1205,7 @@ template <class T> struct Simulation {
         V2<T> n = P.activation_dir[i];
         T len_sq = dot(n, n);
         M2<T> tau_active;
-        if (len_sq > T(F32_EPSILON)) {
+        if (m.model != EMH_MODEL_VISCOELASTIC && len_sq > T(F32_EPSILON)) {  // Viscoelastic: isotropic (transfer.rs:87-89)
             V2<T> n0 = n / o_sqrt(len_sq);
             M2<T> n_outer = m2_cols(n0 * n0.x, n0 * n0.y);
             M2<T> a_mat = n_outer * a;

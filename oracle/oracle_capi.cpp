@@ -383,6 +383,28 @@ void orc_material_corotated(float lambda, float mu, emh_material_params* p, emh_
     p->mu = mu;
 }
 
+void orc_material_von_mises(float lambda, float mu, float yield_stress, float hardening_modulus,
+                            emh_material_params* p, emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_VON_MISES;  // von_mises.rs:47-70, 150-159
+    p->lambda = lambda;
+    p->mu = mu;
+    p->hardening_exponent = yield_stress;
+    p->hardening_modulus = hardening_modulus;
+}
+
+void orc_material_viscoelastic(float lambda, float mu, float viscosity, emh_material_params* p,
+                               emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_VISCOELASTIC;  // viscoelastic.rs:47-56, 149-159
+    p->lambda = lambda;
+    p->mu = mu;
+    p->dynamic_viscosity = viscosity;
+    p->volume_ratio_min = 0.01f;
+}
+
 void orc_material_fluid(float rest_density, float visc, float eos_k, float eos_p, emh_material_params* p,
                         emh_material_ext* e) {
     std::memset(p, 0, sizeof(*p));

@@ -83,6 +83,8 @@ def load() -> ctypes.CDLL:
         "orc_to_radians": (f, [f]),
         "orc_material_neohookean": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_corotated": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_von_mises": (None, [f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_viscoelastic": (None, [f, f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_fluid": (None, [f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_drucker_prager": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_snow": (None, [f] * 7 + [P(CMaterialParams), P(CMaterialExt)]),

@@ -67,6 +67,18 @@ class Mats:
             return eh.StomakhinMaterial.new(lam, mu, 10.0, 0.025, 0.0075, 0.6, 20.0)
         return self._o.material("snow", lam, mu, 10.0, 0.025, 0.0075, 0.6, 20.0)
 
+    def von_mises(self, e, nu, yield_stress, hardening_modulus=0.0):  # von_mises.rs:43-69
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.VonMisesMaterial.with_hardening(lam, mu, yield_stress, hardening_modulus)
+        return self._o.material("von_mises", lam, mu, yield_stress, hardening_modulus)
+
+    def viscoelastic(self, e, nu, viscosity):  # viscoelastic.rs:64-67
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.ViscoelasticMaterial.new(lam, mu, viscosity)
+        return self._o.material("viscoelastic", lam, mu, viscosity)
+
     def nacc(self, e, nu, m_slope, beta, xi, advance_f=False):  # NaccMaterial::from_young_modulus
         lam, mu = self.lame(e, nu)
         kappa = float(np.float32(lam) + np.float32(mu))
@@ -172,6 +184,39 @@ def nacc_block(backend: str, advance_f=True):
     return sim
 
 
+def _add_velocity(sim, v):
+    """Adds a uniform velocity to every particle (a thrown body)."""
+    p = sim.particles()
+    p["v"] = p["v"] + np.array(v, np.float32)
+    sim.set_particles(p)
+    return sim
+
+
+def ductile_drop(backend: str):
+    """Von Mises clay (E = 5e4, nu = 0.35, yield stress E / 200, hardening H = 2e3)
+    thrown at the floor so that it flows plastically."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.von_mises(5e4, 0.35, 250.0, 2e3))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(12, 10), box_center=(32.0, 9.0), rng_seed=17,
+                                position_jitter=0.3, initial_velocity_scale=1.0))
+    return _add_velocity(sim, (1.0, -6.0))
+
+
+def viscoelastic_drop(backend: str):
+    """Kelvin-Voigt body (E = 2e4, nu = 0.45, viscosity 20) landing and ringing down."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.viscoelastic(2e4, 0.45, 20.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(10, 10), box_center=(30.0, 9.0), rng_seed=19,
+                                position_jitter=0.2, initial_velocity_scale=1.0))
+    return _add_velocity(sim, (0.0, -6.0))
+
+
 def multi_material(backend: str):
     """BASELINE config 3 in miniature: NeoHookean, water, snow and corotated
     boxes with N(0, 0.5)-like random velocities, one grid."""
@@ -200,6 +245,8 @@ SCENES: Dict[str, Scene] = {
     "snow_drop": Scene("snow_drop", 10, snow_drop),
     "corotated_drop": Scene("corotated_drop", 20, corotated_drop),
     "nacc_block": Scene("nacc_block", 10, nacc_block),
+    "ductile_drop": Scene("ductile_drop", 10, ductile_drop),
+    "viscoelastic_drop": Scene("viscoelastic_drop", 20, viscoelastic_drop),
     "multi_material": Scene("multi_material", 10, multi_material),
 }
 

@@ -79,6 +79,10 @@ def test_material_constructors_match_oracle(oracle):
         (eh.StomakhinMaterial.new(1.0, 2.0, 10.0, 0.025, 0.0075, 0.6, 20.0),
          oracle.material("snow", 1.0, 2.0, 10.0, 0.025, 0.0075, 0.6, 20.0)),
         (eh.NaccMaterial.new(76923.08, 192307.7, 1.85, 0.5, 0.8), oracle.material("nacc", 76923.08, 192307.7, 1.85, 0.5, 0.8)),
+        (eh.VonMisesMaterial.new(2000.0, 3000.0, 100.0), oracle.material("von_mises", 2000.0, 3000.0, 100.0, 0.0)),
+        (eh.VonMisesMaterial.with_hardening(2000.0, 3000.0, 100.0, 500.0),
+         oracle.material("von_mises", 2000.0, 3000.0, 100.0, 500.0)),
+        (eh.ViscoelasticMaterial.new(1000.0, 800.0, 50.0), oracle.material("viscoelastic", 1000.0, 800.0, 50.0)),
     ]
     for a, b in pairs:
         assert _bytes(a) == _bytes(b), (a, b)
@@ -100,6 +104,21 @@ def test_material_params_field_use():
     na = eh.NaccMaterial.new(3000.0, 5000.0, 1.2, 0.1, 2.0)
     assert na.params.model == 10 and na.params.lambda_ == 2000.0 and na.params.stretch_limit == np.float32(1.2)
     assert na.ext.nacc_kappa == 5000.0 and na.ext.nacc_hardening_enabled == 1
+
+
+def test_new_model_params_field_use():
+    """von_mises.rs:143-152 (yield stress in hardening_exponent), viscoelastic.rs:149-159
+    (j_min in volume_ratio_min), and the with_hardening assert (von_mises.rs:61)."""
+    vm = eh.VonMisesMaterial.with_hardening(2000.0, 3000.0, 100.0, 500.0)
+    assert vm.params.model == 6 and vm.params.hardening_exponent == 100.0 and vm.params.hardening_modulus == 500.0
+    sd = eh.VonMisesMaterial.soft_ductile(5e4, 0.35)
+    assert sd.params.hardening_exponent == np.float32(250.0)
+    ve = eh.ViscoelasticMaterial.new(1000.0, 800.0, 50.0)
+    assert ve.params.model == 9 and ve.params.dynamic_viscosity == 50.0
+    assert ve.params.volume_ratio_min == np.float32(0.01) and ve.params.active_stress_coeff == 0.0
+    with pytest.raises(eh.EmergeError) as ei:
+        eh.VonMisesMaterial.with_hardening(2000.0, 3000.0, 100.0, -1.0)
+    assert ei.value.code == -1
 
 
 def test_cell_key_matches_oracle(oracle):

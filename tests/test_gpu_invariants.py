@@ -95,8 +95,14 @@ def test_isolated_other_models_close(oracle):
         "snow": lambda m: m.snow(1.4e5, 0.2),
         "fluid": lambda m: m.fluid(1.0, 0.1, 100.0, 7.0),
         "nacc": lambda m: m.nacc(2e5, 0.3, 1.85, 0.5, 0.8, advance_f=True),
+        "von_mises": lambda m: m.von_mises(5e4, 0.35, 250.0, 2e3),
+        "viscoelastic": lambda m: m.viscoelastic(2e4, 0.45, 20.0),
     }
+    # start these two strained: Von Mises beyond its yield surface (the return
+    # map runs), the Kelvin-Voigt body with shear and volume change
+    start_f = {"von_mises": (1.02, 0.0, 0.0, 0.98), "viscoelastic": (1.05, 0.01, 0.02, 0.97)}
     for name, fn in makers.items():
+        pts["deformation_gradient"] = start_f.get(name, (1.0, 0.0, 0.0, 1.0))
         g, o = _pair(fn, cfg, pts, n_steps=2)
         o64 = S.new_sim("oracle64", cfg).with_default_material(fn(S.Mats("oracle64")))
         o64.set_particles(pts)
@@ -176,9 +182,14 @@ def test_error_paths():
     cfg = eh.SimConfig.standard(64, 0.05, (0.0, -1.0))
     sim = eh.Simulation.empty(cfg)
     bad = eh.NeoHookeanMaterial.new(1.0, 1.0)
-    bad.params.model = 6  # VonMises: not on the GPU path
+    bad.params.model = 7  # Rankine: not on the GPU path
     with pytest.raises(eh.EmergeError) as ei:
         sim.set_material(0, bad)
+    assert ei.value.code == -2
+    active = eh.ViscoelasticMaterial.new(1.0, 1.0, 1.0)
+    active.params.active_stress_coeff = 0.5  # the particle activation field is not carried
+    with pytest.raises(eh.EmergeError) as ei:
+        sim.set_material(0, active)
     assert ei.value.code == -2
     with pytest.raises(eh.EmergeError) as ei:
         sim.set_material(3, eh.NeoHookeanMaterial.new(1.0, 1.0))  # not contiguous (registry.rs:39-46)
@@ -252,3 +263,45 @@ def test_profiling_counts_substeps():
     assert t["step_setup"][0] == 3
     assert t["k_density_p2g+k_substep"][0] == sim.stats().total_substeps - s0
     assert all(ms > 0.0 for _, ms in t.values())
+
+
+@pytest.mark.gpu
+def test_diagnostics_snapshot_matches_state():
+    """diagnostics_snapshot (snapshot.rs:227-372) against the same sums taken
+    in float64 over the downloaded particles, and against the grid."""
+    sim = S.multi_material("gpu")
+    sim.step_n(3)
+    d = sim.diagnostics_snapshot()
+    p = sim.particles()
+    n = len(p)
+    m = p["mass"].astype(np.float64)
+    v = p["v"].astype(np.float64)
+    F = p["deformation_gradient"].astype(np.float64)
+    J = F[:, 0] * F[:, 3] - F[:, 2] * F[:, 1]
+    assert d["particle_count"] == n and d["valid_particle_count"] == n and d["non_finite_particle_values"] == 0
+    assert abs(d["total_particle_mass"] - m.sum()) <= 1e-6 * m.sum()
+    mom = (m[:, None] * v).sum(0)
+    scale = (m[:, None] * np.abs(v)).sum()
+    assert np.max(np.abs(np.array(d["total_particle_momentum"]) - mom)) <= 1e-6 * scale
+    ke = (0.5 * m * (v ** 2).sum(1)).sum()
+    assert abs(d["total_kinetic_energy"] - ke) <= 1e-5 * ke + 1e-12
+    assert d["max_particle_speed"] == pytest.approx(np.sqrt((v ** 2).sum(1)).max(), rel=1e-6)
+    assert d["min_deformation_j"] == pytest.approx(J.min(), rel=1e-5)
+    assert d["max_deformation_j"] == pytest.approx(J.max(), rel=1e-5)
+    gm, _ = sim.grid()
+    assert d["total_grid_mass"] == pytest.approx(gm.astype(np.float64).sum(), rel=1e-6)
+    assert d["active_grid_cells"] == int((gm > 0).sum())
+    assert d["relative_mass_error"] < 1e-5
+    # materials per cell of floor(x)
+    cells = np.floor(p["x"]).astype(np.int64)
+    r = sim.config.grid_res
+    key = cells[:, 0] * r + cells[:, 1]
+    lo, hi = {}, {}
+    for k, mid in zip(key, p["material_id"]):
+        lo[k] = min(lo.get(k, mid), mid)
+        hi[k] = max(hi.get(k, mid), mid)
+    mixed = {k for k in lo if lo[k] != hi[k]}
+    assert d["mixed_material_cell_ratio"] == pytest.approx(len(mixed) / len(lo), rel=1e-6)
+    assert d["mixed_material_particle_ratio"] == pytest.approx(np.isin(key, list(mixed)).sum() / n, rel=1e-6)
+    st = sim.stats()
+    assert d["substeps_last_step"] == st.last_substeps and d["frame_index"] == st.frame_index == 3

@@ -255,6 +255,134 @@ def test_snow_clamp_exact(oracle):
     assert abs(out["hardening_scale"] - min(max(math.exp(10 * (1 - jp)), 0.1), 7.0)) < 1e-4
 
 
+# -------------------------------------------------------------- Von Mises
+def _vm_state(mat_mu, yield_stress, factor, trace):
+    """von_mises.rs:207-209,213-220: ε = (tr/2 + d, tr/2 − d) with |dev ε| = factor · σY / (2µ)."""
+    d = factor * yield_stress / (2.0 * mat_mu) / math.sqrt(2.0)
+    return math.exp(trace * 0.5 + d), math.exp(trace * 0.5 - d)
+
+
+def _vm_step(m, oracle, s1, s2, kappa=0.0):  # von_mises.rs:187-200
+    out = oracle.update_particle(m, _particle([s1, 0, 0, s2], q=kappa), 1.0)
+    F = out["deformation_gradient"]
+    return np.array([F[0], F[3]], np.float64), float(out["friction_hardening"]), F
+
+
+def test_von_mises_inside_yield_is_elastic(oracle):
+    """von_mises.rs:211-233: 99 % of the yield threshold leaves σ and κ unchanged."""
+    m = oracle.material("von_mises", 2000.0, 3000.0, 100.0, 0.0)
+    s1, s2 = _vm_state(3000.0, 100.0, 0.99, 0.0)
+    sig, kappa, _ = _vm_step(m, oracle, s1, s2)
+    assert np.linalg.norm(sig - [s1, s2]) < 1e-4 and kappa == 0.0
+
+
+def test_von_mises_projects_onto_yield_surface(oracle):
+    """von_mises.rs:235-272: 150 % of the threshold lands on |dev ε| = σY/(2µ),
+    keeps tr ε = 0.4, and κ grows by γ = (2µ|dev| − σY)/(2µ + H)."""
+    m = oracle.material("von_mises", 2000.0, 3000.0, 100.0, 0.0)
+    s1, s2 = _vm_state(3000.0, 100.0, 1.5, 0.4)
+    sig, kappa, F = _vm_step(m, oracle, s1, s2)
+    eps = np.log(np.abs(sig))
+    tr = eps.sum()
+    dev = np.linalg.norm(eps - tr / 2)
+    assert abs(dev - 100.0 / 6000.0) < 1e-4 and abs(tr - 0.4) < 1e-4
+    # the closed form of the increment: 2µ|dev_trial| = 1.5 σY
+    assert abs(kappa - 0.5 * 100.0 / 6000.0) < 1e-5
+    assert F[1] == 0.0 and F[2] == 0.0  # a diagonal trial state stays diagonal
+
+
+def test_von_mises_hardening_shrinks_increment(oracle):
+    """von_mises.rs:274-297: with H > 0 the same trial state yields a smaller γ once κ = 1,
+    and γ = (2µ|dev| − σY − Hκ)/(2µ + H) where it yields."""
+    H = 500.0
+    m = oracle.material("von_mises", 2000.0, 3000.0, 100.0, H)
+    s1, s2 = _vm_state(3000.0, 100.0, 1.5, 0.0)
+    _, k0, _ = _vm_step(m, oracle, s1, s2, 0.0)
+    _, k1, _ = _vm_step(m, oracle, s1, s2, 1.0)
+    assert k1 - 1.0 < k0
+    el = 1.5 * 100.0
+    assert abs(k0 - (el - 100.0) / (6000.0 + H)) < 1e-6
+    assert k1 == 1.0  # σY + Hκ = 600 > 150: elastic
+    _, kh, _ = _vm_step(m, oracle, s1, s2, 0.05)
+    assert abs((kh - 0.05) - (el - 100.0 - H * 0.05) / (6000.0 + H)) < 1e-6
+
+
+def test_von_mises_stress_is_corotated(oracle):
+    """von_mises.rs:94-102: the stress is DP's corotated τ = 2µ(F−R)Fᵀ + λ(J−1)J·I."""
+    f = [1.1, 0.3, -0.2, 0.9]
+    a = oracle.kirchhoff(oracle.material("von_mises", 400.0, 200.0, 50.0, 0.0), _particle(f))
+    b = oracle.kirchhoff(oracle.material("drucker_prager", 400.0, 200.0), _particle(f))
+    assert np.array_equal(a, b)
+
+
+# ----------------------------------------------------------- Viscoelastic
+def test_viscoelastic_small_strain_is_hooke(oracle):
+    """viscoelastic.rs:219-249: F = I + 1e-4·diag(1, −0.4), C = 0 gives 2µε + λ tr ε I to 1e-3."""
+    m = oracle.material("viscoelastic", 1000.0, 800.0, 50.0)
+    E = np.diag([1.0, -0.4])
+    F = np.eye(2) + 1e-4 * E
+    tau = _m(oracle.kirchhoff(m, _particle([F[0, 0], F[1, 0], F[0, 1], F[1, 1]])))
+    pred = _hooke(1000.0, 800.0, 1e-4 * E)
+    assert np.linalg.norm(tau - pred) / np.linalg.norm(pred) < 1e-3
+
+
+def test_viscoelastic_viscous_term_exact(oracle):
+    """viscoelastic.rs:251-278: F = I isolates τ = η·dev(sym C); pure shear C gives η·C."""
+    m = oracle.material("viscoelastic", 1000.0, 800.0, 50.0)
+    tau = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=[0.0, 3.0, 3.0, 0.0])))
+    assert np.linalg.norm(tau - 50.0 * _m([0.0, 3.0, 3.0, 0.0])) < 1e-4
+    # a general C: only the symmetric deviatoric part survives (the spin and the trace drop out)
+    c = [0.4, -1.0, 2.0, 0.1]
+    tau = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=c)))
+    C = _m(c)
+    D = 0.5 * (C + C.T)
+    np.testing.assert_allclose(tau, 50.0 * (D - 0.5 * np.trace(D) * np.eye(2)), atol=1e-4)
+
+
+def test_viscoelastic_terms_are_additive(oracle):
+    """viscoelastic.rs:280-310: τ(F, C) = τ(F, 0) + τ(I, C) to 1e-3 at a finite strain."""
+    m = oracle.material("viscoelastic", 1000.0, 800.0, 50.0)
+    F = np.eye(2) + 0.02 * np.diag([1.0, -0.3])
+    f = [F[0, 0], F[1, 0], F[0, 1], F[1, 1]]
+    c = [0.0, 4.0, 4.0, 0.0]
+    both = _m(oracle.kirchhoff(m, _particle(f, c=c)))
+    el = _m(oracle.kirchhoff(m, _particle(f)))
+    vi = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=c)))
+    assert np.linalg.norm(both - (el + vi)) < 1e-3
+
+
+def test_viscoelastic_closed_form_and_j_clamp(oracle):
+    """viscoelastic.rs:103-123: τ = µ(FFᵀ − I) + λ ln(clamp(J, j_min, 1/j_min)) I, the J
+    clamp at j_min = 0.01 (viscoelastic.rs:58) applying only inside the logarithm."""
+    lam, mu = 400.0, 200.0
+    m = oracle.material("viscoelastic", lam, mu, 0.0)
+    f = [1.3, 0.2, -0.1, 0.8]
+    F = _m(f)
+    ref = mu * (F @ F.T - np.eye(2)) + lam * np.log(np.linalg.det(F)) * np.eye(2)
+    np.testing.assert_allclose(_m(oracle.kirchhoff(m, _particle(f))), ref, rtol=2e-6, atol=1e-3)
+    f = [0.05, 0.0, 0.0, 0.05]  # J = 0.0025 < j_min
+    ref = mu * (0.0025 - 1.0) * np.eye(2) + lam * math.log(0.01) * np.eye(2)
+    np.testing.assert_allclose(_m(oracle.kirchhoff(m, _particle(f))), ref, rtol=1e-5)
+
+
+def test_viscoelastic_update_and_timestep(oracle):
+    """viscoelastic.rs:132-139: F ← (I + dt C) F with no plastic return;
+    viscoelastic.rs:161-190: dt = min(elastic wave dt, cfl_v h² ρ/η)."""
+    m = oracle.material("viscoelastic", 400.0, 200.0, 30.0)
+    c = [0.1, 0.2, -0.3, 0.05]
+    out = oracle.update_particle(m, _particle([1.2, 0.1, 0.0, 0.9], c=c), 0.5)
+    ref = (np.eye(2) + 0.5 * _m(c)) @ _m([1.2, 0.1, 0.0, 0.9])
+    np.testing.assert_allclose(_m(out["deformation_gradient"]), ref, rtol=1e-6)
+    J = np.linalg.det(ref)
+    assert abs(out["volume"] - J) < 1e-6 and abs(out["density"] - 1.0 / J) < 1e-5
+    # utils.rs:145-164 elastic wave dt = cfl·dx / sqrt((λ+2µ)/ρ); viscous = cfl_v·dx²·ρ/η
+    for rho in (2.0, 2000.0):
+        el = 0.5 / math.sqrt(800.0 / rho)
+        assert oracle.timestep_bound(m, rho, 1.0) == pytest.approx(min(el, 0.5 * rho / 30.0), rel=1e-6)
+    assert oracle.timestep_bound(oracle.material("viscoelastic", 400.0, 200.0, 0.0), 2.0, 1.0) == \
+        pytest.approx(0.5 / math.sqrt(400.0), rel=1e-6)
+
+
 # ------------------------------------------------------------------- NACC
 def _pressure(kappa, j):  # nacc.rs:326-329
     return -(kappa * 0.5 * (j - 1.0 / j)) * j
