@@ -105,12 +105,14 @@ pub struct EmhMaterialExt {
     pub nacc_kappa: f32,
     pub nacc_hardening_enabled: u32,
     pub nacc_advance_f: u32,
-    pub reserved: [u32; 2],
+    pub bingham_critical_shear_rate: f32,
+    pub fluid_bingham: u32,
 }
 impl Default for EmhMaterialExt {
     fn default() -> Self {
         Self { min_density: 1e-6, min_volume: 1e-6, dp_volume_correction: 1.0, nacc_kappa: 0.0,
-               nacc_hardening_enabled: 2, nacc_advance_f: 0, reserved: [0; 2] }
+               nacc_hardening_enabled: 2, nacc_advance_f: 0,
+               bingham_critical_shear_rate: 1e-4, fluid_bingham: 0 }
     }
 }
 

@@ -59,6 +59,7 @@ void ext_defaults(emh_material_ext* e) {
     e->min_volume = 1.0e-6f;
     e->dp_volume_correction = 1.0f;
     e->nacc_hardening_enabled = 2u;
+    e->bingham_critical_shear_rate = 1.0e-4f;
 }
 
 // f32::to_radians: self * (PI / 180.0) evaluated in f32.
@@ -110,8 +111,11 @@ int check_supported_material(const emh_material_params& p) {
         case EMH_MODEL_SNOW:
         case EMH_MODEL_DRUCKER_PRAGER:
         case EMH_MODEL_VON_MISES:
+        case EMH_MODEL_RANKINE:
+        case EMH_MODEL_MU_I:
         case EMH_MODEL_VISCOELASTIC:
         case EMH_MODEL_NACC:
+        case EMH_MODEL_GRANULAR_FLUID:
             break;
         default:
             return fail(EMH_ERR_UNSUPPORTED, "constitutive model %u is not implemented on the GPU path", p.model);
@@ -155,6 +159,36 @@ DevMat make_dev_mat(const emh_material_params& p, const emh_material_ext& e) {
             d.surf_tension = p.surface_tension_coeff;
             d.settling = p.dp_h0;
             d.fl_c2_denom = std::fmax(p.rest_density, e.min_density);
+            // BinghamFluidMaterial::params reports model Fluid with the yield
+            // stress in compression_limit (bingham.rs:205-218)
+            if (p.compression_limit > 0.0f || e.fluid_bingham != 0u) d.flags |= kMatBingham;
+            d.bing_yield = p.compression_limit;
+            d.bing_crit = e.bingham_critical_shear_rate > 0.0f ? e.bingham_critical_shear_rate : 1.0e-4f;
+            break;
+        case EMH_MODEL_GRANULAR_FLUID:  // granular_fluid.rs:232-247
+            d.rest_density = p.rest_density;
+            d.eos_k = p.eos_stiffness;
+            d.eos_p = p.eos_power;
+            d.p_floor = p.pressure_floor;
+            d.gf_rest_floor = std::fmax(p.rest_density, 1.0e-6f);
+            d.sn_xi = p.hardening_exponent;
+            d.sn_lo = 1.0f - p.compression_limit;
+            d.sn_hi = 1.0f + p.stretch_limit;
+            d.sn_jmin = p.volume_ratio_min;
+            d.sn_jmax = p.volume_ratio_max;
+            if (p.compression_limit > 0.0f || p.stretch_limit > 0.0f) d.flags |= kMatGfPlastic;
+            break;
+        case EMH_MODEL_RANKINE:  // rankine.rs:206-216; utils.rs:28,43-49
+            d.rk_ts = p.hardening_exponent;
+            d.rk_sr = p.hardening_modulus;
+            d.rk_floor = p.hardening_exponent * 0.05f;
+            d.rk_sat = p.hardening_modulus <= 0.0f ? std::numeric_limits<float>::infinity()
+                                                   : -logf(0.05f) / p.hardening_modulus;
+            break;
+        case EMH_MODEL_MU_I:  // sand_mui.rs:219-229
+            d.mui_s = p.dp_h0;
+            d.mui_d = p.dp_h1;
+            d.mui_q = p.dp_h2;
             break;
         case EMH_MODEL_DRUCKER_PRAGER:
             d.dp_phi0 = p.dp_h0;
@@ -193,7 +227,7 @@ DevMat make_dev_mat(const emh_material_params& p, const emh_material_ext& e) {
             d.na_beta = p.compression_limit;
             d.na_xi = p.hardening_exponent;
             bool hard = e.nacc_hardening_enabled == 2u ? (p.hardening_exponent > 0.0f) : (e.nacc_hardening_enabled != 0u);
-            d.flags = (hard ? 1u : 0u) | (e.nacc_advance_f ? 2u : 0u);
+            d.flags = (hard ? kMatNaccHardening : 0u) | (e.nacc_advance_f ? kMatNaccAdvance : 0u);
             d.wave_lambda = kappa - p.mu;
             break;
         }
@@ -203,7 +237,8 @@ DevMat make_dev_mat(const emh_material_params& p, const emh_material_ext& e) {
     return d;
 }
 
-// init_particle: corotated.rs:57-60, sand.rs:237-246, snow.rs:104-107, nacc.rs:271-273
+// init_particle: corotated.rs:57-60, sand.rs:237-246, snow.rs:104-107, nacc.rs:271-273,
+// sand_mui.rs:146-149, granular_fluid.rs:166-169
 void init_particle(const emh_material_params& p, emh_particle& q) {
     switch (p.model) {
         case EMH_MODEL_COROTATED:
@@ -219,6 +254,13 @@ void init_particle(const emh_material_params& p, emh_particle& q) {
             break;
         case EMH_MODEL_NACC:
             q.log_volume_strain = 0.0f;
+            break;
+        case EMH_MODEL_MU_I:
+            q.friction_hardening = p.dp_h0;
+            break;
+        case EMH_MODEL_GRANULAR_FLUID:
+            q.plastic_volume_ratio = 1.0f;
+            q.hardening_scale = 1.0f;
             break;
         default:
             break;
@@ -505,6 +547,9 @@ struct emh_sim {
                 case EMH_MODEL_NEOHOOKEAN: fields |= kFQ; break;
                 case EMH_MODEL_DRUCKER_PRAGER: fields |= kFQ | kFLvs; break;
                 case EMH_MODEL_VON_MISES: fields |= kFQ; break;
+                case EMH_MODEL_RANKINE: fields |= kFQ; break;
+                case EMH_MODEL_MU_I: fields |= kFQ; break;
+                case EMH_MODEL_GRANULAR_FLUID: fields |= kFJp | kFH | kFVolDens; break;
                 case EMH_MODEL_VISCOELASTIC: break;
                 case EMH_MODEL_COROTATED: fields |= kFH; break;
                 case EMH_MODEL_SNOW: fields |= kFJp | kFH; break;
@@ -1023,6 +1068,75 @@ int emh_material_von_mises(float lambda, float mu, float yield_stress, float har
     p->mu = mu;
     p->hardening_exponent = yield_stress;
     p->hardening_modulus = hardening_modulus;
+    return EMH_OK;
+}
+
+int emh_material_rankine(float lambda, float mu, float tensile_strength, float softening_rate,
+                         emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_RANKINE;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->hardening_exponent = tensile_strength;
+    p->hardening_modulus = softening_rate;
+    return EMH_OK;
+}
+
+int emh_material_mu_i(float lambda, float mu, float mu_static, float mu_dynamic, float inertial_q,
+                      emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_MU_I;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->dp_h0 = mu_static;
+    p->dp_h1 = mu_dynamic;
+    p->dp_h2 = inertial_q;
+    return EMH_OK;
+}
+
+int emh_material_mu_i_default(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
+    return emh_material_mu_i(lambda, mu, tanf(to_radians(20.9f)), tanf(to_radians(32.8f)), 5.58f, p, e);
+}
+
+int emh_material_granular_fluid(float lambda, float mu, float rest_density, float eos_stiffness,
+                                float hardening_exponent, float compression_limit, emh_material_params* p,
+                                emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_GRANULAR_FLUID;
+    p->lambda = lambda;
+    p->mu = mu;
+    p->rest_density = rest_density;
+    p->eos_stiffness = eos_stiffness;
+    p->eos_power = 7.0f;
+    p->hardening_exponent = hardening_exponent;
+    p->compression_limit = compression_limit;
+    p->stretch_limit = 0.01f;
+    p->volume_ratio_min = 0.2f;
+    p->volume_ratio_max = 3.0f;
+    p->pressure_floor = 0.0f;
+    return EMH_OK;
+}
+
+int emh_material_bingham(float rest_density, float dynamic_viscosity, float eos_stiffness, float eos_power,
+                         float yield_stress, emh_material_params* p, emh_material_ext* e) {
+    if (!p || !e) return fail(EMH_ERR_INVALID_ARG, "null output");
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_FLUID;
+    p->rest_density = rest_density;
+    p->eos_stiffness = eos_stiffness;
+    p->eos_power = eos_power;
+    p->dynamic_viscosity = dynamic_viscosity;
+    p->compression_limit = yield_stress;
+    p->volume_ratio_max = 2.0f;
+    p->pressure_floor = 0.0f;
+    e->fluid_bingham = 1u;
     return EMH_OK;
 }
 

@@ -77,7 +77,7 @@ constexpr uint32_t kFAll = 31u;
 // f32 arithmetic / libm the reference uses.
 struct DevMat {
     uint32_t model;
-    uint32_t flags;  // bit0 NACC hardening enabled, bit1 NACC advance F
+    uint32_t flags;  // kMatFlag* bits
     float lambda, mu;
     float min_density, min_volume;
     // NeoHookean
@@ -96,14 +96,26 @@ struct DevMat {
     float vm_yield, vm_hmod;
     // Viscoelastic
     float ve_visc, ve_jmin;
+    // Rankine: tensile strength, softening rate, 5 % residual strength, damage cap
+    float rk_ts, rk_sr, rk_floor, rk_sat;
+    // DruckerPragerMuI: mu_static, mu_dynamic, inertial Q
+    float mui_s, mui_d, mui_q;
+    // Bingham (a Fluid record): yield stress, critical shear rate
+    float bing_yield, bing_crit;
+    // GranularFluid: rest_density.max(1e-6) (EOS on the Fluid fields, clamp on the Snow fields)
+    float gf_rest_floor;
     // CFL modulus helpers
     float wave_lambda;  // lambda used by elastic_wave_dt (NACC: kappa - mu)
     float pad[3];
 };
 
 constexpr uint32_t kModelFallback = 0, kModelFluid = 1, kModelNH = 2, kModelCorot = 3, kModelSnow = 4,
-                   kModelDP = 5, kModelVonMises = 6, kModelViscoelastic = 9, kModelNacc = 10,
-                   kModelMixed = 0xffffffffu;
+                   kModelDP = 5, kModelVonMises = 6, kModelRankine = 7, kModelMuI = 8, kModelViscoelastic = 9,
+                   kModelNacc = 10, kModelGranular = 11, kModelMixed = 0xffffffffu;
+constexpr uint32_t kMatNaccHardening = 1u;  // NACC: hardening enabled
+constexpr uint32_t kMatNaccAdvance = 2u;    // NACC: project (I + dt C) F
+constexpr uint32_t kMatBingham = 4u;        // Fluid: Bingham viscoplastic deviatoric law
+constexpr uint32_t kMatGfPlastic = 8u;      // GranularFluid: compression or stretch limit > 0
 
 // ------------------------------------------------------------ small algebra
 struct M2 {
@@ -308,12 +320,14 @@ EMH_DEV bool project_state(PState& s, const DevConfig& cfg) {
 // the material's model id at run time; a model id compiles that model alone
 // (used when every particle resolves to materials of one model).
 
-// stress_volume: materials/mod.rs:118-120, fluid.rs:193-195 (von_mises.rs:104-106 and
-// viscoelastic.rs:141-143 restate the default)
+// stress_volume: materials/mod.rs:118-120, fluid.rs:193-195, bingham.rs:180-182,
+// granular_fluid.rs:196-198 (the other models restate the default)
 template <int ONLY = -1>
 EMH_DEV float stress_volume(const DevMat& m, const PState& s) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
-    return model == kModelFluid ? rmax(s.vol, m.min_volume) : s.v0;
+    if (model == kModelFluid) return rmax(s.vol, m.min_volume);
+    if (model == kModelGranular) return rmax(s.vol, 1.0e-6f);  // granular_fluid.rs:196-198
+    return s.v0;
 }
 
 EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
@@ -324,7 +338,8 @@ EMH_DEV M2 corot_tau(const M2& f, float j, float mu, float lambda) {
 }
 
 // Kirchhoff stress: elastic.rs:180-255, corotated.rs:62-78, sand.rs:220-231,
-// von_mises.rs:94-102, viscoelastic.rs:103-130, snow.rs:109-136, fluid.rs:135-191,
+// von_mises.rs:94-102, rankine.rs:157-166, sand_mui.rs:132-140, viscoelastic.rs:103-130,
+// snow.rs:109-136, fluid.rs:135-191, bingham.rs:96-178, granular_fluid.rs:171-194,
 // nacc.rs:236-249, Fallback mod.rs:113-115.
 template <int ONLY = -1>
 EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
@@ -362,10 +377,28 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
             return corot_tau(s.f, j, m.mu * s.h * 1.0f, m.lambda * s.h * 1.0f);
         }
         case kModelDP:
-        case kModelVonMises: {
+        case kModelVonMises:
+        case kModelRankine:
+        case kModelMuI: {
             float j = mdet(s.f);
             if (j <= kMinJ) return zero;
             return corot_tau(s.f, j, m.mu, m.lambda);
+        }
+        case kModelGranular: {
+            // Tait pressure on rho0 / J, floored; deviatoric part of the
+            // corotated term scaled by h, plus its lambda h (J - 1) J I
+            float j = rmax(mdet(s.f), kMinJ);
+            float dens = rmax(m.rest_density / j, 1.0e-6f);
+            float ratio = rmax(dens / m.gf_rest_floor, 1.0e-6f);
+            float p = rmax(m.eos_k * (powf(ratio, m.eos_p) - 1.0f), m.p_floor);
+            float h = s.h;
+            M2 r = polar_r(s.f);
+            float mu_eff = m.mu * h;
+            M2 coro = mmul(mscale(msub(s.f, r), 2.0f * mu_eff), mtr(s.f));
+            float ht = (coro.a + coro.d) * 0.5f;
+            M2 t{-p + (coro.a - ht), 0.0f + (coro.b - 0.0f), 0.0f + (coro.c - 0.0f), -p + (coro.d - ht)};
+            float lv = ((m.lambda * h) * (j - 1.0f)) * j;
+            return madd(t, M2{lv, lv * 0.0f, lv * 0.0f, lv});
         }
         case kModelViscoelastic: {
             // neo-Hookean spring mu (F F^T - I) + lambda ln(J) I with J clamped to
@@ -395,6 +428,26 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
             float dens = rmin(rmax(s.dens, m.min_density), m.rest_density * 2.0f);
             float p = rmax(m.eos_k * (powf(dens / m.rest_density, m.eos_p) - 1.0f), m.p_floor);
             M2 t{-p, 0.0f, 0.0f, -p};
+            if (m.flags & kMatBingham) {
+                // apparent viscosity yield / rate + eta on D_dev above the
+                // critical rate of sqrt(2 D_dev : D_dev), rigid plug below
+                M2 d = mscale(madd(s.c, mtr(s.c)), 0.5f);
+                float ht = (d.a + d.d) * 0.5f;
+                M2 dd = M2{d.a - ht, d.b - 0.0f, d.c - 0.0f, d.d - ht};
+                float dsq = (dd.a * dd.a + dd.d * dd.d) + 2.0f * dd.b * dd.b;
+                float rate = sqrtf(2.0f * dsq);
+                M2 dev = zero;
+                if (!(rate < m.bing_crit)) dev = mscale(dd, m.bing_yield / rate + m.dyn_visc);
+                t = madd(t, dev);
+                if (m.surf_tension != 0.0f) {
+                    float j = s.f.a * s.f.d - s.f.b * s.f.c;
+                    float st = m.surf_tension * j;
+                    t = madd(t, M2{st, 0.0f, 0.0f, st});
+                } else {
+                    t = madd(t, zero);
+                }
+                return t;
+            }
             M2 sym = madd(s.c, mtr(s.c));
             float div = sym.a + sym.d;
             float hd = div * 0.5f;
@@ -426,8 +479,9 @@ EMH_DEV M2 kirchhoff(const DevMat& m, const PState& s) {
 }
 
 // timestep_bound: elastic.rs:290-327, corotated.rs:108-125, fluid.rs:228-259,
-// sand.rs:345-362, von_mises.rs:154-178, viscoelastic.rs:161-190, snow.rs:193-211,
-// nacc.rs:279-298 (Fallback: +inf).
+// sand.rs:345-362, von_mises.rs:154-178, rankine.rs:218-229, sand_mui.rs:232-249,
+// viscoelastic.rs:161-190, granular_fluid.rs:250-268, snow.rs:193-211, bingham.rs:220-244
+// (the Fluid form), nacc.rs:279-298 (Fallback: +inf).
 template <int ONLY = -1>
 EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevConfig& cfg) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
@@ -447,7 +501,11 @@ EMH_DEV float timestep_bound(const DevMat& m, float density, float h, const DevC
             return elastic_wave_dt(m.lambda, m.mu, h, density, kMinJ, cfg.cell, cfg.material_cfl);
         case kModelDP:
         case kModelVonMises:
+        case kModelRankine:
+        case kModelMuI:
             return elastic_wave_dt(m.lambda, m.mu, 1.0f, density, kMinJ, cfg.cell, cfg.material_cfl);
+        case kModelGranular:
+            return elastic_wave_dt(m.lambda, m.mu, h, density, kMinJ, cfg.cell, cfg.material_cfl);
         case kModelViscoelastic: {
             float e = elastic_wave_dt(m.lambda, m.mu, 1.0f, density, 1.0e-6f, cfg.cell, cfg.material_cfl);
             float v = kInf;
@@ -509,7 +567,7 @@ EMH_DEV bool dp_project(const DevMat& m, float s1, float s2, float lvs, float q,
 // NaccMaterial::project  nacc.rs:129-223
 EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, float& ao) {
     const float xi = m.na_xi, beta = m.na_beta, mm = m.na_m, kappa = m.na_kappa;
-    const bool hard = (m.flags & 1u) != 0u;
+    const bool hard = (m.flags & kMatNaccHardening) != 0u;
     M2 u, vt;
     float2 sg;
     svd2(f, u, sg, vt);
@@ -594,8 +652,9 @@ EMH_DEV void sync_volume(PState& s) {
 }
 
 // update_particle: elastic.rs:262-269, corotated.rs:84-91, viscoelastic.rs:132-139,
-// sand.rs:248-319, von_mises.rs:108-141, snow.rs:142-176, fluid.rs:197-207,
-// nacc.rs:255-269; Fallback: no-op.
+// sand.rs:248-319, von_mises.rs:108-141, rankine.rs:168-203, sand_mui.rs:151-217,
+// granular_fluid.rs:200-230, snow.rs:142-176, fluid.rs:197-207 (bingham.rs:184-193 is
+// the same), nacc.rs:255-269; Fallback: no-op.
 template <int ONLY = -1>
 EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
     const uint32_t model = ONLY >= 0 ? (uint32_t)ONLY : m.model;
@@ -667,6 +726,92 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
             sync_volume(s);
             break;
         }
+        case kModelRankine: {
+            // principal Kirchhoff stresses of the Hencky strain; components
+            // above the softened cutoff return to it, the strain distance
+            // moved accumulates as damage (capped where the cutoff floors)
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            M2 u, vt;
+            float2 sg;
+            svd2(ft, u, sg, vt);
+            float e1 = logf(rmax(fabsf(sg.x), kLogClamp)), e2 = logf(rmax(fabsf(sg.y), kLogClamp));
+            float a = 2.0f * m.mu + m.lambda;
+            float t1 = a * e1 + m.lambda * e2, t2 = m.lambda * e1 + a * e2;
+            float damage = s.q;
+            float teff = rmax(m.rk_ts * expf(-m.rk_sr * damage), m.rk_floor);
+            bool y1 = t1 > teff, y2 = t2 > teff;
+            float n1 = sg.x, n2 = sg.y;
+            if (y1 || y2) {
+                float p1 = y1 ? teff : t1, p2 = y2 ? teff : t2;
+                float den = 4.0f * m.mu * (m.mu + m.lambda);
+                float ep1 = (a * p1 - m.lambda * p2) / den, ep2 = (a * p2 - m.lambda * p1) / den;
+                float et1 = (a * t1 - m.lambda * t2) / den, et2 = (a * t2 - m.lambda * t1) / den;
+                s.q = rmin(damage + len2(et1 - ep1, et2 - ep2), m.rk_sat);
+                n1 = expf(ep1);
+                n2 = expf(ep2);
+            }
+            s.f = usv(u, n1, n2, vt);
+            sync_volume(s);
+            break;
+        }
+        case kModelMuI: {
+            // rate-dependent Drucker-Prager: tension resets to sigma = 1;
+            // beyond mu_static p the plastic rate solves the quadratic in
+            // closed form and mu(I) is stored in friction_hardening
+            const float kSqrt2 = 1.41421356237309504880f;
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            M2 u, vt;
+            float2 sg;
+            svd2(ft, u, sg, vt);
+            float e1 = logf(rmax(fabsf(sg.x), kLogClamp)), e2 = logf(rmax(fabsf(sg.y), kLogClamp));
+            float tr = e1 + e2;
+            float ptr = -(m.lambda + m.mu) * tr;
+            float n1 = sg.x, n2 = sg.y;
+            float mu_out = m.mui_s;
+            if (ptr <= 0.0f) {
+                n1 = 1.0f;
+                n2 = 1.0f;
+            } else {
+                float ht = tr * 0.5f;
+                float d1 = e1 - ht, d2 = e2 - ht;
+                float dn = len2(d1, d2);
+                float qtr = kSqrt2 * m.mu * dn;
+                float qy = m.mui_s * ptr;
+                if (!(qtr <= qy || dn < kF32Eps)) {
+                    float dq = qtr - qy;
+                    float sp = sqrtf(ptr);
+                    float a = m.mu * dt;
+                    float b = ptr * (m.mui_d - m.mui_s) + a * m.mui_q * sp - dq;
+                    float c = -dq * m.mui_q * sp;
+                    float g = rmax((-b + sqrtf(b * b - 4.0f * a * c)) / (2.0f * a), 0.0f);
+                    if (g > kF32Eps) mu_out = m.mui_s + (m.mui_d - m.mui_s) / (m.mui_q * sp / g + 1.0f);
+                    float sh = g * dt / kSqrt2;
+                    n1 = expf(e1 - (d1 / dn) * sh);
+                    n2 = expf(e2 - (d2 / dn) * sh);
+                }
+            }
+            s.f = usv(u, n1, n2, vt);
+            s.q = mu_out;
+            sync_volume(s);
+            break;
+        }
+        case kModelGranular: {
+            M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
+            if (m.flags & kMatGfPlastic) {
+                M2 u, vt;
+                float2 sg;
+                svd2(ft, u, sg, vt);
+                float c1 = rclamp(sg.x, m.sn_lo, m.sn_hi), c2 = rclamp(sg.y, m.sn_lo, m.sn_hi);
+                float jpn = s.jp * (sg.x * sg.y) / rmax(c1 * c2, 1.0e-10f);
+                s.jp = rclamp(jpn, m.sn_jmin, m.sn_jmax);
+                s.h = rclamp(expf(m.sn_xi * (1.0f - s.jp)), 0.1f, 7.0f);
+                s.f = usv(u, c1, c2, vt);
+            } else {
+                s.f = ft;
+            }
+            sync_volume(s);
+            break;
+        }
         case kModelSnow: {
             M2 ft = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), s.f);
             M2 u, vt;
@@ -693,7 +838,7 @@ EMH_DEV void update_particle(const DevMat& m, PState& s, float dt) {
         }
         case kModelNacc: {
             M2 fin = s.f;
-            if (m.flags & 2u) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
+            if (m.flags & kMatNaccAdvance) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
             M2 nf;
             float na;
             nacc_project(m, fin, s.lvs, nf, na);

@@ -83,7 +83,7 @@ class CMaterialExt(ctypes.Structure):
     _fields_ = [("min_density", ctypes.c_float), ("min_volume", ctypes.c_float),
                 ("dp_volume_correction", ctypes.c_float), ("nacc_kappa", ctypes.c_float),
                 ("nacc_hardening_enabled", ctypes.c_uint32), ("nacc_advance_f", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("bingham_critical_shear_rate", ctypes.c_float), ("fluid_bingham", ctypes.c_uint32)]
 
 
 class CSpawnRegion(ctypes.Structure):
@@ -197,6 +197,11 @@ def load_library() -> ctypes.CDLL:
         "emh_material_snow": (ctypes.c_int, [ctypes.c_float] * 7 + [P(CMaterialParams), P(CMaterialExt)]),
         "emh_material_von_mises": (ctypes.c_int, [ctypes.c_float] * 4 + [P(CMaterialParams), P(CMaterialExt)]),
         "emh_material_viscoelastic": (ctypes.c_int, [ctypes.c_float] * 3 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_rankine": (ctypes.c_int, [ctypes.c_float] * 4 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_mu_i": (ctypes.c_int, [ctypes.c_float] * 5 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_mu_i_default": (ctypes.c_int, [ctypes.c_float] * 2 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_granular_fluid": (ctypes.c_int, [ctypes.c_float] * 6 + [P(CMaterialParams), P(CMaterialExt)]),
+        "emh_material_bingham": (ctypes.c_int, [ctypes.c_float] * 5 + [P(CMaterialParams), P(CMaterialExt)]),
         "emh_material_snow_from_young": (ctypes.c_int, [ctypes.c_float] * 2 + [P(CMaterialParams),
                                                                                P(CMaterialExt)]),
         "emh_material_nacc": (ctypes.c_int, [ctypes.c_float] * 5 + [P(CMaterialParams), P(CMaterialExt)]),
@@ -535,6 +540,71 @@ class ViscoelasticMaterial(Material):
     @classmethod
     def near_incompressible(cls, e: float, viscosity: float) -> "ViscoelasticMaterial":
         return cls.from_young_modulus(e, 0.45, viscosity)
+
+
+class RankineMaterial(Material):
+    """RankineMaterial (matter/materials/rankine.rs): corotated elasticity with a
+    softening tensile cutoff on the principal Kirchhoff stresses."""
+
+    @classmethod
+    def new(cls, lam: float, mu: float, tensile_strength: float, softening_rate: float) -> "RankineMaterial":
+        return cls(*_build("emh_material_rankine", lam, mu, tensile_strength, softening_rate))
+
+    @classmethod
+    def from_young_modulus(cls, e: float, nu: float, tensile_strength: float, softening_rate: float):
+        return cls.new(*lame_from_young(e, nu), tensile_strength, softening_rate)
+
+    @classmethod
+    def stiff_brittle(cls, e: float, nu: float) -> "RankineMaterial":
+        """rankine.rs:78-86: tensile strength 2.5e-4 E, softening 2."""
+        return cls.from_young_modulus(e, nu, e * np.float32(2.5e-4), 2.0)
+
+
+class MuIRheologyMaterial(Material):
+    """MuIRheologyMaterial (matter/materials/sand_mui.rs): rate-dependent
+    Drucker-Prager, mu(I) stored in friction_hardening."""
+
+    @classmethod
+    def new(cls, lam: float, mu: float) -> "MuIRheologyMaterial":
+        return cls(*_build("emh_material_mu_i_default", lam, mu))
+
+    @classmethod
+    def with_params(cls, lam, mu, mu_static, mu_dynamic, inertial_q) -> "MuIRheologyMaterial":
+        return cls(*_build("emh_material_mu_i", lam, mu, mu_static, mu_dynamic, inertial_q))
+
+    @classmethod
+    def from_young_modulus(cls, e: float, nu: float) -> "MuIRheologyMaterial":
+        return cls.new(*lame_from_young(e, nu))
+
+
+class GranularFluidMaterial(Material):
+    """GranularFluidMaterial (matter/materials/granular_fluid.rs): Tait EOS on
+    rho0 / J plus corotated deviatoric stress, Stomakhin-style clamp."""
+
+    @classmethod
+    def new(cls, lam, mu, rest_density, eos_stiffness, hardening_exponent, compression_limit):
+        return cls(*_build("emh_material_granular_fluid", lam, mu, rest_density, eos_stiffness,
+                           hardening_exponent, compression_limit))
+
+    @classmethod
+    def saturated_loam(cls, e: float, nu: float) -> "GranularFluidMaterial":
+        """granular_fluid.rs:101-117."""
+        return cls.new(*lame_from_young(e, nu), 1.0, 200.0, 5.0, 0.4)
+
+
+class BinghamFluidMaterial(Material):
+    """BinghamFluidMaterial (matter/materials/bingham.rs): Tait pressure plus
+    a viscoplastic deviatoric law; uploaded as a Fluid record."""
+
+    @classmethod
+    def new(cls, rest_density, dynamic_viscosity, eos_stiffness, eos_power, yield_stress):
+        return cls(*_build("emh_material_bingham", rest_density, dynamic_viscosity, eos_stiffness,
+                           eos_power, yield_stress))
+
+    @classmethod
+    def high_yield(cls, rest_density: float, eos_stiffness: float) -> "BinghamFluidMaterial":
+        """bingham.rs:74-77: yield 100, viscosity 0.5."""
+        return cls.new(rest_density, 0.5, eos_stiffness, 7.0, 100.0)
 
 
 class StomakhinMaterial(Material):

@@ -51,8 +51,11 @@ extern "C" {
 #define EMH_MODEL_SNOW            4u
 #define EMH_MODEL_DRUCKER_PRAGER  5u
 #define EMH_MODEL_VON_MISES       6u
+#define EMH_MODEL_RANKINE         7u
+#define EMH_MODEL_MU_I            8u  /* DruckerPragerMuI */
 #define EMH_MODEL_VISCOELASTIC    9u
 #define EMH_MODEL_NACC           10u
+#define EMH_MODEL_GRANULAR_FLUID 11u
 
 #define EMH_MAX_MATERIALS  64u   /* MAX_MATERIAL_SLOTS, materials/registry.rs:6 */
 #define EMH_MAX_BOUNDARIES  8u
@@ -95,6 +98,9 @@ typedef struct emh_config {
  *   Fluid       fluid.rs:209-226     DruckerPrager sand.rs:321-343
  *   Snow        snow.rs:178-191      VonMises von_mises.rs:143-152
  *   Viscoelastic viscoelastic.rs:149-159 (active_stress_coeff must be 0)
+ *   Rankine     rankine.rs:206-216   DruckerPragerMuI sand_mui.rs:219-229
+ *   GranularFluid granular_fluid.rs:232-247
+ *   Bingham     bingham.rs:205-218 (model Fluid, yield stress in compression_limit)
  *   Nacc        nacc.rs:300-314 uploads model=NeoHookean; this library needs
  *               model=EMH_MODEL_NACC (10) instead, with the same field use
  *               (lambda = kappa - mu, hardening_exponent = xi,
@@ -136,7 +142,10 @@ typedef struct emh_material_ext {
     uint32_t nacc_hardening_enabled; /* nacc.rs:70 [hardening_factor > 0]; 2 = derive */
     uint32_t nacc_advance_f;         /* 0: F <- project(F_old) as CPU Simulation does (nacc.rs:255-269);
                                         1: F <- project((I + dt C) F_old) as the reference GPU path does */
-    uint32_t reserved[2];
+    float    bingham_critical_shear_rate; /* Bingham bingham.rs:67 [1e-4]; 0 = the default */
+    uint32_t fluid_bingham;          /* a Fluid record is a BinghamFluidMaterial even with yield stress 0
+                                        (a positive compression_limit selects Bingham by itself,
+                                        as BinghamFluidMaterial::params encodes it, bingham.rs:205-218) */
 } emh_material_ext;                  /* 32 bytes */
 
 /* ---- Particle mirror (128 B): src/matter/particle.rs:11-144 ------------- */
@@ -287,6 +296,23 @@ int emh_material_snow_from_young(float young_modulus, float poisson_ratio,
  * reference asserts). */
 int emh_material_von_mises(float lambda, float mu, float yield_stress, float hardening_modulus,
                            emh_material_params* p, emh_material_ext* e);
+/* RankineMaterial::new(lambda, mu, tensile_strength, softening_rate)  rankine.rs:53-60 */
+int emh_material_rankine(float lambda, float mu, float tensile_strength, float softening_rate,
+                         emh_material_params* p, emh_material_ext* e);
+/* MuIRheologyMaterial with explicit (mu_static, mu_dynamic, Q): the fields of sand_mui.rs:44-58 */
+int emh_material_mu_i(float lambda, float mu, float mu_static, float mu_dynamic, float inertial_q,
+                      emh_material_params* p, emh_material_ext* e);
+/* MuIRheologyMaterial::new(lambda, mu)  sand_mui.rs:62-70 (tan 20.9 deg, tan 32.8 deg, Q 5.58) */
+int emh_material_mu_i_default(float lambda, float mu, emh_material_params* p, emh_material_ext* e);
+/* GranularFluidMaterial::new(lambda, mu, rho0, k, xi, theta_c)  granular_fluid.rs:61-83 */
+int emh_material_granular_fluid(float lambda, float mu, float rest_density, float eos_stiffness,
+                                float hardening_exponent, float compression_limit,
+                                emh_material_params* p, emh_material_ext* e);
+/* BinghamFluidMaterial::new(rho0, eta, k, gamma, yield_stress)  bingham.rs:53-71: a model-Fluid
+ * record (bingham.rs:205-218) with ext.fluid_bingham = 1 */
+int emh_material_bingham(float rest_density, float dynamic_viscosity, float eos_stiffness,
+                         float eos_power, float yield_stress,
+                         emh_material_params* p, emh_material_ext* e);
 /* ViscoelasticMaterial::new(lambda, mu, viscosity)  viscoelastic.rs:53-62 (j_min 0.01) */
 int emh_material_viscoelastic(float lambda, float mu, float viscosity, emh_material_params* p,
                               emh_material_ext* e);

@@ -314,6 +314,14 @@ template <class T> struct Material {
     T kappa = 0, friction_m = 0, cohesion_beta = 0, hardening_factor = 0;
     // VonMises (von_mises.rs:29-39), Viscoelastic (viscoelastic.rs:9-16)
     T yield_stress = 0, hardening_modulus = 0, ve_viscosity = 0, ve_j_min = 0;
+    // Rankine (rankine.rs:37-49)
+    T tensile_strength = 0, softening_rate = 0;
+    // DruckerPragerMuI (sand_mui.rs:44-58)
+    T mu_static = 0, mu_dynamic = 0, inertial_q = 0;
+    // Bingham (bingham.rs:26-48): a Fluid record whose params() carries the
+    // yield stress in compression_limit (bingham.rs:205-218)
+    bool bingham = false;
+    T bingham_yield = 0, critical_shear_rate = T(1.0e-4f);
     bool hardening_enabled = false;
     bool nacc_advance_f = false;
 
@@ -348,6 +356,29 @@ template <class T> struct Material {
                 m.settling_damping = T(p.dp_h0);
                 m.min_density = T(min_density);
                 m.min_volume = T(min_volume);
+                m.bingham = p.compression_limit > 0.0f || (e && e->fluid_bingham != 0u);
+                m.bingham_yield = T(p.compression_limit);
+                if (e && e->bingham_critical_shear_rate > 0.0f) m.critical_shear_rate = T(e->bingham_critical_shear_rate);
+                break;
+            case EMH_MODEL_RANKINE:  // rankine.rs:206-216
+                m.tensile_strength = T(p.hardening_exponent);
+                m.softening_rate = T(p.hardening_modulus);
+                break;
+            case EMH_MODEL_MU_I:  // sand_mui.rs:219-229
+                m.mu_static = T(p.dp_h0);
+                m.mu_dynamic = T(p.dp_h1);
+                m.inertial_q = T(p.dp_h2);
+                break;
+            case EMH_MODEL_GRANULAR_FLUID:  // granular_fluid.rs:232-247
+                m.rest_density = T(p.rest_density);
+                m.eos_stiffness = T(p.eos_stiffness);
+                m.eos_power = T(p.eos_power);
+                m.hardening_exponent = T(p.hardening_exponent);
+                m.compression_limit = T(p.compression_limit);
+                m.stretch_limit = T(p.stretch_limit);
+                m.min_plastic_jacobian = T(p.volume_ratio_min);
+                m.max_plastic_jacobian = T(p.volume_ratio_max);
+                m.pressure_floor = T(p.pressure_floor);
                 break;
             case EMH_MODEL_DRUCKER_PRAGER:
                 m.friction_angle = T(p.dp_h0);
@@ -413,6 +444,13 @@ template <class T> struct Material {
             case EMH_MODEL_NACC:
                 p.log_volume_strain = 0.0f;
                 break;
+            case EMH_MODEL_MU_I:  // sand_mui.rs:146-149
+                p.friction_hardening = (float)mu_static;
+                break;
+            case EMH_MODEL_GRANULAR_FLUID:  // granular_fluid.rs:166-169
+                p.plastic_volume_ratio = 1.0f;
+                p.hardening_scale = 1.0f;
+                break;
             default:
                 break;
         }
@@ -421,6 +459,7 @@ template <class T> struct Material {
     // stress_volume: materials/mod.rs:118-120 (default), fluid.rs:193-195
     T stress_volume(const Particles<T>& P, size_t i) const {
         if (model == EMH_MODEL_FLUID) return rmax(P.volume[i], min_volume);
+        if (model == EMH_MODEL_GRANULAR_FLUID) return rmax(P.volume[i], T(1.0e-6f));  // granular_fluid.rs:196-198
         return P.initial_volume[i];
     }
 
@@ -464,7 +503,9 @@ template <class T> struct Material {
                 return ((T(2) * mu_eff) * (f - r)) * f_t + ((lambda_eff * (j - T(1))) * j) * m2_identity<T>();
             }
             case EMH_MODEL_DRUCKER_PRAGER:  // sand.rs:220-231
-            case EMH_MODEL_VON_MISES: {     // von_mises.rs:96-104 (the same corotated form)
+            case EMH_MODEL_RANKINE:         // rankine.rs:157-166 (the same corotated form)
+            case EMH_MODEL_MU_I:            // sand_mui.rs:132-140
+            case EMH_MODEL_VON_MISES: {     // von_mises.rs:96-104
                 M2<T> f = P.f[i];
                 T j = det(f);
                 if (j <= T(MIN_J)) return m2_zero<T>();
@@ -502,7 +543,23 @@ template <class T> struct Material {
                 }
                 return tau;
             }
+            case EMH_MODEL_GRANULAR_FLUID: {  // granular_fluid.rs:171-194
+                M2<T> f = P.f[i];
+                T j = rmax(det(f), T(MIN_J));
+                T density = rmax(rest_density / j, T(1.0e-6f));
+                T ratio = rmax(density / rmax(rest_density, T(1.0e-6f)), T(1.0e-6f));
+                T pressure = rmax(eos_stiffness * (o_pow(ratio, eos_power) - T(1)), pressure_floor);
+                T h = P.hardening_scale[i];
+                M2<T> r = polar_decomposition_2d(f);
+                T mu_eff = mu * h;
+                M2<T> coro = ((T(2) * mu_eff) * (f - r)) * transpose(f);
+                T tr = coro.m00 + coro.m11;
+                M2<T> dev_coro = coro - m2_diag(tr * T(0.5), tr * T(0.5));
+                M2<T> lam_vol = (((lambda * h) * (j - T(1))) * j) * m2_identity<T>();
+                return (m2_diag(-pressure, -pressure) + dev_coro) + lam_vol;
+            }
             case EMH_MODEL_FLUID: {  // fluid.rs:135-191
+                if (bingham) return bingham_stress(P, i);
                 T density = rmin(rmax(P.density[i], min_density), rest_density * T(2));
                 T pressure = rmax(eos_stiffness * (o_pow(density / rest_density, eos_power) - T(1)), pressure_floor);
                 M2<T> stress = m2_diag(-pressure, -pressure);
@@ -541,6 +598,33 @@ template <class T> struct Material {
         }
     }
 
+    // BinghamFluidMaterial::deviatoric_stress + kirchhoff_stress  bingham.rs:96-126,147-178
+    M2<T> bingham_deviatoric(const M2<T>& c) const {
+        M2<T> sym = c + transpose(c);
+        M2<T> d = sym * T(0.5);
+        T trace = d.m00 + d.m11;
+        M2<T> d_dev = d - m2_diag(trace * T(0.5), trace * T(0.5));
+        T dxx = d_dev.m00, dyy = d_dev.m11, dxy = d_dev.m10;
+        T d_sq = dxx * dxx + dyy * dyy + T(2) * dxy * dxy;
+        T shear_rate = o_sqrt(T(2) * d_sq);
+        if (shear_rate < critical_shear_rate) return m2_zero<T>();
+        T eta_app = bingham_yield / shear_rate + dynamic_viscosity;
+        return d_dev * eta_app;
+    }
+    M2<T> bingham_stress(const Particles<T>& P, size_t i) const {
+        T density = rmin(rmax(P.density[i], min_density), rest_density * T(2));
+        T pressure = rmax(eos_stiffness * (o_pow(density / rest_density, eos_power) - T(1)), pressure_floor);
+        M2<T> hydrostatic = m2_diag(-pressure, -pressure);
+        M2<T> deviatoric = bingham_deviatoric(P.c[i]);
+        M2<T> surface = m2_zero<T>();
+        if (surface_tension_coeff != T(0)) {
+            M2<T> f = P.f[i];
+            T j = f.m00 * f.m11 - f.m10 * f.m01;
+            surface = m2_diag(surface_tension_coeff * j, surface_tension_coeff * j);
+        }
+        return (hydrostatic + deviatoric) + surface;
+    }
+
     T activation_scale() const {
         // elastic.rs, corotated.rs, viscoelastic.rs:145-147 report their coefficient
         if (model == EMH_MODEL_NEOHOOKEAN || model == EMH_MODEL_COROTATED || model == EMH_MODEL_VISCOELASTIC)
@@ -567,8 +651,12 @@ template <class T> struct Material {
             case EMH_MODEL_SNOW:
                 return elastic_wave_dt(lambda, mu, hardening_scale, density, T(MIN_J), cell_width, material_cfl);
             case EMH_MODEL_DRUCKER_PRAGER:
+            case EMH_MODEL_RANKINE:    // rankine.rs:218-229
+            case EMH_MODEL_MU_I:       // sand_mui.rs:232-249
             case EMH_MODEL_VON_MISES:  // von_mises.rs:161-178
                 return elastic_wave_dt(lambda, mu, T(1), density, T(MIN_J), cell_width, material_cfl);
+            case EMH_MODEL_GRANULAR_FLUID:  // granular_fluid.rs:250-268
+                return elastic_wave_dt(lambda, mu, hardening_scale, density, T(MIN_J), cell_width, material_cfl);
             case EMH_MODEL_VISCOELASTIC: {  // viscoelastic.rs:161-190
                 T elastic_dt = elastic_wave_dt(lambda, mu, T(1), density, T(1.0e-6f), cell_width, material_cfl);
                 T viscous_dt = inf;
@@ -764,6 +852,110 @@ template <class T> struct Material {
                     sigma_new = {o_exp(eps_proj.x), o_exp(eps_proj.y)};
                 }
                 P.f[i] = (u * m2_diag(sigma_new.x, sigma_new.y)) * vt;
+                T j = rmax(det(P.f[i]), T(MIN_J));
+                T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
+                P.volume[i] = v;
+                P.density[i] = P.mass[i] / v;
+                break;
+            }
+            case EMH_MODEL_RANKINE: {  // rankine.rs:168-203
+                M2<T> f_trial = (m2_identity<T>() + dt * P.c[i]) * P.f[i];
+                M2<T> u, vt;
+                V2<T> sigma;
+                svd2(f_trial, u, sigma, vt);
+                V2<T> eps{o_ln(rmax(std::fabs(sigma.x), T(LOG_CLAMP))), o_ln(rmax(std::fabs(sigma.y), T(LOG_CLAMP)))};
+                T a = T(2) * mu + lambda;
+                V2<T> tau{a * eps.x + lambda * eps.y, lambda * eps.x + a * eps.y};
+                T damage = P.friction_hardening[i];
+                // tensile_strength_eff  rankine.rs:113-117 (residual 5 %, utils.rs:28)
+                T t_eff = rmax(tensile_strength * o_exp(-softening_rate * damage), tensile_strength * T(0.05f));
+                // project_stress  rankine.rs:137-146
+                bool t1 = tau.x > t_eff, t2 = tau.y > t_eff;
+                V2<T> sigma_new = sigma;
+                if (t1 || t2) {
+                    V2<T> tp{t1 ? t_eff : tau.x, t2 ? t_eff : tau.y};
+                    // stress_to_hencky  utils.rs:76-83
+                    auto to_hencky = [&](V2<T> t) {
+                        T dd = T(4) * mu * (mu + lambda);
+                        T aa = T(2) * mu + lambda;
+                        return V2<T>{(aa * t.x - lambda * t.y) / dd, (aa * t.y - lambda * t.x) / dd};
+                    };
+                    V2<T> eps_proj = to_hencky(tp);
+                    V2<T> eps_trial = to_hencky(tau);
+                    // rankine_damage_saturation_point  utils.rs:43-49
+                    T sat = softening_rate <= T(0) ? std::numeric_limits<T>::infinity()
+                                                   : -o_ln(T(0.05f)) / softening_rate;
+                    V2<T> de{eps_trial.x - eps_proj.x, eps_trial.y - eps_proj.y};
+                    P.friction_hardening[i] = rmin(damage + length(de), sat);
+                    sigma_new = {o_exp(eps_proj.x), o_exp(eps_proj.y)};
+                }
+                P.f[i] = (u * m2_diag(sigma_new.x, sigma_new.y)) * vt;
+                T j = rmax(det(P.f[i]), T(MIN_J));
+                T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
+                P.volume[i] = v;
+                P.density[i] = P.mass[i] / v;
+                break;
+            }
+            case EMH_MODEL_MU_I: {  // sand_mui.rs:151-217
+                const T SQRT_2 = T(1.41421356237309504880);
+                M2<T> f_trial = (m2_identity<T>() + dt * P.c[i]) * P.f[i];
+                M2<T> u, vt;
+                V2<T> sigma;
+                svd2(f_trial, u, sigma, vt);
+                V2<T> eps{o_ln(rmax(std::fabs(sigma.x), T(LOG_CLAMP))), o_ln(rmax(std::fabs(sigma.y), T(LOG_CLAMP)))};
+                T tr = eps.x + eps.y;
+                T k_2d = lambda + mu;
+                T p_trial = -k_2d * tr;
+                V2<T> sigma_new = sigma;
+                T mu_out = mu_static;
+                if (p_trial <= T(0)) {
+                    sigma_new = {T(1), T(1)};
+                } else {
+                    V2<T> dev{eps.x - tr * T(0.5), eps.y - tr * T(0.5)};
+                    T dev_norm = length(dev);
+                    T q_trial = SQRT_2 * mu * dev_norm;
+                    T q_yield = mu_static * p_trial;
+                    if (!(q_trial <= q_yield || dev_norm < T(F32_EPSILON))) {
+                        T delta_q = q_trial - q_yield;
+                        T sqrt_p = o_sqrt(p_trial);
+                        T a = mu * dt;
+                        T b = p_trial * (mu_dynamic - mu_static) + a * inertial_q * sqrt_p - delta_q;
+                        T c = -delta_q * inertial_q * sqrt_p;
+                        T gamma_dot = (-b + o_sqrt(b * b - T(4) * a * c)) / (T(2) * a);
+                        gamma_dot = rmax(gamma_dot, T(0));
+                        if (gamma_dot > T(F32_EPSILON))
+                            mu_out = mu_static + (mu_dynamic - mu_static) / (inertial_q * sqrt_p / gamma_dot + T(1));
+                        T delta_gamma = gamma_dot * dt;
+                        V2<T> n_hat = dev / dev_norm;
+                        T s = delta_gamma / SQRT_2;
+                        V2<T> eps_new{eps.x - n_hat.x * s, eps.y - n_hat.y * s};
+                        sigma_new = {o_exp(eps_new.x), o_exp(eps_new.y)};
+                    }
+                }
+                P.f[i] = (u * m2_diag(sigma_new.x, sigma_new.y)) * vt;
+                P.friction_hardening[i] = mu_out;
+                T j = rmax(det(P.f[i]), T(MIN_J));
+                T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
+                P.volume[i] = v;
+                P.density[i] = P.mass[i] / v;
+                break;
+            }
+            case EMH_MODEL_GRANULAR_FLUID: {  // granular_fluid.rs:200-230
+                M2<T> f_trial = (m2_identity<T>() + dt * P.c[i]) * P.f[i];
+                if (compression_limit > T(0) || stretch_limit > T(0)) {
+                    M2<T> u, vt;
+                    V2<T> sigma;
+                    svd2(f_trial, u, sigma, vt);
+                    T lo = T(1) - compression_limit, hi = T(1) + stretch_limit;
+                    V2<T> sc{rclamp(sigma.x, lo, hi), rclamp(sigma.y, lo, hi)};
+                    T jp_new = P.plastic_volume_ratio[i] * (sigma.x * sigma.y) / rmax(sc.x * sc.y, T(1.0e-10f));
+                    P.plastic_volume_ratio[i] = rclamp(jp_new, min_plastic_jacobian, max_plastic_jacobian);
+                    P.hardening_scale[i] =
+                        rclamp(o_exp(hardening_exponent * (T(1) - P.plastic_volume_ratio[i])), T(0.1f), T(7.0f));
+                    P.f[i] = (u * m2_diag(sc.x, sc.y)) * vt;
+                } else {
+                    P.f[i] = f_trial;
+                }
                 T j = rmax(det(P.f[i]), T(MIN_J));
                 T v = rmax(P.initial_volume[i] * j, T(1.0e-6f));
                 P.volume[i] = v;

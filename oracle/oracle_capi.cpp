@@ -365,6 +365,7 @@ static void ext_defaults(emh_material_ext* e) {
     e->min_volume = 1.0e-6f;
     e->dp_volume_correction = 1.0f;
     e->nacc_hardening_enabled = 2u;
+    e->bingham_critical_shear_rate = 1.0e-4f;
 }
 
 void orc_material_neohookean(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
@@ -392,6 +393,70 @@ void orc_material_von_mises(float lambda, float mu, float yield_stress, float ha
     p->mu = mu;
     p->hardening_exponent = yield_stress;
     p->hardening_modulus = hardening_modulus;
+}
+
+void orc_material_rankine(float lambda, float mu, float tensile_strength, float softening_rate,
+                          emh_material_params* p, emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_RANKINE;  // rankine.rs:53-60, 206-216
+    p->lambda = lambda;
+    p->mu = mu;
+    p->hardening_exponent = tensile_strength;
+    p->hardening_modulus = softening_rate;
+}
+
+// f32::to_radians then f32::tan (sand_mui.rs:62-70)
+static float tan_deg(float deg) { return tanf(deg * (3.14159265358979323846f / 180.0f)); }
+
+void orc_material_mu_i(float lambda, float mu, float mu_static, float mu_dynamic, float inertial_q,
+                       emh_material_params* p, emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_MU_I;  // sand_mui.rs:62-70, 219-229
+    p->lambda = lambda;
+    p->mu = mu;
+    p->dp_h0 = mu_static;
+    p->dp_h1 = mu_dynamic;
+    p->dp_h2 = inertial_q;
+}
+
+void orc_material_mu_i_default(float lambda, float mu, emh_material_params* p, emh_material_ext* e) {
+    orc_material_mu_i(lambda, mu, tan_deg(20.9f), tan_deg(32.8f), 5.58f, p, e);
+}
+
+void orc_material_granular_fluid(float lambda, float mu, float rest_density, float eos_stiffness,
+                                 float hardening_exponent, float compression_limit, emh_material_params* p,
+                                 emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_GRANULAR_FLUID;  // granular_fluid.rs:61-83, 232-247
+    p->lambda = lambda;
+    p->mu = mu;
+    p->rest_density = rest_density;
+    p->eos_stiffness = eos_stiffness;
+    p->eos_power = 7.0f;
+    p->hardening_exponent = hardening_exponent;
+    p->compression_limit = compression_limit;
+    p->stretch_limit = 0.01f;
+    p->volume_ratio_min = 0.2f;
+    p->volume_ratio_max = 3.0f;
+    p->pressure_floor = 0.0f;
+}
+
+void orc_material_bingham(float rest_density, float dynamic_viscosity, float eos_stiffness, float eos_power,
+                          float yield_stress, emh_material_params* p, emh_material_ext* e) {
+    std::memset(p, 0, sizeof(*p));
+    ext_defaults(e);
+    p->model = EMH_MODEL_FLUID;  // bingham.rs:53-71, 205-218
+    p->rest_density = rest_density;
+    p->eos_stiffness = eos_stiffness;
+    p->eos_power = eos_power;
+    p->dynamic_viscosity = dynamic_viscosity;
+    p->compression_limit = yield_stress;
+    p->volume_ratio_max = 2.0f;
+    p->pressure_floor = 0.0f;
+    e->fluid_bingham = 1u;
 }
 
 void orc_material_viscoelastic(float lambda, float mu, float viscosity, emh_material_params* p,

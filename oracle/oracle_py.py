@@ -85,6 +85,11 @@ def load() -> ctypes.CDLL:
         "orc_material_corotated": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_von_mises": (None, [f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_viscoelastic": (None, [f, f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_rankine": (None, [f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_mu_i": (None, [f, f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_mu_i_default": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_granular_fluid": (None, [f, f, f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
+        "orc_material_bingham": (None, [f, f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_fluid": (None, [f, f, f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_drucker_prager": (None, [f, f, P(CMaterialParams), P(CMaterialExt)]),
         "orc_material_snow": (None, [f] * 7 + [P(CMaterialParams), P(CMaterialExt)]),

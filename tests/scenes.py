@@ -79,6 +79,29 @@ class Mats:
             return eh.ViscoelasticMaterial.new(lam, mu, viscosity)
         return self._o.material("viscoelastic", lam, mu, viscosity)
 
+    def rankine(self, e, nu, tensile_strength, softening_rate):  # rankine.rs:62-70
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.RankineMaterial.new(lam, mu, tensile_strength, softening_rate)
+        return self._o.material("rankine", lam, mu, tensile_strength, softening_rate)
+
+    def mu_i(self, e, nu):  # MuIRheologyMaterial::from_young_modulus  sand_mui.rs:73-76
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.MuIRheologyMaterial.new(lam, mu)
+        return self._o.material("mu_i_default", lam, mu)
+
+    def saturated_loam(self, e, nu):  # granular_fluid.rs:101-117
+        lam, mu = self.lame(e, nu)
+        if self.backend == "gpu":
+            return eh.GranularFluidMaterial.new(lam, mu, 1.0, 200.0, 5.0, 0.4)
+        return self._o.material("granular_fluid", lam, mu, 1.0, 200.0, 5.0, 0.4)
+
+    def bingham(self, rest_density, visc, eos_k, eos_p, yield_stress):  # bingham.rs:53-71
+        if self.backend == "gpu":
+            return eh.BinghamFluidMaterial.new(rest_density, visc, eos_k, eos_p, yield_stress)
+        return self._o.material("bingham", rest_density, visc, eos_k, eos_p, yield_stress)
+
     def nacc(self, e, nu, m_slope, beta, xi, advance_f=False):  # NaccMaterial::from_young_modulus
         lam, mu = self.lame(e, nu)
         kappa = float(np.float32(lam) + np.float32(mu))
@@ -217,6 +240,56 @@ def viscoelastic_drop(backend: str):
     return _add_velocity(sim, (0.0, -6.0))
 
 
+def brittle_drop(backend: str):
+    """Rankine rock (E = 1e5, nu = 0.25, tensile strength 40, softening 2) thrown
+    at the floor: the rebound puts it in tension and it cracks."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.rankine(1e5, 0.25, 40.0, 2.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(12, 8), box_center=(32.0, 8.0), rng_seed=23,
+                                position_jitter=0.2, initial_velocity_scale=1.0))
+    return _add_velocity(sim, (0.0, -8.0))
+
+
+def mui_column(backend: str):
+    """mu(I) sand column (E = 1e5, nu = 0.3, canonical fine-sand mu(I)) collapsing."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.mu_i(1e5, 0.3))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(8, 16), box_center=(20.0, 10.5), rng_seed=29,
+                                position_jitter=0.2))
+    return sim
+
+
+def mud_drop(backend: str):
+    """Granular-fluid mud (saturated loam, E = 1e5, nu = 0.2) slammed onto the floor."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.saturated_loam(1e5, 0.2))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(12, 8), box_center=(32.0, 7.0), rng_seed=31,
+                                position_jitter=0.2, initial_velocity_scale=1.0))
+    return _add_velocity(sim, (2.0, -8.0))
+
+
+def bingham_dam(backend: str):
+    """Bingham slurry (rest density 4, viscosity 0.5, Tait k = 100, yield stress 2)
+    dam break: the front flows, the plug behind it stays rigid."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.bingham(4.0, 0.5, 100.0, 7.0, 2.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(10, 14), box_center=(8.0, 9.5), rng_seed=37,
+                                position_jitter=0.2))
+    return sim
+
+
 def multi_material(backend: str):
     """BASELINE config 3 in miniature: NeoHookean, water, snow and corotated
     boxes with N(0, 0.5)-like random velocities, one grid."""
@@ -247,6 +320,10 @@ SCENES: Dict[str, Scene] = {
     "nacc_block": Scene("nacc_block", 10, nacc_block),
     "ductile_drop": Scene("ductile_drop", 10, ductile_drop),
     "viscoelastic_drop": Scene("viscoelastic_drop", 20, viscoelastic_drop),
+    "brittle_drop": Scene("brittle_drop", 10, brittle_drop),
+    "mui_column": Scene("mui_column", 10, mui_column),
+    "mud_drop": Scene("mud_drop", 10, mud_drop),
+    "bingham_dam": Scene("bingham_dam", 10, bingham_dam),
     "multi_material": Scene("multi_material", 10, multi_material),
 }
 

@@ -83,6 +83,14 @@ def test_material_constructors_match_oracle(oracle):
         (eh.VonMisesMaterial.with_hardening(2000.0, 3000.0, 100.0, 500.0),
          oracle.material("von_mises", 2000.0, 3000.0, 100.0, 500.0)),
         (eh.ViscoelasticMaterial.new(1000.0, 800.0, 50.0), oracle.material("viscoelastic", 1000.0, 800.0, 50.0)),
+        (eh.RankineMaterial.new(2000.0, 3000.0, 100.0, 1.0), oracle.material("rankine", 2000.0, 3000.0, 100.0, 1.0)),
+        (eh.MuIRheologyMaterial.new(2000.0, 3000.0), oracle.material("mu_i_default", 2000.0, 3000.0)),
+        (eh.MuIRheologyMaterial.with_params(2000.0, 3000.0, 0.4, 0.7, 1.12),
+         oracle.material("mu_i", 2000.0, 3000.0, 0.4, 0.7, 1.12)),
+        (eh.GranularFluidMaterial.new(27777.78, 41666.67, 1.0, 200.0, 5.0, 0.4),
+         oracle.material("granular_fluid", 27777.78, 41666.67, 1.0, 200.0, 5.0, 0.4)),
+        (eh.BinghamFluidMaterial.new(1000.0, 0.5, 5000.0, 7.0, 100.0),
+         oracle.material("bingham", 1000.0, 0.5, 5000.0, 7.0, 100.0)),
     ]
     for a, b in pairs:
         assert _bytes(a) == _bytes(b), (a, b)

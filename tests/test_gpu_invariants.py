@@ -97,10 +97,17 @@ def test_isolated_other_models_close(oracle):
         "nacc": lambda m: m.nacc(2e5, 0.3, 1.85, 0.5, 0.8, advance_f=True),
         "von_mises": lambda m: m.von_mises(5e4, 0.35, 250.0, 2e3),
         "viscoelastic": lambda m: m.viscoelastic(2e4, 0.45, 20.0),
+        "rankine": lambda m: m.rankine(1e5, 0.25, 40.0, 2.0),
+        "mu_i": lambda m: m.mu_i(1e5, 0.3),
+        "granular_fluid": lambda m: m.saturated_loam(1e5, 0.2),
+        "bingham": lambda m: m.bingham(1.0, 0.1, 100.0, 7.0, 2.0),
     }
-    # start these two strained: Von Mises beyond its yield surface (the return
-    # map runs), the Kelvin-Voigt body with shear and volume change
-    start_f = {"von_mises": (1.02, 0.0, 0.0, 0.98), "viscoelastic": (1.05, 0.01, 0.02, 0.97)}
+    # start the solids strained so that their return maps run: Von Mises and
+    # mu(I) beyond yield (mu(I) compressed), Rankine in tension, the
+    # granular fluid past its compression limit, Kelvin-Voigt sheared
+    start_f = {"von_mises": (1.02, 0.0, 0.0, 0.98), "viscoelastic": (1.05, 0.01, 0.02, 0.97),
+               "rankine": (1.01, 0.002, 0.0, 1.0), "mu_i": (0.96, 0.01, 0.0, 1.01),
+               "granular_fluid": (0.55, 0.02, 0.0, 1.0)}
     for name, fn in makers.items():
         pts["deformation_gradient"] = start_f.get(name, (1.0, 0.0, 0.0, 1.0))
         g, o = _pair(fn, cfg, pts, n_steps=2)
@@ -182,7 +189,7 @@ def test_error_paths():
     cfg = eh.SimConfig.standard(64, 0.05, (0.0, -1.0))
     sim = eh.Simulation.empty(cfg)
     bad = eh.NeoHookeanMaterial.new(1.0, 1.0)
-    bad.params.model = 7  # Rankine: not on the GPU path
+    bad.params.model = 12  # no such ConstitutiveModel (materials/mod.rs:53-66)
     with pytest.raises(eh.EmergeError) as ei:
         sim.set_material(0, bad)
     assert ei.value.code == -2

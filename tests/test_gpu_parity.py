@@ -25,6 +25,18 @@ FLOOR = 2e-6
 # amplifies f32 roundoff most in the Drucker-Prager / NACC log-strain terms).
 ONE_STEP_TOL = {"x": 1e-6, "v": 2e-3, "C": 5e-3, "F": 1e-4, "Jp": 1e-5, "q": 1e-4, "lvs": 2e-2, "density": 2e-5}
 CHAOTIC = {"quickstart_impact"}
+# Fields whose law is discontinuous, compared by counting particles instead of
+# by the largest error.  mu(I) stores mu(I) in friction_hardening as a
+# diagnostic that nothing reads back; it jumps from mu_static (p_trial <= 0,
+# sand_mui.rs:158-167) to nearly mu_dynamic as p_trial -> 0+ (sand_mui.rs:
+# 176-200), so free-surface particles at p_trial ~ 0 flip on summation order
+# alone.  The GPU may flip at most a few times as many particles as the f32
+# oracle does in either particle order, plus 2 % of the particles.
+FLIP_FIELDS = {"mui_column": ("q", 1e-3)}
+
+
+def _flips(a, ref, tol):
+    return int(np.sum(np.abs(np.asarray(a, np.float64) - ref) > tol))
 
 
 def _run(scene, backend, steps, perm_seed=None):
@@ -70,16 +82,32 @@ def test_scene_parity(name, oracle, capsys):
             for f in ("x", "v", "C", "F", "Jp", "q", "lvs", "density"):
                 print(f"  {f:8s} gpu-vs-o32 {e_g32[f]:.3e}  gpu-vs-o64 {e_g64[f]:.3e}  o32-vs-o64 {e_a[f]:.3e}"
                       f"  o32(permuted)-vs-o64 {e_b[f]:.3e}")
+        flip = FLIP_FIELDS.get(name)
+        if flip is not None:
+            f, tol = flip
+            n = len(g[k][f])
+            fg, fa, fb = (_flips(x[f], o64[k][f], tol) for x in (g[k], o32[k], o32p[k]))
+            with capsys.disabled():
+                print(f"  {f} flips beyond {tol}: gpu {fg}, o32 {fa}, o32(permuted) {fb} of {n}")
+            assert fg <= 4 * max(fa, fb) + 0.02 * n, (name, s, f, fg, fa, fb)
         if k == 0:
             # absolute bound, widened where the reference algorithm itself
             # spreads f32 roundoff further (|gpu - o32| <= |gpu - o64| + |o32 - o64|)
             for f, tol in ONE_STEP_TOL.items():
+                if flip is not None and f == flip[0]:
+                    continue
                 bound = max(tol, (RATIO + 1.0) * e_3264[f] + FLOOR)
                 assert e_g32[f] <= bound, (name, s, f, e_g32[f], bound)
         if k == 0 or name not in CHAOTIC:
             for f in ONE_STEP_TOL:
+                if flip is not None and f == flip[0]:
+                    continue
                 assert e_g64[f] <= RATIO * e_3264[f] + FLOOR, (name, s, f, e_g64[f], e_3264[f])
     # invariants that hold for every scene after N steps
     p = gsim.particles()
     assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["deformation_gradient"]))
     assert np.array_equal(p["mass"], osim.particles()["mass"])
+    if name == "mui_column":  # mu(I) stays between mu_static and mu_dynamic (sand_mui.rs:190-196)
+        mp = S.Mats("gpu").mu_i(1e5, 0.3).params
+        q = p["friction_hardening"]
+        assert q.min() >= mp.dp_h0 and q.max() <= mp.dp_h1

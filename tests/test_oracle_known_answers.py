@@ -383,6 +383,235 @@ def test_viscoelastic_update_and_timestep(oracle):
         pytest.approx(0.5 / math.sqrt(400.0), rel=1e-6)
 
 
+# ----------------------------------------------------------------- Rankine
+def _rk_step(m, oracle, s1, s2, damage=0.0):  # rankine.rs:247-261
+    out = oracle.update_particle(m, _particle([s1, 0, 0, s2], q=damage), 1.0)
+    F = out["deformation_gradient"]
+    return np.array([F[0], F[3]], np.float64), float(out["friction_hardening"])
+
+
+def _rk_tau(lam, mu, sig):  # principal Kirchhoff stress of a Hencky strain (rankine.rs:174-178)
+    e = np.log(np.abs(sig))
+    a = 2 * mu + lam
+    return np.array([a * e[0] + lam * e[1], lam * e[0] + a * e[1]])
+
+
+def test_rankine_inside_cutoff_is_elastic(oracle):
+    """rankine.rs:271-287: τ1 = 0.99 σt leaves σ and the damage unchanged."""
+    m = oracle.material("rankine", 2000.0, 3000.0, 100.0, 1.0)
+    ex = 0.99 * 100.0 / 8000.0
+    sig, dmg = _rk_step(m, oracle, math.exp(ex), 1.0)
+    assert np.linalg.norm(sig - [math.exp(ex), 1.0]) < 1e-4 and dmg == 0.0
+
+
+def test_rankine_projects_to_cutoff(oracle):
+    """rankine.rs:289-334: τ1 = 1.5 σt lands on τ1 = σt with τ2 unchanged, and the
+    damage is |ε_trial − ε_proj| (rankine.rs:189-191), 0.5 σt / (2µ + λ) · ... in closed form."""
+    lam, mu = 2000.0, 3000.0
+    m = oracle.material("rankine", lam, mu, 100.0, 1.0)
+    ex = 1.5 * 100.0 / 8000.0
+    sig, dmg = _rk_step(m, oracle, math.exp(ex), 1.0)
+    tau = _rk_tau(lam, mu, sig)
+    assert abs(tau[0] - 100.0) < 1e-3 and abs(tau[1] - lam * ex) < 1e-3
+    # ε_trial − ε_proj = S (τ_trial − τ_proj) with the compliance S of the 2D law, Δτ = (50, 0)
+    det = 4 * mu * (mu + lam)
+    de = np.array([(2 * mu + lam) * 50.0 / det, -lam * 50.0 / det])
+    assert abs(dmg - np.linalg.norm(de)) < 1e-6
+
+
+def test_rankine_biaxial_corner_and_softening(oracle):
+    """rankine.rs:336-361: equal biaxial tension returns both components to the corner;
+    rankine.rs:113-117 + utils.rs:43-49: the cutoff softens as σt e^{-s d}, floored at 5 %,
+    and the damage saturates at −ln(0.05)/s."""
+    lam, mu = 2000.0, 3000.0
+    m = oracle.material("rankine", lam, mu, 100.0, 1.0)
+    e = 1.5 * 100.0 / (2 * mu + 2 * lam)
+    sig, dmg = _rk_step(m, oracle, math.exp(e), math.exp(e))
+    np.testing.assert_allclose(_rk_tau(lam, mu, sig), [100.0, 100.0], atol=1e-3)
+    assert dmg > 0.0
+    # with damage 0.5 the cutoff is 100 e^{-0.5}
+    ex = 1.5 * 100.0 / 8000.0
+    sig, _ = _rk_step(m, oracle, math.exp(ex), 1.0, damage=0.5)
+    assert abs(_rk_tau(lam, mu, sig)[0] - 100.0 * math.exp(-0.5)) < 1e-3
+    # past saturation the cutoff is the 5 % residual and the damage stays at the cap
+    sat = -math.log(0.05) / 1.0
+    sig, dmg = _rk_step(m, oracle, math.exp(ex), 1.0, damage=10.0)
+    assert abs(_rk_tau(lam, mu, sig)[0] - 5.0) < 1e-3
+    assert abs(dmg - sat) < 1e-6
+
+
+def test_rankine_stress_is_corotated(oracle):
+    """rankine.rs:157-166: the stress is DP's corotated form."""
+    f = [1.1, 0.3, -0.2, 0.9]
+    a = oracle.kirchhoff(oracle.material("rankine", 400.0, 200.0, 50.0, 1.0), _particle(f))
+    b = oracle.kirchhoff(oracle.material("drucker_prager", 400.0, 200.0), _particle(f))
+    assert np.array_equal(a, b)
+
+
+# -------------------------------------------------------------------- µ(I)
+def _mui(oracle, lam=2000.0, mu=3000.0):
+    return oracle.material("mu_i_default", lam, mu)
+
+
+def _mui_step(m, oracle, s1, s2, dt):  # sand_mui.rs:260-273 (init_particle: q = µ1)
+    out = oracle.update_particle(m, _particle([s1, 0, 0, s2], q=m.params.dp_h0), dt)
+    F = out["deformation_gradient"]
+    return np.array([F[0], F[3]], np.float64), out["friction_hardening"]
+
+
+def test_mui_defaults():
+    """sand_mui.rs:62-70: µ1 = tan 20.9°, µ2 = tan 32.8°, Q = 5.58."""
+    import oracle_py as o
+    m = o.material("mu_i_default", 2000.0, 3000.0)
+    assert abs(m.params.dp_h0 - math.tan(math.radians(20.9))) < 1e-6
+    assert abs(m.params.dp_h1 - math.tan(math.radians(32.8))) < 1e-6
+    assert m.params.dp_h2 == np.float32(5.58) and m.params.model == 8
+
+
+def test_mui_tension_projects_to_identity(oracle):
+    """sand_mui.rs:281-296: p_trial ≤ 0 gives σ = 1 exactly and µ = µ1."""
+    m = _mui(oracle)
+    sig, mu_after = _mui_step(m, oracle, 1.05, 1.02, 0.1)
+    assert np.linalg.norm(sig - 1.0) < 1e-5 and mu_after == m.params.dp_h0
+
+
+def test_mui_inside_static_cone_is_elastic(oracle):
+    """sand_mui.rs:298-339: q = 0.99 µ1 p stays elastic with µ = µ1."""
+    m = _mui(oracle)
+    mu1 = float(m.params.dp_h0)
+    ex = -0.01
+    p_trial = -(5000.0) * ex
+    dev_needed = 0.99 * mu1 * p_trial / (math.sqrt(2) * 3000.0)
+    ey = ex - dev_needed * math.sqrt(2)
+    sig, mu_after = _mui_step(m, oracle, math.exp(ex), math.exp(ey), 0.1)
+    assert np.linalg.norm(sig - [math.exp(ex), math.exp(ey)]) < 1e-4 and mu_after == m.params.dp_h0
+
+
+def test_mui_rate_is_a_root_of_its_quadratic(oracle):
+    """sand_mui.rs:341-405: beyond yield, µ(I) > µ1 and the γ̇ it implies solves
+    a γ̇² + b γ̇ + c = 0 (relative residual < 1e-3); the strain moves along −n̂ by γ̇ dt/√2."""
+    m = _mui(oracle)
+    dt, ex, ey = 0.1, -0.08, 0.02
+    sig, mu_i = _mui_step(m, oracle, math.exp(ex), math.exp(ey), dt)
+    mu1, mu2, Q, G = float(m.params.dp_h0), float(m.params.dp_h1), float(m.params.dp_h2), 3000.0
+    tr = ex + ey
+    p = -5000.0 * tr
+    dev = np.array([ex - tr / 2, ey - tr / 2])
+    dn = np.linalg.norm(dev)
+    dq = math.sqrt(2) * G * dn - mu1 * p
+    a, sp = G * dt, math.sqrt(p)
+    b = p * (mu2 - mu1) + a * Q * sp - dq
+    c = -dq * Q * sp
+    assert mu_i > mu1
+    g = Q * sp / ((mu2 - mu1) / (mu_i - mu1) - 1.0)
+    assert abs(a * g * g + b * g + c) / max(abs(a * g * g), abs(b * g), abs(c)) < 1e-3
+    eps_new = np.array([ex, ey]) - dev / dn * (g * dt / math.sqrt(2))
+    np.testing.assert_allclose(np.log(sig), eps_new, atol=2e-5)
+
+
+# ------------------------------------------------------------ GranularFluid
+def _loam(oracle):  # GranularFluidMaterial::saturated_loam(1e5, 0.2)  granular_fluid.rs:101-117
+    lam, mu = oracle.lame_from_young(1.0e5, 0.2)
+    m = oracle.material("granular_fluid", lam, mu, 1.0, 200.0, 5.0, 0.4)
+    return m, lam, mu
+
+
+def test_granular_fluid_stress_closed_form_and_symmetric(oracle):
+    """granular_fluid.rs:171-194 at the F of physics_correctness.rs:415-440: τ =
+    −p I + dev(2µh(F−R)Fᵀ) + λh(J−1)J I with p = max(k((ρ0/J / ρ0)^7 − 1), 0); τ is symmetric."""
+    m, lam, mu = _loam(oracle)
+    for f, h in (([1.1, 0.05, 0.1, 0.95], 1.0), ([0.9, 0.02, -0.03, 0.85], 1.6)):
+        F = _m(f)
+        J = np.linalg.det(F)
+        U, _, Vt = np.linalg.svd(F)
+        R = U @ Vt
+        p = max(200.0 * ((1.0 / J) ** 7 - 1.0), 0.0)
+        coro = 2 * mu * h * (F - R) @ F.T
+        ref = -p * np.eye(2) + coro - 0.5 * np.trace(coro) * np.eye(2) + lam * h * (J - 1) * J * np.eye(2)
+        tau = _m(oracle.kirchhoff(m, _particle(f, h=h)))
+        np.testing.assert_allclose(tau, ref, rtol=2e-5, atol=2e-2)
+        assert abs(tau[0, 1] - tau[1, 0]) / max(abs(tau[0, 1]), 1.0) < 1e-5
+        if h == 1.6:
+            assert p > 100.0  # the compressed case exercises the EOS term
+
+
+def test_granular_fluid_clamp_matches_snow_form(oracle):
+    """granular_fluid.rs:200-230: σ clamps to [1 − θc, 1 + θs], Jp = Jp σ1σ2/(σc1σc2)
+    clamped to [0.2, 3], h = clamp(exp(ξ(1 − Jp)), 0.1, 7)."""
+    m, _, _ = _loam(oracle)
+    out = oracle.update_particle(m, _particle([1.05, 0, 0, 0.8], jp=1.0), 1.0)
+    F = out["deformation_gradient"]
+    assert abs(F[0] - 1.01) < 1e-6 and abs(F[3] - 0.8) < 1e-6
+    jp = 1.05 / 1.01
+    assert abs(out["plastic_volume_ratio"] - jp) < 1e-6
+    assert abs(out["hardening_scale"] - min(max(math.exp(5.0 * (1 - jp)), 0.1), 7.0)) < 1e-5
+    assert abs(out["volume"] - 1.01 * 0.8) < 1e-6
+    # both limits 0: F is advanced with no projection
+    m.params.compression_limit = 0.0
+    m.params.stretch_limit = 0.0
+    out = oracle.update_particle(m, _particle([1.2, 0, 0, 0.7], c=[0.1, 0.0, 0.0, -0.2]), 0.5)
+    np.testing.assert_allclose(out["deformation_gradient"], [1.2 * 1.05, 0, 0, 0.7 * 0.9], rtol=1e-6)
+
+
+def test_granular_fluid_mass_conserved(oracle):
+    """physics_correctness.rs:146-159: saturated loam, 100 zero-gravity steps."""
+    m, _, _ = _loam(oracle)
+    sim = oracle.OracleSim.new(_zero_gravity(32), _center_spawn(32, 6))
+    sim.with_default_material(m)
+    m0 = sim.particles()["mass"].sum()
+    sim.step_n(100)
+    p = sim.particles()
+    assert abs(p["mass"].sum() - m0) < 1e-6
+    F = p["deformation_gradient"]
+    assert np.min(F[:, 0] * F[:, 3] - F[:, 1] * F[:, 2]) > 0
+
+
+# ------------------------------------------------------------------ Bingham
+def _bingham(oracle):  # BinghamFluidMaterial::new(1000, 0.5, 5000, 7, 100)  bingham.rs:262-263
+    return oracle.material("bingham", 1000.0, 0.5, 5000.0, 7.0, 100.0)
+
+
+def test_bingham_plug_below_critical_rate(oracle):
+    """bingham.rs:261-274: a sub-critical shear rate gives zero deviatoric stress, so
+    τ is the Tait pressure alone."""
+    m = _bingham(oracle)
+    tau = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=[0.0, 1e-6, 1e-6, 0.0], density=1000.0)))
+    assert np.array_equal(tau, np.zeros((2, 2)))
+    tau = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=[0.0, 1e-6, 1e-6, 0.0], density=1100.0)))
+    p = 5000.0 * (1.1 ** 7 - 1.0)
+    np.testing.assert_allclose(tau, -p * np.eye(2), rtol=1e-5)
+
+
+def test_bingham_formula_above_yield(oracle):
+    """bingham.rs:276-298: pure shear C = [[0, g], [g, 0]] gives γ̇ = 2g and
+    τ = (τ0/γ̇ + η) C; bingham.rs:300-318: the apparent viscosity falls with the rate."""
+    m = _bingham(oracle)
+    for g in (1.0, 5.0, 10.0):
+        C = [0.0, g, g, 0.0]
+        tau = _m(oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=C, density=1000.0)))
+        pred = (100.0 / (2 * g) + 0.5) * _m(C)
+        assert np.linalg.norm(tau - pred) < 1e-3
+    slow = oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=[0, 1.0, 1.0, 0], density=1000.0))[1] / 2.0
+    fast = oracle.kirchhoff(m, _particle([1, 0, 0, 1], c=[0, 10.0, 10.0, 0], density=1000.0))[1] / 20.0
+    assert fast < slow
+
+
+def test_bingham_record_is_a_fluid(oracle):
+    """bingham.rs:205-218: params() reports model Fluid with the yield stress in
+    compression_limit; a Newtonian record (compression_limit 0) keeps the Newtonian law,
+    whose viscous term is η (C + Cᵀ)_dev, twice the Bingham η D_dev."""
+    m = _bingham(oracle)
+    assert m.params.model == 1 and m.params.compression_limit == 100.0 and m.params.pressure_floor == 0.0
+    newt = oracle.material("fluid", 1000.0, 0.5, 5000.0, 7.0)
+    C = [0.0, 3.0, 3.0, 0.0]
+    tn = _m(oracle.kirchhoff(newt, _particle([1, 0, 0, 1], c=C, density=1000.0)))
+    np.testing.assert_allclose(tn, 0.5 * 2 * _m(C), rtol=1e-6)
+    # a zero-yield Bingham record (flag in the extension) is η D_dev above the critical rate
+    m0 = oracle.material("bingham", 1000.0, 0.5, 5000.0, 7.0, 0.0)
+    tb = _m(oracle.kirchhoff(m0, _particle([1, 0, 0, 1], c=C, density=1000.0)))
+    np.testing.assert_allclose(tb, 0.5 * _m(C), rtol=1e-6)
+
+
 # ------------------------------------------------------------------- NACC
 def _pressure(kappa, j):  # nacc.rs:326-329
     return -(kappa * 0.5 * (j - 1.0 / j)) * j
