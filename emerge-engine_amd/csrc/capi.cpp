@@ -337,7 +337,14 @@ struct emh_sim {
     emh_config cfg{};
     std::vector<emh_material_params> mats;
     std::vector<emh_material_ext> exts;
-    std::vector<uint32_t> slips;
+    struct HostBoundary {  // one installed BoundaryCondition (boundary/*.rs)
+        uint32_t kind, thickness;
+        float mu, wall_min;
+        std::vector<float> heights;
+    };
+    std::vector<HostBoundary> bnds;
+    float* d_heights = nullptr;   // terrain heights of the heightmap boundaries
+    size_t heights_cap = 0;
     std::vector<emh_particle> host;  // authoritative when !device_ahead
     uint32_t next_tag = 1;
     bool host_dirty = true;     // host changed since the last upload
@@ -397,6 +404,7 @@ struct emh_sim {
         release_events();
         release_particles();
         if (d_mats) (void)hipFree(d_mats);
+        if (d_heights) (void)hipFree(d_heights);
         if (grid_block) (void)hipFree(grid_block);
         if (d_cell_count) (void)hipFree(d_cell_count);
         if (d_cell_start) (void)hipFree(d_cell_start);
@@ -409,6 +417,8 @@ struct emh_sim {
         d_snap_cells = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         d_mats = nullptr;
+        d_heights = nullptr;
+        heights_cap = 0;
         grid_block = nullptr;
         d_cell_count = d_cell_start = d_tile_count = d_tile_start = nullptr;
         d_ctrl = nullptr;
@@ -512,12 +522,25 @@ struct emh_sim {
         d.max_substeps = cfg.max_substeps_per_step;
         d.apic_blend = cfg.apic_blend;
         d.j_max = cfg.j_max;
-        d.n_slip = (uint32_t)slips.size();
-        for (size_t i = 0; i < slips.size(); ++i) {
-            d.slip[i] = slips[i];
-            d.slip_lo[i] = (float)(slips[i] > 0 ? slips[i] - 1 : 0u);
-            d.slip_hi[i] = (float)(cfg.grid_res > slips[i] ? cfg.grid_res - slips[i] : 0u);
+        d.n_bnd = (uint32_t)bnds.size();
+        uint32_t hoff = 0;
+        for (size_t i = 0; i < bnds.size(); ++i) {
+            const HostBoundary& b = bnds[i];
+            d.bnd_kind[i] = b.kind;
+            d.bnd_t[i] = b.thickness;
+            d.bnd_mu[i] = b.mu;
+            if (b.kind == kBndPredictive) {  // predictive.rs:31-35
+                d.bnd_lo[i] = b.wall_min;
+                d.bnd_hi[i] = ((float)cfg.grid_res - 1.0f) - b.wall_min;
+            } else {  // clamp_position_inside_grid  boundary/mod.rs:103-111
+                d.bnd_lo[i] = (float)(b.thickness > 0 ? b.thickness - 1 : 0u);
+                d.bnd_hi[i] = (float)(cfg.grid_res > b.thickness ? cfg.grid_res - b.thickness : 0u);
+            }
+            d.bnd_hoff[i] = hoff;
+            d.bnd_hlen[i] = (uint32_t)b.heights.size();
+            hoff += (uint32_t)b.heights.size();
         }
+        d.heights = d_heights;
         const uint32_t bt = cfg.boundary_thickness;
         d.proj_lo = (float)(bt > 0 ? bt - 1 : 0u);
         d.proj_hi = (float)(cfg.grid_res > bt ? cfg.grid_res - bt : 0u);
@@ -561,6 +584,27 @@ struct emh_sim {
         d.only_model = only;
     }
 
+    // Terrain heights of every heightmap boundary, concatenated in boundary order.
+    int upload_heights() {
+        std::vector<float> all;
+        for (const HostBoundary& b : bnds) all.insert(all.end(), b.heights.begin(), b.heights.end());
+        if (all.size() > heights_cap) {
+            if (d_heights) (void)hipFree(d_heights);
+            d_heights = nullptr;
+            heights_cap = 0;
+            if (hipMalloc((void**)&d_heights, all.size() * sizeof(float)) != hipSuccess) {
+                d_heights = nullptr;
+                return fail(EMH_ERR_OUT_OF_MEMORY, "heightmap of %zu heights", all.size());
+            }
+            heights_cap = all.size();
+        }
+        if (!all.empty()) {
+            HIP_TRY(hipMemcpyAsync(d_heights, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
+        return EMH_OK;
+    }
+
     int upload_materials() {
         std::vector<DevMat> dm(mats.size());
         for (size_t i = 0; i < mats.size(); ++i) dm[i] = make_dev_mat(mats[i], exts[i]);
@@ -577,6 +621,8 @@ struct emh_sim {
         }
         const uint32_t n = (uint32_t)host.size();
         int rc = ensure_capacity(n);
+        if (rc) return rc;
+        rc = upload_heights();
         if (rc) return rc;
         fill_dev_config();
         rc = upload_materials();
@@ -1191,7 +1237,7 @@ int emh_sim_create(const emh_config* cfg, emh_sim** out) {  // Simulation::empty
     ext_defaults(&fe);
     s->mats.push_back(fb);  // FallbackMaterial
     s->exts.push_back(fe);
-    s->slips.push_back(cfg->boundary_thickness);
+    s->bnds.push_back(emh_sim::HostBoundary{kBndSlip, cfg->boundary_thickness, 0.0f, 0.0f, {}});
     s->stats.last_step_dt = cfg->dt;
     rc = s->init_device();
     if (rc) {
@@ -1270,22 +1316,44 @@ int emh_sim_reinit_particle_state(emh_sim* s) {
     return EMH_OK;
 }
 
-int emh_sim_add_slip_boundary(emh_sim* s, uint32_t t) {
+static int add_boundary(emh_sim* s, uint32_t kind, uint32_t t, float mu, float wall_min, const float* heights,
+                        uint32_t n_heights) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
-    if (s->slips.size() >= EMH_MAX_BOUNDARIES) return fail(EMH_ERR_UNSUPPORTED, "at most %u boundaries", EMH_MAX_BOUNDARIES);
-    if (t + 1 > s->cfg.grid_res) return fail(EMH_ERR_INVALID_ARG, "slip thickness %u too large for grid_res %u", t, s->cfg.grid_res);
+    if (s->bnds.size() >= EMH_MAX_BOUNDARIES) return fail(EMH_ERR_UNSUPPORTED, "at most %u boundaries", EMH_MAX_BOUNDARIES);
+    if (t + 1 > s->cfg.grid_res) return fail(EMH_ERR_INVALID_ARG, "boundary thickness %u too large for grid_res %u", t, s->cfg.grid_res);
     int rc = s->host_current();
     if (rc) return rc;
-    s->slips.push_back(t);
+    emh_sim::HostBoundary b{kind, t, mu, wall_min, {}};
+    if (heights) b.heights.assign(heights, heights + n_heights);
+    s->bnds.push_back(std::move(b));
     s->host_dirty = true;
     return EMH_OK;
+}
+
+int emh_sim_add_slip_boundary(emh_sim* s, uint32_t t) { return add_boundary(s, kBndSlip, t, 0.0f, 0.0f, nullptr, 0); }
+
+int emh_sim_add_friction_boundary(emh_sim* s, uint32_t t, float mu) {
+    if (!(mu >= 0.0f && mu <= 1.0f))  // friction.rs:29-32
+        return fail(EMH_ERR_INVALID_ARG, "friction_coefficient must be in [0.0, 1.0], got %g", (double)mu);
+    return add_boundary(s, kBndFriction, t, mu, 0.0f, nullptr, 0);
+}
+
+int emh_sim_add_heightmap_boundary(emh_sim* s, const float* heights, uint32_t n_heights, float friction,
+                                   uint32_t wall_thickness) {
+    if (n_heights && !heights) return fail(EMH_ERR_INVALID_ARG, "null heights");
+    if (n_heights > (1u << 24)) return fail(EMH_ERR_INVALID_ARG, "too many heights");
+    return add_boundary(s, kBndHeightmap, wall_thickness, friction, 0.0f, heights, n_heights);
+}
+
+int emh_sim_add_predictive_boundary(emh_sim* s, uint32_t t, float predictive_wall_min) {
+    return add_boundary(s, kBndPredictive, t, 0.0f, predictive_wall_min, nullptr, 0);
 }
 
 int emh_sim_clear_boundaries(emh_sim* s) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     int rc = s->host_current();
     if (rc) return rc;
-    s->slips.clear();
+    s->bnds.clear();
     s->host_dirty = true;
     return EMH_OK;
 }

@@ -49,17 +49,24 @@ struct DevConfig {
     uint32_t max_substeps;
     float apic_blend;
     float j_max;
-    uint32_t n_slip;
-    uint32_t slip[8];
+    // installed boundaries, applied in order (kBnd* kinds)
+    uint32_t n_bnd;
+    uint32_t bnd_kind[8];
+    uint32_t bnd_t[8];            // Slip / Friction / Predictive thickness, Heightmap wall thickness
+    float bnd_mu[8];              // Friction coefficient, Heightmap terrain friction
+    uint32_t bnd_hoff[8], bnd_hlen[8];  // Heightmap: its heights in `heights`
+    const float* heights;         // device: terrain heights of every heightmap boundary
     // derived
     float proj_lo, proj_hi;  // particle clamp range of project_particle_state_to_admissible
-    float slip_lo[8], slip_hi[8];
+    float bnd_lo[8], bnd_hi[8];   // each boundary's particle clamp box
     uint32_t tiles_per_dim;   // 16x16-cell tiles per grid side
     uint32_t n_bins;          // tiles_per_dim^2 * 256
     uint32_t density_pass;    // any material needs_density_recompute || config flag
     uint32_t fields;          // kF* bits: optional particle fields the particles' materials use
     uint32_t only_model;      // the model every particle's material has, or kModelMixed
 };
+
+constexpr uint32_t kBndSlip = 0, kBndFriction = 1, kBndHeightmap = 2, kBndPredictive = 3;
 
 // Optional particle fields (DevConfig::fields).  A field no registered
 // material reads or writes stays untouched in memory between reorders.

@@ -333,6 +333,75 @@ __device__ __forceinline__ uint32_t seq_mod3(uint64_t s) { return (uint32_t)(s %
 // walls and the speed clamp (grid/mod.rs:264-291, step.rs:140-165), from the
 // P2G accumulators of the current substep; (0, 0) for untouched cells
 // (velocity_at of a missing cell, grid/mod.rs:215-233).
+// Coulomb wall  boundary/mod.rs:57-75 (outward normal (nx, ny))
+__device__ __forceinline__ void coulomb_wall(float& vx, float& vy, float nx, float ny, float mu) {
+    const float vn = vx * nx + vy * ny;
+    if (vn >= 0.0f) return;
+    const float ns = fabsf(vn);
+    const float tx = vx - nx * vn, ty = vy - ny * vn;
+    const float tl = len2(tx, ty);
+    const float fi = mu * ns;
+    if (tl > fi) {
+        const float k = (tl - fi) / tl;
+        vx = tx * k;
+        vy = ty * k;
+    } else {
+        vx = 0.0f;
+        vy = 0.0f;
+    }
+}
+__device__ __forceinline__ float terrain_height(const DevConfig& cfg, uint32_t b, uint32_t x) {
+    const uint32_t n = cfg.bnd_hlen[b];
+    return n == 0u ? 0.0f : cfg.heights[cfg.bnd_hoff[b] + (x < n - 1u ? x : n - 1u)];
+}
+// BoundaryCondition::apply_to_grid_velocity of boundary b on cell (x, y):
+// slip.rs:16-18 (mod.rs:78-101), friction.rs:40-59, heightmap.rs:66-98, predictive.rs:27-29
+__device__ __forceinline__ void boundary_velocity(const DevConfig& cfg, uint32_t b, uint32_t x, uint32_t y,
+                                                  float& vx, float& vy) {
+    const uint32_t R = cfg.grid_res, t = cfg.bnd_t[b], kind = cfg.bnd_kind[b];
+    const uint32_t hi = R > t + 1u ? R - (t + 1u) : 0u;
+    if (kind == kBndSlip || kind == kBndPredictive) {
+        if (x < t) vx = rmax(vx, 0.0f);
+        if (x > hi) vx = rmin(vx, 0.0f);
+        if (y < t) vy = rmax(vy, 0.0f);
+        if (y > hi) vy = rmin(vy, 0.0f);
+    } else if (kind == kBndFriction) {
+        const float mu = cfg.bnd_mu[b];
+        if (x < t) coulomb_wall(vx, vy, 1.0f, 0.0f, mu);
+        if (x > hi) coulomb_wall(vx, vy, -1.0f, 0.0f, mu);
+        if (y < t) coulomb_wall(vx, vy, 0.0f, 1.0f, mu);
+        if (y > hi) coulomb_wall(vx, vy, 0.0f, -1.0f, mu);
+    } else {  // heightmap: slip side and top walls, Coulomb floor on the terrain
+        if (x < t) vx = rmax(vx, 0.0f);
+        if (x > hi) vx = rmin(vx, 0.0f);
+        if (y > hi) vy = rmin(vy, 0.0f);
+        if ((float)y <= terrain_height(cfg, b, x) && vy < 0.0f) {
+            const float vn = fabsf(vy);
+            vy = 0.0f;
+            const float mu = cfg.bnd_mu[b];
+            if (mu > 0.0f) {
+                const float fi = mu * vn;
+                const float vt = fabsf(vx);
+                vx = vt > fi ? vx * (1.0f - fi / vt) : 0.0f;
+            }
+        }
+    }
+}
+// BoundaryCondition::clamp_particle_position of every boundary in order:
+// mod.rs:103-111, predictive.rs:31-35, heightmap.rs:100-114
+__device__ __forceinline__ void boundary_clamp(const DevConfig& cfg, float& nx, float& ny) {
+    for (uint32_t q = 0; q < cfg.n_bnd; ++q) {
+        nx = rmin(rmax(nx, cfg.bnd_lo[q]), cfg.bnd_hi[q]);
+        ny = rmin(rmax(ny, cfg.bnd_lo[q]), cfg.bnd_hi[q]);
+        if (cfg.bnd_kind[q] == kBndHeightmap) {
+            uint32_t xc = nx >= 1.0f ? (uint32_t)nx : 0u;  // `as usize` saturates at 0
+            xc = xc < cfg.grid_res - 1u ? xc : cfg.grid_res - 1u;
+            const float h = terrain_height(cfg, q, xc);
+            if (ny < h + 1.0f) ny = h + 1.0f;
+        }
+    }
+}
+
 __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set, float dt, float ddt, int cx, int cy) {
     const DevConfig& cfg = p.cfg;
     const int R = (int)cfg.grid_res;
@@ -348,14 +417,7 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
     vx = vx + cfg.gx * dt;
     vy = vy + cfg.gy * dt;
     const uint32_t x = (uint32_t)cx, y = (uint32_t)cy;
-    for (uint32_t b = 0; b < cfg.n_slip; ++b) {
-        const uint32_t t = cfg.slip[b];
-        const uint32_t hi = (uint32_t)R - (t + 1);
-        if (x < t) vx = rmax(vx, 0.0f);
-        if (x > hi) vx = rmin(vx, 0.0f);
-        if (y < t) vy = rmax(vy, 0.0f);
-        if (y > hi) vy = rmin(vy, 0.0f);
-    }
+    for (uint32_t b = 0; b < cfg.n_bnd; ++b) boundary_velocity(cfg, b, x, y, vx, vy);
     const float vel_limit = cfg.cell / dt;
     const float spd = len2(vx, vy);
     if (spd > vel_limit) {
@@ -864,10 +926,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
                         ++n_clamp;
                     }
                     float nx = s.x + nvx * dt, ny = s.y + nvy * dt;
-                    for (uint32_t q = 0; q < cfg.n_slip; ++q) {
-                        nx = rmin(rmax(nx, cfg.slip_lo[q]), cfg.slip_hi[q]);
-                        ny = rmin(rmax(ny, cfg.slip_lo[q]), cfg.slip_hi[q]);
-                    }
+                    boundary_clamp(cfg, nx, ny);
                     s.vx = nvx;
                     s.vy = nvy;
                     s.c = M2{(b.a * kDInv) * cfg.apic_blend, (b.b * kDInv) * cfg.apic_blend,

@@ -214,6 +214,10 @@ def load_library() -> ctypes.CDLL:
                                                 P(CMaterialExt)]),
         "emh_sim_reinit_particle_state": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_add_slip_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_sim_add_friction_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
+        "emh_sim_add_heightmap_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                          ctypes.c_float, ctypes.c_uint32]),
+        "emh_sim_add_predictive_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
         "emh_sim_clear_boundaries": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_add_body": (ctypes.c_int, [ctypes.c_void_p, P(CSpawnRegion), P(ctypes.c_uint32)]),
         "emh_sim_set_particles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
@@ -416,6 +420,47 @@ class SlipBoundary:
     @classmethod
     def new(cls, thickness: int) -> "SlipBoundary":
         return cls(thickness)
+
+
+@dataclasses.dataclass(frozen=True)
+class FrictionBoundary:
+    """FrictionBoundary (forces/boundary/friction.rs): no-penetration walls with
+    Coulomb friction on the tangential grid velocity."""
+    thickness: int
+    friction_coefficient: float
+
+    @classmethod
+    def new(cls, thickness: int, friction_coefficient: float) -> "FrictionBoundary":
+        return cls(thickness, friction_coefficient)
+
+
+@dataclasses.dataclass(frozen=True)
+class HeightmapBoundary:
+    """HeightmapBoundary (forces/boundary/heightmap.rs): terrain heights per x
+    column (Coulomb friction on it) inside slip side and top walls."""
+    heights: Tuple[float, ...]
+    friction: float
+    wall_thickness: int
+
+    @classmethod
+    def new(cls, heights, friction: float, wall_thickness: int) -> "HeightmapBoundary":
+        return cls(tuple(float(h) for h in heights), friction, wall_thickness)
+
+    @classmethod
+    def flat_floor(cls, grid_res: int, floor_height: float, friction: float) -> "HeightmapBoundary":
+        return cls.new([floor_height] * grid_res, friction, 2)
+
+
+@dataclasses.dataclass(frozen=True)
+class PredictiveBoundary:
+    """PredictiveBoundary (forces/boundary/predictive.rs): slip walls with particles
+    kept inside [wall_min, R - 1 - wall_min]."""
+    thickness: int
+    predictive_wall_min: float
+
+    @classmethod
+    def new(cls, thickness: int, predictive_wall_min: float) -> "PredictiveBoundary":
+        return cls(thickness, predictive_wall_min)
 
 
 # --------------------------------------------------------------- materials
@@ -699,14 +744,24 @@ class Simulation:
         self.set_material(material_id, material)
         return self
 
-    def with_boundary(self, boundary: SlipBoundary) -> "Simulation":
+    def with_boundary(self, boundary) -> "Simulation":
         self.add_boundary_condition(boundary)
         return self
 
-    def add_boundary_condition(self, boundary: SlipBoundary) -> None:
-        if not isinstance(boundary, SlipBoundary):
-            raise EmergeError(-2, f"{type(boundary).__name__} is not supported (slip walls only)")
-        _check(load_library().emh_sim_add_slip_boundary(self._h, boundary.thickness))
+    def add_boundary_condition(self, boundary) -> None:
+        lib = load_library()
+        if isinstance(boundary, SlipBoundary):
+            _check(lib.emh_sim_add_slip_boundary(self._h, boundary.thickness))
+        elif isinstance(boundary, FrictionBoundary):
+            _check(lib.emh_sim_add_friction_boundary(self._h, boundary.thickness, boundary.friction_coefficient))
+        elif isinstance(boundary, HeightmapBoundary):
+            h = np.ascontiguousarray(boundary.heights, np.float32)
+            _check(lib.emh_sim_add_heightmap_boundary(self._h, h.ctypes.data, len(h), boundary.friction,
+                                                      boundary.wall_thickness))
+        elif isinstance(boundary, PredictiveBoundary):
+            _check(lib.emh_sim_add_predictive_boundary(self._h, boundary.thickness, boundary.predictive_wall_min))
+        else:
+            raise EmergeError(-2, f"{type(boundary).__name__} is not supported on the GPU path")
 
     def clear_boundaries(self) -> None:
         _check(load_library().emh_sim_clear_boundaries(self._h))

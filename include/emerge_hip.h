@@ -342,6 +342,16 @@ int emh_sim_set_material(emh_sim* sim, uint32_t material_id, const emh_material_
 int emh_sim_reinit_particle_state(emh_sim* sim);
 /* with_boundary(Box::new(SlipBoundary::new(t)))  lifecycle.rs:114-117, slip.rs:5-24 */
 int emh_sim_add_slip_boundary(emh_sim* sim, uint32_t thickness);
+/* with_boundary(Box::new(FrictionBoundary::new(t, mu)))  friction.rs:28-63: no-penetration
+ * walls with Coulomb friction (boundary/mod.rs:57-75); mu outside [0, 1] is EMH_ERR_INVALID_ARG. */
+int emh_sim_add_friction_boundary(emh_sim* sim, uint32_t thickness, float friction_coefficient);
+/* with_boundary(Box::new(HeightmapBoundary::new(heights, friction, wall_thickness)))
+ * heightmap.rs:44-114: terrain height per x column (the last height repeats past
+ * the end), Coulomb friction on it, slip side and top walls.  The heights are copied. */
+int emh_sim_add_heightmap_boundary(emh_sim* sim, const float* heights, uint32_t n_heights,
+                                   float friction, uint32_t wall_thickness);
+/* with_boundary(Box::new(PredictiveBoundary::new(t, wall_min)))  predictive.rs:17-35 */
+int emh_sim_add_predictive_boundary(emh_sim* sim, uint32_t thickness, float predictive_wall_min);
 /* clear_boundaries  lifecycle.rs:364-366 */
 int emh_sim_clear_boundaries(emh_sim* sim);
 /* add_body(spawn) -> tag  solver/particles.rs:355-424 */

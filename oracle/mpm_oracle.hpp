@@ -1167,11 +1167,98 @@ inline bool spawn_fits(const emh_spawn_region& sp, const emh_config& cfg) {
     return minx >= dmin && miny >= dmin && maxx <= dmax && maxy <= dmax;
 }
 
+// ---------------------------------------------------------------- boundaries
+// One record per installed BoundaryCondition, applied in installation order
+// (step.rs:334-344 per cell, g2p.rs:316-320 per particle).
+enum : uint32_t { BND_SLIP = 0, BND_FRICTION = 1, BND_HEIGHTMAP = 2, BND_PREDICTIVE = 3 };
+struct Boundary {
+    uint32_t kind = BND_SLIP;
+    uint32_t thickness = 0;         // Slip / Friction thickness, Heightmap wall_thickness
+    float mu = 0.0f;                // Friction coefficient, Heightmap terrain friction
+    float wall_min = 0.0f;          // Predictive predictive_wall_min
+    std::vector<float> heights;     // Heightmap terrain height per x column
+};
+
+// Coulomb wall  boundary/mod.rs:57-75
+template <class T> inline void coulomb_wall(V2<T>& v, V2<T> n, T mu) {
+    T vn = dot(v, n);
+    if (vn >= T(0)) return;
+    T normal_speed = std::fabs(vn);
+    V2<T> vt = v - n * vn;
+    T vt_len = length(vt);
+    T fi = mu * normal_speed;
+    v = vt_len > fi ? vt * ((vt_len - fi) / vt_len) : V2<T>{T(0), T(0)};
+}
+
+// BoundaryCondition::apply_to_grid_velocity: slip.rs:16-18 -> mod.rs:78-101,
+// friction.rs:40-59, heightmap.rs:66-98, predictive.rs:27-29
+template <class T> inline void boundary_velocity(const Boundary& b, uint32_t idx, uint32_t R, V2<T>& v) {
+    const uint32_t x = idx / R, y = idx % R, t = b.thickness;
+    switch (b.kind) {
+        case BND_SLIP:
+        case BND_PREDICTIVE: {
+            uint32_t hi = R - (t + 1);
+            if (x < t) v.x = rmax(v.x, T(0));
+            if (x > hi) v.x = rmin(v.x, T(0));
+            if (y < t) v.y = rmax(v.y, T(0));
+            if (y > hi) v.y = rmin(v.y, T(0));
+            break;
+        }
+        case BND_FRICTION: {
+            uint32_t hi = R > t + 1 ? R - (t + 1) : 0u;
+            T mu = T(b.mu);
+            if (x < t) coulomb_wall(v, V2<T>{T(1), T(0)}, mu);
+            if (x > hi) coulomb_wall(v, V2<T>{T(-1), T(0)}, mu);  // Vec2::NEG_X
+            if (y < t) coulomb_wall(v, V2<T>{T(0), T(1)}, mu);
+            if (y > hi) coulomb_wall(v, V2<T>{T(0), T(-1)}, mu);  // Vec2::NEG_Y
+            break;
+        }
+        case BND_HEIGHTMAP: {
+            uint32_t hi = R > t + 1 ? R - (t + 1) : 0u;
+            if (x < t) v.x = rmax(v.x, T(0));
+            if (x > hi) v.x = rmin(v.x, T(0));
+            if (y > hi) v.y = rmin(v.y, T(0));
+            T h = b.heights.empty() ? T(0) : T(b.heights[std::min<size_t>(x, b.heights.size() - 1)]);
+            if (T((float)y) <= h && v.y < T(0)) {
+                T vn = std::fabs(v.y);
+                v.y = T(0);
+                if (T(b.mu) > T(0)) {
+                    T fi = T(b.mu) * vn;
+                    T vt = std::fabs(v.x);
+                    v.x = vt > fi ? v.x * (T(1) - fi / vt) : T(0);
+                }
+            }
+            break;
+        }
+    }
+}
+
+// BoundaryCondition::clamp_particle_position: mod.rs:103-111 (Slip, Friction),
+// heightmap.rs:100-114, predictive.rs:31-35
+template <class T> inline V2<T> boundary_clamp(const Boundary& b, V2<T> pos, uint32_t R) {
+    auto clamp2 = [](V2<T> p, T lo, T hi) { return V2<T>{rmin(rmax(p.x, lo), hi), rmin(rmax(p.y, lo), hi)}; };
+    const uint32_t t = b.thickness;
+    if (b.kind == BND_PREDICTIVE) {
+        T lo = T(b.wall_min);
+        T hi = T((float)R - 1.0f) - T(b.wall_min);
+        return clamp2(pos, lo, hi);
+    }
+    pos = clamp2(pos, T((float)(t > 0 ? t - 1 : 0u)), T((float)(R > t ? R - t : 0u)));
+    if (b.kind == BND_HEIGHTMAP) {
+        // `as usize` saturates: negative and NaN give 0
+        uint32_t xc = (pos.x >= T(1)) ? (uint32_t)std::min<T>(pos.x, T(4294967295.0)) : 0u;
+        xc = std::min<uint32_t>(xc, R > 0 ? R - 1 : 0u);
+        T h = b.heights.empty() ? T(0) : T(b.heights[std::min<size_t>(xc, b.heights.size() - 1)]);
+        if (pos.y < h + T(1)) pos.y = h + T(1);
+    }
+    return pos;
+}
+
 // ---------------------------------------------------------------- simulation
 template <class T> struct Simulation {
     emh_config cfg{};
     std::vector<Material<T>> materials;
-    std::vector<uint32_t> slip_thickness;
+    std::vector<Boundary> boundaries;
     Particles<T> P;
     Grid<T> grid;
     uint32_t next_tag = 1;
@@ -1185,7 +1272,10 @@ template <class T> struct Simulation {
     void init_empty(const emh_config& c) {
         cfg = c;
         materials.assign(1, Material<T>{});  // FallbackMaterial
-        slip_thickness.assign(1, c.boundary_thickness);
+        Boundary b;
+        b.kind = BND_SLIP;
+        b.thickness = c.boundary_thickness;
+        boundaries.assign(1, b);
         grid.init(c.grid_res);
         last_step_dt = T(c.dt);
     }
@@ -1445,17 +1535,11 @@ template <class T> struct Simulation {
             Cell<T>& c = grid.cells[s];
             if (c.mass > T(0)) c.momentum = c.momentum + gdt;
         }
-        for (uint32_t t : slip_thickness) {  // apply_slip_wall_velocity  boundary/mod.rs:78-101
+        for (const Boundary& b : boundaries) {  // apply_boundary_conditions_to_grid  step.rs:334-344
             for (uint32_t s : grid.dirty_slots) {
                 Cell<T>& c = grid.cells[s];
                 if (!(c.mass > T(0))) continue;
-                uint32_t idx = grid.keys[s];
-                uint32_t hi = R - (t + 1);
-                uint32_t x = idx / R, y = idx % R;
-                if (x < t) c.momentum.x = rmax(c.momentum.x, T(0));
-                if (x > hi) c.momentum.x = rmin(c.momentum.x, T(0));
-                if (y < t) c.momentum.y = rmax(c.momentum.y, T(0));
-                if (y > hi) c.momentum.y = rmin(c.momentum.y, T(0));
+                boundary_velocity(b, grid.keys[s], R, c.momentum);
             }
         }
         for (uint32_t s : grid.dirty_slots) {
@@ -1499,11 +1583,7 @@ template <class T> struct Simulation {
                 clamped++;
             }
             V2<T> np = x + new_v * dt;
-            for (uint32_t t : slip_thickness) {  // clamp_position_inside_grid  boundary/mod.rs:103-111
-                T lo = T((float)(t > 0 ? t - 1 : 0));
-                T hi = T((float)(R > t ? R - t : 0));
-                np = {rmin(rmax(np.x, lo), hi), rmin(rmax(np.y, lo), hi)};
-            }
+            for (const Boundary& bd : boundaries) np = boundary_clamp(bd, np, R);  // g2p.rs:316-320
             P.v[i] = new_v;
             P.c[i] = (b * T(KERNEL_D_INVERSE)) * apic;
             P.x[i] = np;

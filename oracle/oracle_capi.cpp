@@ -102,16 +102,76 @@ int orc_reinit_particle_state(void* hv) {
     return with_sim(h, [&](auto& s) { s.reinit_all(); return EMH_OK; });
 }
 
-int orc_add_slip_boundary(void* hv, uint32_t t) {
+static int add_boundary(void* hv, const Boundary& b) {
     Handle* h = (Handle*)hv;
     if (!h) return EMH_ERR_INVALID_ARG;
-    return with_sim(h, [&](auto& s) { s.slip_thickness.push_back(t); return EMH_OK; });
+    return with_sim(h, [&](auto& s) { s.boundaries.push_back(b); return EMH_OK; });
+}
+
+int orc_add_slip_boundary(void* hv, uint32_t t) {  // SlipBoundary::new  slip.rs:9-13
+    Boundary b;
+    b.kind = BND_SLIP;
+    b.thickness = t;
+    return add_boundary(hv, b);
+}
+
+int orc_add_friction_boundary(void* hv, uint32_t t, float mu) {  // FrictionBoundary::new  friction.rs:28-37
+    if (!(mu >= 0.0f && mu <= 1.0f)) return EMH_ERR_INVALID_ARG;
+    Boundary b;
+    b.kind = BND_FRICTION;
+    b.thickness = t;
+    b.mu = mu;
+    return add_boundary(hv, b);
+}
+
+int orc_add_heightmap_boundary(void* hv, const float* heights, uint32_t n, float friction, uint32_t wall_t) {
+    Boundary b;  // HeightmapBoundary::new  heightmap.rs:44-50
+    b.kind = BND_HEIGHTMAP;
+    b.thickness = wall_t;
+    b.mu = friction;
+    b.heights.assign(heights, heights + n);
+    return add_boundary(hv, b);
+}
+
+int orc_add_predictive_boundary(void* hv, uint32_t t, float wall_min) {  // predictive.rs:17-24
+    Boundary b;
+    b.kind = BND_PREDICTIVE;
+    b.thickness = t;
+    b.wall_min = wall_min;
+    return add_boundary(hv, b);
 }
 
 int orc_clear_boundaries(void* hv) {
     Handle* h = (Handle*)hv;
     if (!h) return EMH_ERR_INVALID_ARG;
-    return with_sim(h, [&](auto& s) { s.slip_thickness.clear(); return EMH_OK; });
+    return with_sim(h, [&](auto& s) { s.boundaries.clear(); return EMH_OK; });
+}
+
+// One boundary's grid-velocity law / particle clamp on one cell or position (f32).
+void orc_boundary_velocity(uint32_t kind, uint32_t t, float mu, float wall_min, const float* heights,
+                           uint32_t n_heights, uint32_t cell_index, uint32_t grid_res, float v[2]) {
+    Boundary b;
+    b.kind = kind;
+    b.thickness = t;
+    b.mu = mu;
+    b.wall_min = wall_min;
+    if (heights) b.heights.assign(heights, heights + n_heights);
+    V2<float> vv{v[0], v[1]};
+    boundary_velocity(b, cell_index, grid_res, vv);
+    v[0] = vv.x;
+    v[1] = vv.y;
+}
+
+void orc_boundary_clamp(uint32_t kind, uint32_t t, float wall_min, const float* heights, uint32_t n_heights,
+                        uint32_t grid_res, float pos[2]) {
+    Boundary b;
+    b.kind = kind;
+    b.thickness = t;
+    b.wall_min = wall_min;
+    if (heights) b.heights.assign(heights, heights + n_heights);
+    V2<float> p = boundary_clamp(b, V2<float>{pos[0], pos[1]}, grid_res);
+    pos[0] = p.x;
+    pos[1] = p.y;
 }
 
 int orc_add_body(void* hv, const emh_spawn_region* sp, uint32_t* tag) {

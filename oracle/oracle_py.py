@@ -56,6 +56,15 @@ def load() -> ctypes.CDLL:
         "orc_reinit_particle_state": (ctypes.c_int, [ctypes.c_void_p]),
         "orc_add_slip_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "orc_clear_boundaries": (ctypes.c_int, [ctypes.c_void_p]),
+        "orc_add_friction_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
+        "orc_add_heightmap_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                      ctypes.c_float, ctypes.c_uint32]),
+        "orc_add_predictive_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
+        "orc_boundary_velocity": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         P(ctypes.c_float)]),
+        "orc_boundary_clamp": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p,
+                                      ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_float)]),
         "orc_add_body": (ctypes.c_int, [ctypes.c_void_p, P(CSpawnRegion), P(ctypes.c_uint32)]),
         "orc_set_particles": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
         "orc_num_particles": (ctypes.c_size_t, [ctypes.c_void_p]),
@@ -158,7 +167,19 @@ class OracleSim:
         return self
 
     def with_boundary(self, b):
-        self._ok(load().orc_add_slip_boundary(self._h, b.thickness))
+        lib = load()
+        kind = type(b).__name__
+        if kind == "SlipBoundary":
+            self._ok(lib.orc_add_slip_boundary(self._h, b.thickness))
+        elif kind == "FrictionBoundary":
+            self._ok(lib.orc_add_friction_boundary(self._h, b.thickness, b.friction_coefficient))
+        elif kind == "HeightmapBoundary":
+            h = np.ascontiguousarray(b.heights, np.float32)
+            self._ok(lib.orc_add_heightmap_boundary(self._h, h.ctypes.data, len(h), b.friction, b.wall_thickness))
+        elif kind == "PredictiveBoundary":
+            self._ok(lib.orc_add_predictive_boundary(self._h, b.thickness, b.predictive_wall_min))
+        else:
+            raise ValueError(kind)
         return self
 
     def clear_boundaries(self):
@@ -287,6 +308,27 @@ def nacc_project(m: Material, f, alpha: float):
     ao = ctypes.c_float()
     load().orc_nacc_project(ctypes.byref(m.params), ctypes.byref(m.ext), _f4(f), alpha, fo, ctypes.byref(ao))
     return np.array(fo[:], np.float32), ao.value
+
+
+BOUNDARY_KINDS = {"slip": 0, "friction": 1, "heightmap": 2, "predictive": 3}
+
+
+def boundary_velocity(kind: str, cell_index: int, grid_res: int, v, thickness=2, mu=0.0, wall_min=0.0,
+                      heights=None):
+    vv = (ctypes.c_float * 2)(*v)
+    h = None if heights is None else np.ascontiguousarray(heights, np.float32)
+    load().orc_boundary_velocity(BOUNDARY_KINDS[kind], thickness, mu, wall_min,
+                                 None if h is None else h.ctypes.data, 0 if h is None else len(h),
+                                 cell_index, grid_res, vv)
+    return (vv[0], vv[1])
+
+
+def boundary_clamp(kind: str, pos, grid_res: int, thickness=2, wall_min=0.0, heights=None):
+    pp = (ctypes.c_float * 2)(*pos)
+    h = None if heights is None else np.ascontiguousarray(heights, np.float32)
+    load().orc_boundary_clamp(BOUNDARY_KINDS[kind], thickness, wall_min,
+                              None if h is None else h.ctypes.data, 0 if h is None else len(h), grid_res, pp)
+    return (pp[0], pp[1])
 
 
 def slip_velocity(thickness: int, cell_index: int, grid_res: int, v):

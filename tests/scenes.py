@@ -290,6 +290,34 @@ def bingham_dam(backend: str):
     return sim
 
 
+def terrain_slide(backend: str):
+    """Sand block dropped on a heightmap slope (terrain friction 0.3) inside
+    Coulomb-friction walls (mu = 0.4): it lands, slides down and piles against
+    the right wall."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    heights = [max(3.0, 20.0 - 0.3 * x) for x in range(64)]
+    sim = new_sim(backend, cfg).with_default_material(m.drucker_prager(3.5e5, 0.3, friction_deg=30.0))
+    sim.clear_boundaries()
+    sim.with_boundary(eh.HeightmapBoundary.new(heights, 0.3, 2)).with_boundary(eh.FrictionBoundary.new(2, 0.4))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(10, 6), box_center=(14.0, 21.5), rng_seed=41,
+                                position_jitter=0.2))
+    return _add_velocity(sim, (3.0, -5.0))
+
+
+def predictive_bounce(backend: str):
+    """Corotated block thrown at the right wall of a predictive boundary
+    (keep-out 4 cells) on top of the default slip walls."""
+    cfg = eh.SimConfig.standard(48, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.corotated(2000.0, 1000.0))
+           .with_boundary(eh.PredictiveBoundary.new(2, 4.0)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(8, 8), box_center=(38.0, 12.0), rng_seed=43,
+                                position_jitter=0.2))
+    return _add_velocity(sim, (12.0, 0.0))
+
+
 def multi_material(backend: str):
     """BASELINE config 3 in miniature: NeoHookean, water, snow and corotated
     boxes with N(0, 0.5)-like random velocities, one grid."""
@@ -324,6 +352,8 @@ SCENES: Dict[str, Scene] = {
     "mui_column": Scene("mui_column", 10, mui_column),
     "mud_drop": Scene("mud_drop", 10, mud_drop),
     "bingham_dam": Scene("bingham_dam", 10, bingham_dam),
+    "terrain_slide": Scene("terrain_slide", 15, terrain_slide),
+    "predictive_bounce": Scene("predictive_bounce", 10, predictive_bounce),
     "multi_material": Scene("multi_material", 10, multi_material),
 }
 

@@ -198,6 +198,9 @@ def test_error_paths():
     with pytest.raises(eh.EmergeError) as ei:
         sim.set_material(0, active)
     assert ei.value.code == -2
+    with pytest.raises(eh.EmergeError) as ei:  # friction.rs:29-32
+        sim.with_boundary(eh.FrictionBoundary.new(2, 1.5))
+    assert ei.value.code == -1
     with pytest.raises(eh.EmergeError) as ei:
         sim.set_material(3, eh.NeoHookeanMaterial.new(1.0, 1.0))  # not contiguous (registry.rs:39-46)
     assert ei.value.code == -1

@@ -673,6 +673,63 @@ def test_slip_wall_laws(oracle):
     assert oracle.slip_velocity(2, 30 * r + 30, r, (-9.0, -9.0)) == (-9.0, -9.0)  # interior
 
 
+def test_coulomb_wall_laws(oracle):
+    """boundary/mod.rs:154-222 through FrictionBoundary (friction.rs:40-59) on the
+    left wall (x = 0, y = 32 of a 64 grid, thickness 2): µ = 0 is slip with the
+    normal zeroed; the tangential speed drops by exactly µ|v_n|; it never reverses;
+    outward velocity is untouched."""
+    bv = oracle.boundary_velocity
+    assert bv("friction", 32, 64, (-3.0, 7.5), mu=0.0) == (0.0, 7.5)
+    vx, vy = bv("friction", 32, 64, (-4.0, 10.0), mu=0.3)
+    assert vx == 0.0 and abs(vy - (10.0 - 0.3 * 4.0)) < 1e-5
+    assert bv("friction", 32, 64, (-10.0, 2.0), mu=0.9) == (0.0, 0.0)
+    assert bv("friction", 32, 64, (4.0, -2.0), mu=0.9) == (4.0, -2.0)
+    # floor (y < t, normal +Y) and right wall (x > R - t - 1, normal -X)
+    r = 64
+    vx, vy = bv("friction", 30 * r + 1, r, (6.0, -2.0), mu=0.5)
+    assert vy == 0.0 and abs(vx - 5.0) < 1e-6
+    vx, vy = bv("friction", 62 * r + 30, r, (2.0, -3.0), mu=0.5)
+    assert vx == 0.0 and abs(vy + 2.0) < 1e-6
+    # a corner cell is processed wall by wall: left wall then floor
+    vx, vy = bv("friction", 0 * r + 0, r, (-1.0, -4.0), mu=0.25)
+    assert (vx, vy) == (0.0, 0.0) or (vx == 0.0 and abs(vy) < 1e-6)
+    assert bv("friction", 30 * r + 30, r, (-9.0, -9.0), mu=1.0) == (-9.0, -9.0)
+
+
+def test_heightmap_boundary_laws(oracle):
+    """heightmap.rs:66-114: cells at or below the terrain lose downward velocity and
+    µ|v_n| of their horizontal speed; slip side and top walls, no bottom wall; heights
+    past the end repeat the last one; particles are pushed to terrain + 1."""
+    r = 64
+    hts = [3.0] * 20 + [3.0 + 0.5 * k for k in range(10)]  # 30 columns, ramp at the end
+    bv = lambda idx, v, mu=0.4: oracle.boundary_velocity("heightmap", idx, r, v, mu=mu, heights=hts)
+    vx, vy = bv(10 * r + 3, (5.0, -2.0))             # on the floor: y = 3 <= 3
+    assert vy == 0.0 and abs(vx - 5.0 * (1 - 0.8 / 5.0)) < 1e-6
+    assert bv(10 * r + 4, (5.0, -2.0)) == (5.0, -2.0)  # above the terrain
+    assert bv(10 * r + 3, (5.0, 2.0)) == (5.0, 2.0)    # moving up
+    assert bv(10 * r + 3, (0.5, -2.0)) == (0.0, 0.0)   # friction stops it
+    assert bv(10 * r + 3, (5.0, -2.0), mu=0.0) == (5.0, 0.0)
+    assert bv(50 * r + 8, (1.0, -1.0)) == (1.0, -1.0)  # column 50 -> last height 7.5: 8 is above
+    vx, vy = bv(50 * r + 7, (1.0, -1.0))                 # 7 <= 7.5: blocked
+    assert vy == 0.0 and abs(vx - 0.6) < 1e-6
+    assert bv(1 * r + 30, (-2.0, 1.0)) == (0.0, 1.0)    # left wall
+    assert bv(30 * r + 62, (1.0, 3.0)) == (1.0, 0.0)    # top wall
+    assert bv(30 * r + 1, (1.0, -3.0))[1] == 0.0        # floor comes from the terrain alone
+    bc = lambda p: oracle.boundary_clamp("heightmap", p, r, heights=hts)
+    assert bc((10.5, 2.0)) == (10.5, 4.0)
+    assert bc((10.5, 9.0)) == (10.5, 9.0)
+    assert bc((-5.0, 3.5)) == (1.0, 4.0)                # wall clamp, then column 1
+    assert bc((100.0, 5.0)) == (62.0, 8.5)              # x clamped to R - t, last height 7.5
+
+
+def test_predictive_and_friction_clamps(oracle):
+    """predictive.rs:31-35: clamp to [wall_min, R - 1 - wall_min]; friction.rs:61-63 and
+    mod.rs:103-111: [t - 1, R - t]."""
+    assert oracle.boundary_clamp("predictive", (0.5, 70.0), 64, wall_min=3.5) == (3.5, 59.5)
+    assert oracle.boundary_clamp("friction", (0.5, 70.0), 64, thickness=3) == (2.0, 61.0)
+    assert oracle.boundary_velocity("predictive", 32, 64, (-3.0, 7.5)) == (0.0, 7.5)
+
+
 def test_lcg_closed_form(oracle):
     """LcgRng (solver/mod.rs:154-175): s <- s·1664525 + 1013904223 mod 2^32; f = s / 2^32 in f32."""
     u, f = oracle.lcg(1, 5)
