@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/emerge_hip.h"
+#include "interop_kernels.hpp"
 #include "kernels.hpp"
 #include "spawn_kernels.hpp"
 
@@ -345,6 +346,9 @@ struct emh_sim {
     std::vector<HostBoundary> bnds;
     float* d_heights = nullptr;   // terrain heights of the heightmap boundaries
     size_t heights_cap = 0;
+    emh_particle* d_cold = nullptr;  // the last upload's 128 B records, caller order (export)
+    size_t cold_cap = 0;
+    bool cold_valid = false;
     std::vector<emh_particle> host;  // authoritative when !device_ahead
     uint32_t next_tag = 1;
     bool host_dirty = true;     // host changed since the last upload
@@ -405,6 +409,7 @@ struct emh_sim {
         release_particles();
         if (d_mats) (void)hipFree(d_mats);
         if (d_heights) (void)hipFree(d_heights);
+        if (d_cold) (void)hipFree(d_cold);
         if (grid_block) (void)hipFree(grid_block);
         if (d_cell_count) (void)hipFree(d_cell_count);
         if (d_cell_start) (void)hipFree(d_cell_start);
@@ -419,6 +424,9 @@ struct emh_sim {
         d_mats = nullptr;
         d_heights = nullptr;
         heights_cap = 0;
+        d_cold = nullptr;
+        cold_cap = 0;
+        cold_valid = false;
         grid_block = nullptr;
         d_cell_count = d_cell_start = d_tile_count = d_tile_start = nullptr;
         d_ctrl = nullptr;
@@ -689,6 +697,89 @@ struct emh_sim {
         substeps_since_reorder = kReorderSubsteps;  // first step puts the upload in cell-key order
         host_dirty = false;
         device_ahead = false;
+        cold_valid = false;
+        return EMH_OK;
+    }
+
+    // The control state the last launch left (st[seq & 1]).
+    int read_ctrl(CtrlState& st) {
+        Ctrl ctrl;
+        HIP_TRY(hipMemcpyAsync(&ctrl, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        st = ctrl.st[seq & 1u];
+        return EMH_OK;
+    }
+
+    // After a device-side edit of the stored particles between steps: the
+    // loop's pending CFL reduction and the next substep's P2G were formed
+    // from the old state, so restart the loop from the stored particles in
+    // buffer `cur_buf` as an upload does (no reorder is forced).
+    int restart_loop(uint32_t cur_buf) {
+        Ctrl ctrl;
+        std::memset(&ctrl, 0, sizeof(ctrl));
+        ctrl.st[0].total_substeps = stats.total_substeps;
+        ctrl.st[0].last_dt = stats.last_step_dt;
+        ctrl.st[0].dt = stats.last_step_dt;
+        ctrl.st[0].free_dt = cfg.dt;
+        ctrl.st[0].cur_buf = cur_buf & 1u;
+        for (auto& r : ctrl.red) {
+            r.max_speed = 0u;
+            r.min_mdt = kInfBits;
+            r.dirty = 0u;
+        }
+        seq = 0;
+        HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
+        HIP_TRY(launch_zero_all(pipe, stream));
+        HIP_TRY(launch_prologue(pipe, seq++, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        device_ahead = true;
+        return EMH_OK;
+    }
+
+    // Simulation::apply_impulse / apply_radial_impulse  solver/particles.rs:115-165
+    int impulse(float cx, float cy, float radius, float fx, float fy, float strength, uint32_t radial) {
+        if (host_dirty) {
+            int rc = upload();
+            if (rc) return rc;
+        }
+        if (pipe.n == 0) return EMH_OK;
+        CtrlState st;
+        int rc = read_ctrl(st);
+        if (rc) return rc;
+        const float vel_limit = cfg.grid_cell_size / cfg.min_dt;
+        HIP_TRY(launch_impulse(pipe.buf[st.cur_buf & 1u], pipe.n, cx, cy, radius, fx, fy, strength, radial, vel_limit,
+                               stream));
+        return restart_loop(st.cur_buf);
+    }
+
+    // Particle records in caller order into a device buffer of `cap` records.
+    int export_particles(void* dst, size_t cap) {
+        if (host_dirty) {
+            int rc = upload();
+            if (rc) return rc;
+        }
+        const uint32_t n = pipe.n;
+        if (cap < n) return fail(EMH_ERR_INVALID_ARG, "buffer holds %zu particles, %u needed", cap, n);
+        if (n == 0) return EMH_OK;
+        if (!cold_valid) {
+            if (cold_cap < n) {
+                if (d_cold) (void)hipFree(d_cold);
+                d_cold = nullptr;
+                cold_cap = 0;
+                if (hipMalloc((void**)&d_cold, sizeof(emh_particle) * n) != hipSuccess) {
+                    d_cold = nullptr;
+                    return fail(EMH_ERR_OUT_OF_MEMORY, "export staging of %u particles", n);
+                }
+                cold_cap = n;
+            }
+            HIP_TRY(hipMemcpyAsync(d_cold, host.data(), sizeof(emh_particle) * n, hipMemcpyHostToDevice, stream));
+            cold_valid = true;
+        }
+        CtrlState st;
+        int rc = read_ctrl(st);
+        if (rc) return rc;
+        HIP_TRY(launch_export_particles(pipe.buf[st.cur_buf & 1u], n, d_cold, (emh_particle*)dst, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
         return EMH_OK;
     }
 
@@ -1401,6 +1492,39 @@ int emh_sim_step(emh_sim* s, uint32_t n) {
 int emh_sim_stats(const emh_sim* s, emh_step_stats* out) {
     if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
     *out = s->stats;
+    return EMH_OK;
+}
+
+int emh_sim_set_gravity(emh_sim* s, float gx, float gy) {  // lifecycle.rs:403-405
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    s->cfg.gravity[0] = gx;
+    s->cfg.gravity[1] = gy;
+    s->pipe.cfg.gx = gx;  // read by the next grid update; nothing queued holds the old value
+    s->pipe.cfg.gy = gy;
+    return EMH_OK;
+}
+
+int emh_sim_apply_impulse(emh_sim* s, float cx, float cy, float radius, float fx, float fy) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    return s->impulse(cx, cy, radius, fx, fy, 0.0f, 0u);
+}
+
+int emh_sim_apply_radial_impulse(emh_sim* s, float cx, float cy, float radius, float strength) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    return s->impulse(cx, cy, radius, 0.0f, 0.0f, strength, 1u);
+}
+
+int emh_sim_export_particles_device(emh_sim* s, void* dst, size_t capacity) {
+    if (!s || !dst) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    return s->export_particles(dst, capacity);
+}
+
+int emh_sim_export_grid_device(emh_sim* s, void* dst, size_t capacity_cells) {
+    if (!s || !dst) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    const size_t R2 = (size_t)s->res() * s->res();
+    if (capacity_cells < R2) return fail(EMH_ERR_INVALID_ARG, "buffer holds %zu cells, %zu needed", capacity_cells, R2);
+    HIP_TRY(launch_export_grid(s->pipe, s->seq, (float4*)dst, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     return EMH_OK;
 }
 

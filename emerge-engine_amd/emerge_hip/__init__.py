@@ -214,6 +214,11 @@ def load_library() -> ctypes.CDLL:
                                                 P(CMaterialExt)]),
         "emh_sim_reinit_particle_state": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_add_slip_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_sim_export_particles_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "emh_sim_export_grid_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "emh_sim_set_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]),
+        "emh_sim_apply_impulse": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 5),
+        "emh_sim_apply_radial_impulse": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 4),
         "emh_sim_add_friction_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
         "emh_sim_add_heightmap_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                           ctypes.c_float, ctypes.c_uint32]),
@@ -834,6 +839,29 @@ class Simulation:
             v = getattr(out, name)
             d[name] = tuple(v) if isinstance(v, ctypes.Array) else v
         return d
+
+    def set_gravity(self, gravity) -> None:
+        """Simulation::set_gravity (lifecycle.rs:403-405)."""
+        _check(load_library().emh_sim_set_gravity(self._h, float(gravity[0]), float(gravity[1])))
+        self.config = self.config.replace(gravity=(float(np.float32(gravity[0])), float(np.float32(gravity[1]))))
+
+    def apply_impulse(self, center, radius: float, force) -> None:
+        """Simulation::apply_impulse (solver/particles.rs:115-139), on the device."""
+        _check(load_library().emh_sim_apply_impulse(self._h, float(center[0]), float(center[1]), float(radius),
+                                                    float(force[0]), float(force[1])))
+
+    def apply_radial_impulse(self, center, radius: float, strength: float) -> None:
+        """Simulation::apply_radial_impulse (solver/particles.rs:141-165), on the device."""
+        _check(load_library().emh_sim_apply_radial_impulse(self._h, float(center[0]), float(center[1]),
+                                                           float(radius), float(strength)))
+
+    def export_particles_device(self, dst_ptr: int, capacity: int) -> None:
+        """Writes the particles (128 B records, caller order) into device memory at dst_ptr."""
+        _check(load_library().emh_sim_export_particles_device(self._h, ctypes.c_void_p(dst_ptr), capacity))
+
+    def export_grid_device(self, dst_ptr: int, capacity_cells: int) -> None:
+        """Writes grid_res^2 cells {momentum.x, momentum.y, mass, 0} into device memory at dst_ptr."""
+        _check(load_library().emh_sim_export_grid_device(self._h, ctypes.c_void_p(dst_ptr), capacity_cells))
 
     def grid(self) -> Tuple[np.ndarray, np.ndarray]:
         """(mass[R*R], momentum[R*R, 2]) of the last substep's P2G, flat index x*R+y."""

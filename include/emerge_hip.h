@@ -377,6 +377,26 @@ int emh_sim_snapshot(emh_sim* sim, emh_snapshot* out);
 /* Wait for all queued device work of this handle. */
 int emh_sim_synchronize(emh_sim* sim);
 
+/* ---- device-resident access (GpuSimulation's buffer API) -----------------
+ * For consumers on the same GPU, e.g. a renderer: the state is written into
+ * caller-owned device memory (hipMalloc'd or interop-mapped) with no host
+ * round trip.  Each call returns once the copy is complete. */
+/* GpuSimulation::particle_buffer (gpu/solver/mod.rs:466-471): every particle as
+ * its 128 B Particle record, in caller order, into `dst` (capacity in records). */
+int emh_sim_export_particles_device(emh_sim* sim, void* dst, size_t capacity);
+/* GpuSimulation::grid_buffer (gpu/solver/mod.rs:510-515): grid_res^2 cells of
+ * 16 B {momentum.x, momentum.y, mass, 0} (x-major), the same grid as
+ * emh_sim_get_grid. */
+int emh_sim_export_grid_device(emh_sim* sim, void* dst, size_t capacity_cells);
+/* Simulation::set_gravity  lifecycle.rs:403-405 (applies from the next step) */
+int emh_sim_set_gravity(emh_sim* sim, float gx, float gy);
+/* Simulation::apply_impulse(center, radius, force)  solver/particles.rs:115-139:
+ * v += force (1 - |d| / radius) within radius, speed clamped to
+ * grid_cell_size / min_dt.  Applied on the device. */
+int emh_sim_apply_impulse(emh_sim* sim, float cx, float cy, float radius, float fx, float fy);
+/* Simulation::apply_radial_impulse(center, radius, strength)  solver/particles.rs:141-165 */
+int emh_sim_apply_radial_impulse(emh_sim* sim, float cx, float cy, float radius, float strength);
+
 /* ---- per-kernel timing ---------------------------------------------------
  * HIP events on the library's stream, three per step: two entries, summed
  * over the steps taken since enabling -- "step_setup" (launches = steps): the

@@ -1268,6 +1268,32 @@ template <class T> struct Simulation {
     uint64_t frame_index = 0;
     std::vector<T> substep_dts;  // record of every substep dt taken (diagnostics)
 
+    // Simulation::apply_impulse / apply_radial_impulse  solver/particles.rs:115-165
+    // (every particle is awake: the sleep partition is off)
+    void apply_impulse(V2<T> center, T radius, V2<T> force, T strength, bool radial) {
+        T vel_limit = T(cfg.grid_cell_size) / T(cfg.min_dt);
+        T r2 = radius * radius;
+        for (size_t i = 0; i < P.size(); ++i) {
+            V2<T> d = P.x[i] - center;
+            T dist2 = dot(d, d);
+            if (!(dist2 <= r2 && dist2 > T(1e-8f))) continue;
+            if (radial) {
+                T dist = o_sqrt(dist2);
+                T falloff = T(1) - dist / radius;
+                P.v[i] = P.v[i] + ((d / dist) * strength) * falloff;
+            } else {
+                T falloff = T(1) - o_sqrt(dist2 / r2);
+                P.v[i] = P.v[i] + force * falloff;
+            }
+            T spd = length(P.v[i]);
+            if (spd > vel_limit) P.v[i] = P.v[i] * (vel_limit / spd);
+        }
+    }
+
+    void apply_impulse_f(float cx, float cy, float radius, float fx, float fy, float strength, bool radial) {
+        apply_impulse(V2<T>{T(cx), T(cy)}, T(radius), V2<T>{T(fx), T(fy)}, T(strength), radial);
+    }
+
     // Simulation::empty  lifecycle.rs:26-55
     void init_empty(const emh_config& c) {
         cfg = c;

@@ -141,6 +141,26 @@ int orc_add_predictive_boundary(void* hv, uint32_t t, float wall_min) {  // pred
     return add_boundary(hv, b);
 }
 
+int orc_apply_impulse(void* hv, float cx, float cy, float radius, float fx, float fy, float strength,
+                      uint32_t radial) {
+    Handle* h = (Handle*)hv;
+    if (!h) return EMH_ERR_INVALID_ARG;
+    return with_sim(h, [&](auto& s) {
+        s.apply_impulse_f(cx, cy, radius, fx, fy, strength, radial != 0u);
+        return EMH_OK;
+    });
+}
+
+int orc_set_gravity(void* hv, float gx, float gy) {  // lifecycle.rs:403-405
+    Handle* h = (Handle*)hv;
+    if (!h) return EMH_ERR_INVALID_ARG;
+    return with_sim(h, [&](auto& s) {
+        s.cfg.gravity[0] = gx;
+        s.cfg.gravity[1] = gy;
+        return EMH_OK;
+    });
+}
+
 int orc_clear_boundaries(void* hv) {
     Handle* h = (Handle*)hv;
     if (!h) return EMH_ERR_INVALID_ARG;

@@ -56,6 +56,8 @@ def load() -> ctypes.CDLL:
         "orc_reinit_particle_state": (ctypes.c_int, [ctypes.c_void_p]),
         "orc_add_slip_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "orc_clear_boundaries": (ctypes.c_int, [ctypes.c_void_p]),
+        "orc_apply_impulse": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 6 + [ctypes.c_uint32]),
+        "orc_set_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]),
         "orc_add_friction_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
         "orc_add_heightmap_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                       ctypes.c_float, ctypes.c_uint32]),
@@ -184,6 +186,17 @@ class OracleSim:
 
     def clear_boundaries(self):
         self._ok(load().orc_clear_boundaries(self._h))
+
+    def set_gravity(self, g):
+        self._ok(load().orc_set_gravity(self._h, float(g[0]), float(g[1])))
+
+    def apply_impulse(self, center, radius, force):
+        self._ok(load().orc_apply_impulse(self._h, float(center[0]), float(center[1]), float(radius),
+                                          float(force[0]), float(force[1]), 0.0, 0))
+
+    def apply_radial_impulse(self, center, radius, strength):
+        self._ok(load().orc_apply_impulse(self._h, float(center[0]), float(center[1]), float(radius), 0.0, 0.0,
+                                          float(strength), 1))
 
     def reinit_all_particle_state(self):
         self._ok(load().orc_reinit_particle_state(self._h))
