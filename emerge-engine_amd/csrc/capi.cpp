@@ -1682,10 +1682,7 @@ uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t R) {
     int32_t hi = (int32_t)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
-    const uint32_t tpd = (R + 15u) / 16u;
-    const uint32_t tx = (uint32_t)cx >> 4, ty = (uint32_t)cy >> 4;
-    const uint32_t tile = tx * tpd + ((tx & 1u) ? (tpd - 1u - ty) : ty);
-    return tile * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
+    return emh::cell_key(cx, cy, R, (R + 15u) / 16u);
 }
 
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out, uint32_t* perm_out) {

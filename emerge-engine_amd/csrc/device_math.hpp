@@ -870,16 +870,21 @@ EMH_DEV float affine_cfl(const M2& c, float cell) {
 }
 
 // Cell key: 16x16-cell tiles in serpentine order (tile column tx runs up in
-// y when tx is even and down when odd, so consecutive tiles always share an
-// edge), local cell x-major inside the tile; cell coordinates are clamped
-// into the grid.
-EMH_DEV uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t tpd) {
+// y when tx is even and down when odd), and inside a tile rows of 16 cells
+// (x within the row) taken in the same vertical direction as the tile column.
+// Consecutive rows are then always vertically adjacent (across tiles too), so
+// any run of keys covers a compact strip at most 16 cells wide (32 at a turn
+// between tile columns).  Cell coordinates are clamped into the grid.
+__host__ __device__ inline uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t tpd) {
     int hi = (int)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
     const uint32_t tx = (uint32_t)cx >> 4, ty = (uint32_t)cy >> 4;
-    const uint32_t tile = tx * tpd + ((tx & 1u) ? (tpd - 1u - ty) : ty);
-    return tile * 256u + (((uint32_t)cx & 15u) << 4) + ((uint32_t)cy & 15u);
+    const uint32_t odd = tx & 1u;
+    const uint32_t tile = tx * tpd + (odd ? (tpd - 1u - ty) : ty);
+    const uint32_t ly = (uint32_t)cy & 15u;
+    const uint32_t row = odd ? 15u - ly : ly;
+    return tile * 256u + (row << 4) + ((uint32_t)cx & 15u);
 }
 
 }  // namespace emh

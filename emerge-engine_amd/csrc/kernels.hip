@@ -132,7 +132,7 @@ __device__ void block_maxmin(float& a, float& d, float* s_red) {
 
 // Arrival rank of `key` among the lanes and waves that add to counts[key];
 // same-key lanes of a wave share one atomic.  Every lane of the wave must call.
-__device__ uint32_t agg_rank(uint32_t* counts, uint32_t key, bool valid) {
+__device__ __forceinline__ uint32_t agg_rank(uint32_t* counts, uint32_t key, bool valid) {
     const uint32_t lane = lane_id();
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     uint32_t rank = 0;
@@ -748,6 +748,63 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
     return block_box(mnx, mny, mxx, mxy, margin, s_box);
 }
 
+// Processing order of a chunk: its nvalid particles sorted by base cell (a
+// counting sort over the box cells in LDS, arrival order inside a cell), so
+// that the lanes of a wave hold long runs of one cell however far the stored
+// order has drifted since the last physical reorder.  Lane t's k-th particle
+// is slot s_perm[k * kBlock + t] of the chunk.  s_cnt: kLdsCells scratch.
+__device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
+                                            uint32_t nvalid, uint32_t* s_cnt, uint32_t* s_perm, uint32_t* s_scan) {
+    if (!box.lds) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = k * kBlock + threadIdx.x;
+        __syncthreads();
+        return;
+    }
+    const int cells = box.w * box.h;
+    for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
+    __syncthreads();
+    uint32_t cell[kPer], rank[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t q = k * kBlock + threadIdx.x;
+        const bool valid = q < nvalid;
+        cell[k] = 0u;
+        if (valid) {  // the base cell chunk_box saw (same projection)
+            float2 x = a.x[base + q];
+            if (project) project_x(x.x, x.y, p.cfg);
+            cell[k] = (uint32_t)box.index(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)));
+        }
+        // one returning LDS atomic per particle: same-cell lanes serialise in
+        // the LDS unit, with no per-cell loop of dependent round trips
+        rank[k] = valid ? atomicAdd(&s_cnt[cell[k]], 1u) : 0u;
+    }
+    __syncthreads();
+    constexpr int kCellsPerThread = kLdsCells / kBlock;
+    uint32_t cnt[kCellsPerThread], sum = 0;
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+        const int idx = threadIdx.x * kCellsPerThread + i;
+        cnt[i] = idx < cells ? s_cnt[idx] : 0u;
+        sum += cnt[i];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(sum, s_scan, total);
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+        const int idx = threadIdx.x * kCellsPerThread + i;
+        if (idx < cells) s_cnt[idx] = off;
+        off += cnt[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t q = k * kBlock + threadIdx.x;
+        if (q < nvalid) s_perm[s_cnt[cell[k]] + rank[k]] = q;
+    }
+    __syncthreads();
+}
+
 // P2G terms of a (projected) particle state (p2g.rs:24-72, tau from
 // transfer.rs:77-104); the stress coefficient as p2g.rs:42-44 forms it,
 // ((-V) * 4) * dt', and without dt' for the correction moment.
@@ -787,6 +844,75 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
     scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, s_acc[kAccSy], g[kAccSy], R);
 }
 
+// Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
+// sorted by base cell inside the slice's box: s_perm[k*kBlock + r].
+__device__ __forceinline__ void slice_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
+                                            uint32_t k, uint32_t nvalid, uint32_t* s_cnt, uint32_t* s_perm,
+                                            uint32_t* s_scan) {
+    const uint32_t q = k * kBlock + threadIdx.x;
+    const bool valid = q < nvalid;
+    if (!box.lds) {
+        __syncthreads();
+        return;
+    }
+    const int cells = box.w * box.h;
+    for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
+    __syncthreads();
+    uint32_t cell = 0u, rank = 0u;
+    if (valid) {
+        float2 x = a.x[base + q];
+        if (project) project_x(x.x, x.y, p.cfg);
+        cell = (uint32_t)box.index(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)));
+        rank = atomicAdd(&s_cnt[cell], 1u);
+    }
+    __syncthreads();
+    constexpr int kCellsPerThread = kLdsCells / kBlock;
+    uint32_t cnt[kCellsPerThread], sum = 0;
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+        const int idx = threadIdx.x * kCellsPerThread + i;
+        cnt[i] = idx < cells ? s_cnt[idx] : 0u;
+        sum += cnt[i];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(sum, s_scan, total);
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+        const int idx = threadIdx.x * kCellsPerThread + i;
+        if (idx < cells) s_cnt[idx] = off;
+        off += cnt[i];
+    }
+    __syncthreads();
+    if (valid) s_perm[k * kBlock + s_cnt[cell] + rank] = q;
+    __syncthreads();
+}
+
+// Box of one kBlock-particle slice (lane's particle j when valid).
+__device__ __forceinline__ Box slice_box(const Pipeline& p, const PBuf& a, uint32_t j, bool valid, bool project, int margin,
+                                         int* s_box) {
+    int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
+    if (valid) {
+        float2 x = a.x[j];
+        if (project) project_x(x.x, x.y, p.cfg);
+        mnx = mxx = sat_i32(floorf(x.x));
+        mny = mxy = sat_i32(floorf(x.y));
+    }
+    return block_box(mnx, mny, mxx, mxy, margin, s_box);
+}
+
+// Clear the box's LDS accumulators and stage its updated grid velocity.
+template <bool G2P, int NCH>
+__device__ __forceinline__ void stage_box(const Pipeline& p, const Box& box, uint32_t cur, float dt, float ddt,
+                                          float (*s_acc)[kLdsCells], float2* s_vel) {
+    if (!box.lds) return;
+    for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0f;
+        if (G2P) s_vel[i] = cell_velocity(p, cur, dt, ddt, box.ox + i / box.h, box.oy + i % box.h);
+    }
+    __syncthreads();
+}
+
 // Add the LDS box into the global grid: one atomic per touched cell and channel.
 template <int NCH>
 __device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCells], float* const* g, int R) {
@@ -814,6 +940,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
     constexpr int NCH = P2G ? kAccChannels : 1;
     __shared__ float s_acc[NCH][kLdsCells];
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];
+    __shared__ uint32_t s_perm[kChunk];
+    __shared__ uint32_t s_scan[17];
     __shared__ int s_box[4 * kWaves];
     __shared__ float s_red[2 * kWaves];
 
@@ -857,18 +985,20 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
 
     // 1. the chunk's box; a substep's output base cell moves at most one cell
     //    (the G2P speed clamp bounds |v dt| by one cell), so the box is
-    //    widened by two for G2P and one for a scatter-only pass
-    const Box box = chunk_box(p, a, base, project_in, G2P ? 2 : 1, s_box);
+    //    widened by two for G2P and one for a scatter-only pass; then the
+    //    chunk's processing order, sorted by base cell
+    //    A chunk whose box exceeds the LDS budget (sparse, scattered
+    //    particles) runs slice by slice instead: each kBlock-particle slice
+    //    with a box of its own, so that one such chunk does not fall back to
+    //    global-memory G2P and P2G and set the launch's duration.
+    const int margin = G2P ? 2 : 1;
+    Box box = chunk_box(p, a, base, project_in, margin, s_box);
+    const bool sliced = !box.lds;
+    const uint32_t nvalid = base < p.n ? min((uint32_t)kChunk, p.n - base) : 0u;
+    chunk_order(p, a, base, project_in, box, nvalid, reinterpret_cast<uint32_t*>(s_acc[0]), s_perm, s_scan);
 
     // 2. clear the LDS accumulators; stage the updated grid velocity
-    if (box.lds) {
-        for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0f;
-            if (G2P) s_vel[i] = cell_velocity(p, cur, dt, ddt, box.ox + i / box.h, box.oy + i % box.h);
-        }
-        __syncthreads();
-    }
+    if (!sliced) stage_box<G2P, NCH>(p, box, cur, dt, ddt, s_acc, s_vel);
 
     float* gdst[NCH];
 #pragma unroll
@@ -879,8 +1009,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
     bool any_dirty = false;
 #pragma unroll 1
     for (int k = 0; k < kPer; ++k) {
-        const uint32_t j = base + k * kBlock + threadIdx.x;
-        const bool valid = j < p.n;
+        const uint32_t q = k * kBlock + threadIdx.x;
+        const bool valid = q < nvalid;
+        if (sliced) {
+            box = slice_box(p, a, base + q, valid, project_in, margin, s_box);
+            slice_order(p, a, base, project_in, box, k, nvalid, reinterpret_cast<uint32_t*>(s_acc[0]), s_perm, s_scan);
+            stage_box<G2P, NCH>(p, box, cur, dt, ddt, s_acc, s_vel);
+        }
+        const uint32_t j = base + (valid ? s_perm[q] : q);
         PState sp = neutral_state();
         if (valid) {
             PState s = neutral_state();
@@ -958,10 +1094,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
             const bool fma_ok = __ballot(valid && !(fabsf(t.mass) < 1e18f)) == 0ull;
             scatter_channel<kAccM>(t, valid, sm, scan_pred(sm), fma_ok, box, s_acc[0], gdst[0], R);
         }
+        if (sliced) {
+            flush_box<NCH>(box, s_acc, gdst, R);
+            __syncthreads();  // the next slice stages into the same LDS
+        }
     }
 
     // 6. flush the LDS box into the next accumulator set
-    flush_box<NCH>(box, s_acc, gdst, R);
+    if (!sliced) flush_box<NCH>(box, s_acc, gdst, R);
     // 7. clear this workgroup's slice of the set the substep after next uses
     if (G2P) {
         const uint32_t R2 = (uint32_t)R * (uint32_t)R;

@@ -419,7 +419,8 @@ int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint3
  * src/systems/gpu/shaders/particle_sort.wgsl:94-245).  This is the device
  * cell key the substep reorders particles by: key = emh_cell_key(floor(x),
  * floor(y)) -- 16x16-cell tiles numbered column by column with odd columns
- * reversed (serpentine), then 16*lx + ly inside the tile -- stable by input
+ * reversed (serpentine); inside a tile 16 rows of 16 cells, the rows in the
+ * tile column's vertical direction, x ascending in a row -- stable by input
  * index.  keys_out[i] is the key of input i; perm_out[j] is the input
  * index at sorted position j. */
 uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);

@@ -609,16 +609,19 @@ void orc_material_nacc(float mu, float kappa, float friction, float cohesion, fl
 // ---------------------------------------------------------------- sort
 // Cell key (no reference counterpart: the reference's CPU path has no sort).
 // Tiles of 16x16 cells are numbered column by column, the odd columns
-// top-down (serpentine); inside a tile the local cell index is 16*lx + ly.
+// top-down (serpentine); inside a tile the 16 rows follow the same vertical
+// direction as the tile column, x ascending within a row.
 uint32_t orc_cell_key(int32_t cx, int32_t cy, uint32_t R) {
     int32_t hi = (int32_t)R - 1;
     cx = std::min(std::max(cx, 0), hi);
     cy = std::min(std::max(cy, 0), hi);
     const uint32_t tiles = (R + 15u) / 16u;
     const uint32_t col = (uint32_t)cx / 16u, row = (uint32_t)cy / 16u;
-    const uint32_t row_in_col = (col % 2u == 0u) ? row : tiles - 1u - row;
+    const bool down = col % 2u == 1u;
+    const uint32_t row_in_col = down ? tiles - 1u - row : row;
     const uint32_t lx = (uint32_t)cx % 16u, ly = (uint32_t)cy % 16u;
-    return (col * tiles + row_in_col) * 256u + lx * 16u + ly;
+    const uint32_t local_row = down ? 15u - ly : ly;
+    return (col * tiles + row_in_col) * 256u + local_row * 16u + lx;
 }
 
 // std::stable_sort of input indices by the cell key of floor(x).

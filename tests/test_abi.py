@@ -141,8 +141,12 @@ def test_cell_key_matches_oracle(oracle):
 def test_cell_key_layout(r):
     """Properties that fix the key without restating it: a bijection from the
     R x R cells into [0, 256 * ceil(R/16)^2); 256 consecutive keys per 16x16
-    tile; consecutive tiles share an edge (serpentine), so a run of keys is a
-    compact patch of cells."""
+    tile; consecutive tiles share an edge (serpentine); inside the key order,
+    consecutive rows are vertically adjacent, so every run of 64 keys covers a
+    compact strip: a bounding box of at most 5 partial rows of 16 cells (80)
+    inside a tile column, or 4 rows across two tile columns at a turn (128).
+    A tile-local column-major order would reach 16 x 32 = 512 where a run
+    crosses from one tile into the next."""
     keys = np.array([[eh.cell_key(x, y, r) for y in range(r)] for x in range(r)])
     tpd = (r + 15) // 16
     assert len(np.unique(keys)) == r * r and keys.min() >= 0 and keys.max() < 256 * tpd * tpd
@@ -160,6 +164,17 @@ def test_cell_key_layout(r):
         assert abs(ax - bx) + abs(ay - by) == 1
     # clamped outside the grid
     assert eh.cell_key(-3, 5, r) == eh.cell_key(0, 5, r) and eh.cell_key(r + 9, r, r) == eh.cell_key(r - 1, r - 1, r)
+    # compact runs: cells in key order, every window of 64 (full tiles only: a
+    # ragged last tile row / column skips the keys of cells outside the grid)
+    if r % 16:
+        return
+    xy = np.argwhere(np.ones((r, r), bool))
+    ordered = xy[np.argsort(keys[xy[:, 0], xy[:, 1]])]
+    worst = 0
+    for s in range(0, len(ordered) - 63):
+        wnd = ordered[s:s + 64]
+        worst = max(worst, int(np.ptp(wnd[:, 0]) + 1) * int(np.ptp(wnd[:, 1]) + 1))
+    assert worst <= 128, worst
 
 
 def _no_gpu():

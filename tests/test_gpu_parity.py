@@ -2,10 +2,12 @@
 FP64 oracle, per particle, after 1 step and after the scene's N steps.
 
 Criterion: the GPU is as accurate as the reference's own f32 arithmetic.  The
-f32 error of the reference algorithm is measured as the larger error against
-the FP64 oracle of two f32 oracle runs: particles in the scene's order, and
-particles in a seeded random order (the grid sums then add in another order,
-as the GPU's do).  For each field, the GPU's error against the FP64 oracle
+f32 error of the reference algorithm is measured as the largest error against
+the FP64 oracle of five f32 oracle runs: particles in the scene's order, and
+in four seeded random orders (the grid sums then add in other orders, as the
+GPU's do).  Some scenes are ill-conditioned at f32 roundoff: one NACC step
+spreads C by 7e-4 .. 1.4e-2 over particle orders alone, so two orders
+underestimate the reference's spread.  For each field, the GPU's error against the FP64 oracle
 must stay within RATIO x that f32 error (plus an absolute floor of a few f32
 ulp), and after one step the GPU must also be within the stated relative
 tolerance of the f32 oracle directly.  Errors are relative to
@@ -20,6 +22,7 @@ import scenes as S
 
 RATIO = 8.0
 FLOOR = 2e-6
+PERM_SEEDS = (11, 1, 2, 3)
 # One step, GPU vs f32 oracle.  Set from the measured spread of the f32 oracle
 # itself against the FP64 oracle on these scenes (the reference algorithm
 # amplifies f32 roundoff most in the Drucker-Prager / NACC log-strain terms).
@@ -31,7 +34,7 @@ CHAOTIC = {"quickstart_impact"}
 # sand_mui.rs:158-167) to nearly mu_dynamic as p_trial -> 0+ (sand_mui.rs:
 # 176-200), so free-surface particles at p_trial ~ 0 flip on summation order
 # alone.  The GPU may flip at most a few times as many particles as the f32
-# oracle does in either particle order, plus 2 % of the particles.
+# oracle does in any of its particle orders, plus 2 % of the particles.
 FLIP_FIELDS = {"mui_column": ("q", 1e-3)}
 
 
@@ -69,27 +72,29 @@ def test_scene_parity(name, oracle, capsys):
     steps = [1, scene.steps]
     g, gsim = _run(scene, "gpu", steps)
     o32, osim = _run(scene, "oracle32", steps)
-    o32p, _ = _run(scene, "oracle32", steps, perm_seed=11)
+    o32p = [_run(scene, "oracle32", steps, perm_seed=seed)[0] for seed in PERM_SEEDS]
     o64, _ = _run(scene, "oracle64", steps)
     for k, s in enumerate(steps):
         e_g32 = S.rel_errors(g[k], o32[k])
         e_g64 = S.rel_errors(g[k], o64[k])
         e_a = S.rel_errors(o32[k], o64[k])
-        e_b = S.rel_errors(o32p[k], o64[k])
+        e_p = [S.rel_errors(x[k], o64[k]) for x in o32p]
+        e_b = {f: max(e[f] for e in e_p) for f in e_a}
         e_3264 = {f: max(e_a[f], e_b[f]) for f in e_a}
         with capsys.disabled():
             print(f"\n[{name} after {s} steps] n={len(g[k]['x'])}")
             for f in ("x", "v", "C", "F", "Jp", "q", "lvs", "density"):
                 print(f"  {f:8s} gpu-vs-o32 {e_g32[f]:.3e}  gpu-vs-o64 {e_g64[f]:.3e}  o32-vs-o64 {e_a[f]:.3e}"
-                      f"  o32(permuted)-vs-o64 {e_b[f]:.3e}")
+                      f"  o32(permuted, max of {len(PERM_SEEDS)})-vs-o64 {e_b[f]:.3e}")
         flip = FLIP_FIELDS.get(name)
         if flip is not None:
             f, tol = flip
             n = len(g[k][f])
-            fg, fa, fb = (_flips(x[f], o64[k][f], tol) for x in (g[k], o32[k], o32p[k]))
+            fg = _flips(g[k][f], o64[k][f], tol)
+            fo = max(_flips(x[f], o64[k][f], tol) for x in [o32[k]] + [y[k] for y in o32p])
             with capsys.disabled():
-                print(f"  {f} flips beyond {tol}: gpu {fg}, o32 {fa}, o32(permuted) {fb} of {n}")
-            assert fg <= 4 * max(fa, fb) + 0.02 * n, (name, s, f, fg, fa, fb)
+                print(f"  {f} flips beyond {tol}: gpu {fg}, f32 oracle (max over orders) {fo} of {n}")
+            assert fg <= 4 * fo + 0.02 * n, (name, s, f, fg, fo)
         if k == 0:
             # absolute bound, widened where the reference algorithm itself
             # spreads f32 roundoff further (|gpu - o32| <= |gpu - o64| + |o32 - o64|)
