@@ -753,6 +753,10 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
 // that the lanes of a wave hold long runs of one cell however far the stored
 // order has drifted since the last physical reorder.  Lane t's k-th particle
 // is slot s_perm[k * kBlock + t] of the chunk.  s_cnt: kLdsCells scratch.
+// A chunk whose stored order still has few runs (right after a physical
+// reorder, or where particles barely move) keeps it: the sort would not
+// shorten its runs and costs about 5% of the substep.
+constexpr uint32_t kSortRunsPerWave = 8;
 __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
                                             uint32_t nvalid, uint32_t* s_cnt, uint32_t* s_perm, uint32_t* s_scan) {
     if (!box.lds) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
@@ -761,9 +765,12 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
         __syncthreads();
         return;
     }
+    __shared__ uint32_t s_runs;
     const int cells = box.w * box.h;
     for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
+    if (threadIdx.x == 0) s_runs = 0u;
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
     uint32_t cell[kPer], rank[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -775,9 +782,24 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
             if (project) project_x(x.x, x.y, p.cfg);
             cell[k] = (uint32_t)box.index(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)));
         }
+        // runs of the stored order: lanes whose cell differs from the lane before
+        const uint32_t prev = (uint32_t)__shfl_up((int)cell[k], 1);
+        const uint64_t starts = __ballot(valid && (lane == 0u || prev != cell[k]));
+        if (lane == 0u && starts != 0ull) atomicAdd(&s_runs, (uint32_t)__popcll(starts));
+    }
+    __syncthreads();
+    if (s_runs <= kSortRunsPerWave * (kChunk / 64)) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = k * kBlock + threadIdx.x;
+        __syncthreads();
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
         // one returning LDS atomic per particle: same-cell lanes serialise in
         // the LDS unit, with no per-cell loop of dependent round trips
-        rank[k] = valid ? atomicAdd(&s_cnt[cell[k]], 1u) : 0u;
+        const uint32_t q = k * kBlock + threadIdx.x;
+        rank[k] = q < nvalid ? atomicAdd(&s_cnt[cell[k]], 1u) : 0u;
     }
     __syncthreads();
     constexpr int kCellsPerThread = kLdsCells / kBlock;

@@ -3,11 +3,12 @@ FP64 oracle, per particle, after 1 step and after the scene's N steps.
 
 Criterion: the GPU is as accurate as the reference's own f32 arithmetic.  The
 f32 error of the reference algorithm is measured as the largest error against
-the FP64 oracle of five f32 oracle runs: particles in the scene's order, and
-in four seeded random orders (the grid sums then add in other orders, as the
-GPU's do).  Some scenes are ill-conditioned at f32 roundoff: one NACC step
-spreads C by 7e-4 .. 1.4e-2 over particle orders alone, so two orders
-underestimate the reference's spread.  For each field, the GPU's error against the FP64 oracle
+the FP64 oracle of thirteen f32 oracle runs: particles in the scene's order,
+and in twelve seeded random orders (the grid sums then add in other orders,
+as the GPU's atomics do).  Scenes with plasticity thresholds are
+ill-conditioned at f32 roundoff: one NACC step spreads C by 7e-4 .. 1.4e-2
+over particle orders alone, ten Von Mises steps spread v by 1e-5 .. 4e-5, so
+a few orders underestimate the reference's spread.  For each field, the GPU's error against the FP64 oracle
 must stay within RATIO x that f32 error (plus an absolute floor of a few f32
 ulp), and after one step the GPU must also be within the stated relative
 tolerance of the f32 oracle directly.  Errors are relative to
@@ -22,7 +23,7 @@ import scenes as S
 
 RATIO = 8.0
 FLOOR = 2e-6
-PERM_SEEDS = (11, 1, 2, 3)
+PERM_SEEDS = tuple(range(1, 13))
 # One step, GPU vs f32 oracle.  Set from the measured spread of the f32 oracle
 # itself against the FP64 oracle on these scenes (the reference algorithm
 # amplifies f32 roundoff most in the Drucker-Prager / NACC log-strain terms).
