@@ -20,6 +20,7 @@
 #include "interop_kernels.hpp"
 #include "kernels.hpp"
 #include "spawn_kernels.hpp"
+#include "surrogate_kernels.hpp"
 
 using namespace emh;
 
@@ -357,6 +358,9 @@ struct emh_sim {
     uint32_t predicted_substeps = 1;
     uint32_t substeps_since_reorder = 0;
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
+    // learned NACC return mapping: 0 plain, 1 harvest trial states, 2 infer
+    uint32_t sg_mode = 0;
+    SgDev sg_host{};  // the descriptor Ctrl::sg holds
 
     // Live timing: three events per step on the library stream -- before
     // the step's set-up launches (k_begin_step and, on reorder steps, the
@@ -689,6 +693,7 @@ struct emh_sim {
             r.min_mdt = kInfBits;
             r.dirty = 0u;
         }
+        ctrl.sg = sg_host;
         seq = 0;
         HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
         HIP_TRY(launch_zero_all(pipe, stream));
@@ -727,6 +732,7 @@ struct emh_sim {
             r.min_mdt = kInfBits;
             r.dirty = 0u;
         }
+        ctrl.sg = sg_host;
         seq = 0;
         HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
         HIP_TRY(launch_zero_all(pipe, stream));
@@ -864,7 +870,7 @@ struct emh_sim {
             uint32_t more = (running(st) ? 1u : 0u) + st.pending * std::max<uint32_t>(predicted_substeps, 1u);
             more = std::min<uint32_t>(std::max<uint32_t>(more, 1u), 64u);
             if (extra + more > limit) return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
-            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream));
+            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode));
             extra += more;
         }
         // the last step: finished but not yet recorded if no launch followed it
@@ -907,6 +913,9 @@ struct emh_sim {
             int rc = upload();
             if (rc) return rc;
         }
+        if (sg_mode != 0 && pipe.n > 0 && (pipe.cfg.only_model != kModelNacc || pipe.cfg.density_pass))
+            return fail(EMH_ERR_UNSUPPORTED, "the surrogate harvest / inference modes need a scene whose particles "
+                                             "are all NACC");
         if (pipe.n == 0) {  // no particles: the loop still advances time
             for (uint32_t s = 0; s < n_steps; ++s) {
                 float remaining = cfg.dt;
@@ -944,7 +953,7 @@ struct emh_sim {
             if (reorder) substeps_since_reorder = 0;
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream));
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode));
             substeps_since_reorder += slots;
             if (profiling || (s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
                 int rc = drain();
@@ -1726,6 +1735,530 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out
     (void)hipFree(d_tc);
     (void)hipFree(d_ts);
     (void)hipStreamDestroy(st);
+    return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ learned NACC surrogate
+// Device state of one surrogate network: FP32 master weights with Adam
+// moments, the packed BF16 operand tiles, the normalisation block, harvested
+// trial-state records and the training set.
+struct emh_surrogate {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* d_w = nullptr;     // kSgParams master weights
+    float* d_m = nullptr;     // Adam first moments
+    float* d_v = nullptr;     // Adam second moments
+    float* d_grad = nullptr;  // gradient accumulator (cleared by each Adam step)
+    float* d_norm = nullptr;  // kSgNorm
+    uint4* d_pk = nullptr;    // kPkTiles * 64 packed operand lanes
+    unsigned long long* d_counts = nullptr;  // [0..2] return-mapping counts, [3] harvest offers
+    float* d_rec = nullptr;   // harvested records, 12 floats each
+    uint64_t rec_cap = 0;
+    float* d_xs = nullptr;    // training inputs [n][8]
+    float* d_ts = nullptr;    // training targets [n][4]
+    uint64_t n_set = 0, set_cap = 0;
+    float* d_scratch = nullptr;  // per-slot GEMM operands of one batch
+    uint32_t scratch_nb = 0;
+    SgTrainBufs bufs{};
+    double* d_sums = nullptr;  // 18 normalisation sums
+    float* d_loss = nullptr;
+    uint64_t adam_t = 0;
+    float norm[kSgNorm] = {};
+
+    ~emh_surrogate() { release(); }
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* q : {(void*)d_w, (void*)d_m, (void*)d_v, (void*)d_grad, (void*)d_norm, (void*)d_pk,
+                        (void*)d_counts, (void*)d_rec, (void*)d_xs, (void*)d_ts, (void*)d_scratch, (void*)d_sums,
+                        (void*)d_loss})
+            if (q) (void)hipFree(q);
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    SgDev dev(bool harvest) const {
+        SgDev d{};
+        d.pk = d_pk;
+        d.prm = d_w;
+        d.norm = d_norm;
+        d.counts = d_counts;
+        if (harvest) {
+            d.harvest = d_rec;
+            d.harvest_count = d_counts + 3;
+            d.harvest_cap = rec_cap;
+        }
+        return d;
+    }
+    int upload_norm() {
+        HIP_TRY(hipMemcpyAsync(d_norm, norm, sizeof(norm), hipMemcpyHostToDevice, stream));
+        return EMH_OK;
+    }
+    int pack() {
+        HIP_TRY(sg_launch_pack(d_w, d_pk, stream));
+        return EMH_OK;
+    }
+    // Normalisation from the sums of x (6), residual y - x[0:3] (3), and their squares.
+    void set_norm(const double* sums, uint64_t n) {
+        auto sd_of = [](double mean, double sq, double cnt) {
+            const double var = std::max(sq / cnt - mean * mean, 0.0);
+            const double floor_ = std::max(1.0e-3 * std::fabs(mean), 1.0e-6);
+            return (float)std::max(std::sqrt(var), floor_);
+        };
+        const double cnt = (double)std::max<uint64_t>(n, 1);
+        for (int k = 0; k < kSgIn; ++k) {
+            const double mean = sums[k] / cnt;
+            norm[kSgInMu + k] = (float)mean;
+            norm[kSgInSd + k] = sd_of(mean, sums[9 + k], cnt);
+        }
+        for (int k = 0; k < kSgOut; ++k) {
+            const double mean = sums[6 + k] / cnt;
+            norm[kSgOutMu + k] = (float)mean;
+            norm[kSgOutSd + k] = sd_of(mean, sums[15 + k], cnt);
+        }
+        norm[kSgZMax] = 8.0f;
+    }
+    int ensure_set(uint64_t n) {
+        if (n <= set_cap) return EMH_OK;
+        if (d_xs) (void)hipFree(d_xs);
+        if (d_ts) (void)hipFree(d_ts);
+        d_xs = dalloc<float>(n * 8);
+        d_ts = dalloc<float>(n * 4);
+        set_cap = (d_xs && d_ts) ? n : 0;
+        if (!set_cap) return fail(EMH_ERR_OUT_OF_MEMORY, "training-set allocation failed");
+        return EMH_OK;
+    }
+    int ensure_scratch(uint32_t nb) {
+        if (nb <= scratch_nb) return EMH_OK;
+        if (d_scratch) (void)hipFree(d_scratch);
+        const size_t per = 8 + 3 * kSgHid + 3 * kSgHid + 4;
+        d_scratch = dalloc<float>(per * nb);
+        if (!d_scratch) {
+            scratch_nb = 0;
+            return fail(EMH_ERR_OUT_OF_MEMORY, "training scratch allocation failed");
+        }
+        scratch_nb = nb;
+        float* q = d_scratch;
+        bufs.hin = q;
+        q += 8 * (size_t)nb;
+        float** hs[6] = {&bufs.h1, &bufs.h2, &bufs.h3, &bufs.dz1, &bufs.dz2, &bufs.dz3};
+        for (float** h : hs) {
+            *h = q;
+            q += (size_t)kSgHid * nb;
+        }
+        bufs.dout = q;
+        bufs.loss = d_loss;
+        return EMH_OK;
+    }
+    SgBatch batch(const float* xs, const float* ts, uint64_t n_set_, uint32_t nb, uint32_t n_valid) const {
+        SgBatch b{};
+        b.xs = xs;
+        b.ts = ts;
+        b.n_set = n_set_;
+        b.nb = nb;
+        b.n_valid = n_valid;
+        b.inv_n = 1.0f / (float)std::max<uint32_t>(n_valid, 1u);
+        b.pk = d_pk;
+        b.prm = d_w;
+        b.norm = d_norm;
+        return b;
+    }
+};
+
+namespace {
+uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+// splitmix64
+uint64_t mix64(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+uint32_t round_up_256(uint64_t n) { return (uint32_t)((n + 255u) / 256u * 256u); }
+}  // namespace
+
+extern "C" {
+
+int emh_surrogate_create(uint64_t seed, emh_surrogate** out) {
+    if (!out) return fail(EMH_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
+    auto* sg = new emh_surrogate();
+    auto bail = [&](int rc) {
+        delete sg;
+        return rc;
+    };
+    if (hipGetDevice(&sg->device) != hipSuccess || hipStreamCreateWithFlags(&sg->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EMH_ERR_DEVICE, "stream creation failed"));
+    sg->d_w = dalloc<float>(kSgParams);
+    sg->d_m = dalloc<float>(kSgParams);
+    sg->d_v = dalloc<float>(kSgParams);
+    sg->d_grad = dalloc<float>(kSgParams);
+    sg->d_norm = dalloc<float>(kSgNorm);
+    sg->d_pk = dalloc<uint4>((size_t)kPkTiles * 64);
+    sg->d_counts = dalloc<unsigned long long>(4);
+    sg->d_sums = dalloc<double>(18);
+    sg->d_loss = dalloc<float>(1);
+    if (!sg->d_w || !sg->d_m || !sg->d_v || !sg->d_grad || !sg->d_norm || !sg->d_pk || !sg->d_counts || !sg->d_sums ||
+        !sg->d_loss)
+        return bail(fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed"));
+    // uniform(-1, 1) * sqrt(6 / (fan_in + fan_out)) weights, zero biases
+    std::vector<float> w(kSgParams, 0.0f);
+    uint64_t st = seed;
+    auto fill = [&](int off, int n_out, int n_in) {
+        const float lim = std::sqrt(6.0f / (float)(n_in + n_out));
+        for (int i = 0; i < n_out * n_in; ++i) {
+            const double u = (double)(mix64(st) >> 11) * (1.0 / 9007199254740992.0);
+            w[off + i] = (float)((2.0 * u - 1.0) * lim);
+        }
+    };
+    fill(kSgW1, kSgHid, kSgIn);
+    fill(kSgW2, kSgHid, kSgHid);
+    fill(kSgW3, kSgHid, kSgHid);
+    fill(kSgW4, kSgOut, kSgHid);
+    for (int k = 0; k < kSgIn; ++k) sg->norm[kSgInSd + k] = 1.0f;
+    for (int k = 0; k < kSgOut; ++k) sg->norm[kSgOutSd + k] = 1.0f;
+    sg->norm[kSgZMax] = 8.0f;
+    const hipStream_t s = sg->stream;
+    if (hipMemcpyAsync(sg->d_w, w.data(), sizeof(float) * kSgParams, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(sg->d_m, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
+        hipMemsetAsync(sg->d_v, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
+        hipMemsetAsync(sg->d_grad, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
+        hipMemsetAsync(sg->d_counts, 0, sizeof(unsigned long long) * 4, s) != hipSuccess)
+        return bail(fail(EMH_ERR_DEVICE, "surrogate initialisation failed"));
+    int rc = sg->upload_norm();
+    if (!rc) rc = sg->pack();
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(EMH_ERR_DEVICE, "surrogate initialisation failed");
+    if (rc) return bail(rc);
+    *out = sg;
+    return EMH_OK;
+}
+
+int emh_surrogate_destroy(emh_surrogate* sg) {
+    delete sg;
+    return EMH_OK;
+}
+
+int emh_surrogate_get_params(emh_surrogate* sg, float* params, float* norm) {
+    if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
+    if (params) HIP_TRY(hipMemcpyAsync(params, sg->d_w, sizeof(float) * kSgParams, hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    if (norm) std::memcpy(norm, sg->norm, sizeof(sg->norm));
+    return EMH_OK;
+}
+
+int emh_surrogate_set_params(emh_surrogate* sg, const float* params, const float* norm) {
+    if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
+    if (norm) {
+        for (int k = 0; k < kSgIn; ++k)
+            if (!(norm[kSgInSd + k] > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "input scales must be positive");
+        for (int k = 0; k < kSgOut; ++k)
+            if (!(norm[kSgOutSd + k] > 0.0f)) return fail(EMH_ERR_INVALID_ARG, "output scales must be positive");
+        std::memcpy(sg->norm, norm, sizeof(sg->norm));
+        int rc = sg->upload_norm();
+        if (rc) return rc;
+    }
+    if (params) {
+        HIP_TRY(hipMemcpyAsync(sg->d_w, params, sizeof(float) * kSgParams, hipMemcpyHostToDevice, sg->stream));
+        HIP_TRY(hipMemsetAsync(sg->d_m, 0, sizeof(float) * kSgParams, sg->stream));
+        HIP_TRY(hipMemsetAsync(sg->d_v, 0, sizeof(float) * kSgParams, sg->stream));
+        sg->adam_t = 0;
+        int rc = sg->pack();
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    return EMH_OK;
+}
+
+int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    if (mode > 2) return fail(EMH_ERR_INVALID_ARG, "mode must be 0 (plain), 1 (harvest) or 2 (surrogate)");
+    if (mode != 0 && !sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
+    if (mode == 1 && harvest_capacity == 0) return fail(EMH_ERR_INVALID_ARG, "harvest capacity must be > 0");
+    if (mode != 0 && sg->device != s->device) return fail(EMH_ERR_INVALID_ARG, "surrogate and sim are on different devices");
+    if (mode == 1) {
+        if (harvest_capacity > sg->rec_cap) {
+            if (sg->d_rec) (void)hipFree(sg->d_rec);
+            sg->d_rec = dalloc<float>(harvest_capacity * 12);
+            sg->rec_cap = sg->d_rec ? harvest_capacity : 0;
+            if (!sg->d_rec) return fail(EMH_ERR_OUT_OF_MEMORY, "harvest buffer allocation failed");
+        }
+        HIP_TRY(hipMemsetAsync(sg->d_counts + 3, 0, sizeof(unsigned long long), sg->stream));
+        HIP_TRY(hipStreamSynchronize(sg->stream));
+    }
+    // drain queued work before the descriptor changes under it
+    if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
+    s->sg_host = mode == 0 ? SgDev{} : sg->dev(mode == 1);
+    if (mode == 1) s->sg_host.harvest_cap = harvest_capacity;
+    s->sg_mode = mode;
+    if (s->d_ctrl) {
+        HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(s->d_ctrl) + offsetof(Ctrl, sg), &s->sg_host, sizeof(SgDev),
+                               hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    return EMH_OK;
+}
+
+int emh_surrogate_harvest_count(emh_surrogate* sg, uint64_t* offered, uint64_t* stored) {
+    if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, sg->d_counts + 3, sizeof(c), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    if (offered) *offered = c;
+    if (stored) *stored = std::min<uint64_t>(c, sg->rec_cap);
+    return EMH_OK;
+}
+
+int emh_surrogate_get_records(emh_surrogate* sg, uint64_t first, uint64_t n, float* out) {
+    if (!sg || (n && !out)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    uint64_t stored = 0;
+    int rc = emh_surrogate_harvest_count(sg, nullptr, &stored);
+    if (rc) return rc;
+    if (first + n > stored) return fail(EMH_ERR_INVALID_ARG, "range beyond the harvested records");
+    if (n == 0) return EMH_OK;
+    HIP_TRY(hipMemcpyAsync(out, sg->d_rec + first * 12, sizeof(float) * 12 * n, hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    return EMH_OK;
+}
+
+int emh_surrogate_build_training_set(emh_surrogate* sg, uint64_t* n_samples) {
+    if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
+    uint64_t offered = 0, n_rec = 0;
+    int rc = emh_surrogate_harvest_count(sg, &offered, &n_rec);
+    if (rc) return rc;
+    if (n_rec == 0) return fail(EMH_ERR_STATE, "no harvested trial states");
+    rc = sg->ensure_set(n_rec);
+    if (rc) return rc;
+    unsigned long long* d_n = sg->d_counts;  // reuse slot 0 as the output counter, reset below
+    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long) * 3, sg->stream));
+    HIP_TRY(hipMemsetAsync(sg->d_sums, 0, sizeof(double) * 18, sg->stream));
+    HIP_TRY(sg_launch_prepare(sg->d_rec, n_rec, sg->d_xs, sg->d_ts, d_n, sg->d_sums, sg->stream));
+    unsigned long long n = 0;
+    double sums[18];
+    HIP_TRY(hipMemcpyAsync(&n, d_n, sizeof(n), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipMemcpyAsync(sums, sg->d_sums, sizeof(sums), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long) * 3, sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    sg->n_set = n;
+    if (n == 0) return fail(EMH_ERR_STATE, "no harvested trial state fails the yield check");
+    sg->set_norm(sums, n);
+    rc = sg->upload_norm();
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    if (n_samples) *n_samples = n;
+    return EMH_OK;
+}
+
+int emh_surrogate_set_training_set(emh_surrogate* sg, const float* x, const float* y, uint64_t n) {
+    if (!sg || (n && (!x || !y))) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return fail(EMH_ERR_INVALID_ARG, "empty training set");
+    int rc = sg->ensure_set(n);
+    if (rc) return rc;
+    std::vector<float> xs(n * 8, 0.0f), ts(n * 4, 0.0f);
+    double sums[18] = {};
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < kSgIn; ++k) {
+            const double v = x[i * kSgIn + k];
+            xs[i * 8 + k] = x[i * kSgIn + k];
+            sums[k] += v;
+            sums[9 + k] += v * v;
+        }
+        for (int k = 0; k < kSgOut; ++k) {
+            ts[i * 4 + k] = y[i * kSgOut + k];
+            const double d = (double)y[i * kSgOut + k] - (double)x[i * kSgIn + k];
+            sums[6 + k] += d;
+            sums[15 + k] += d * d;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(sg->d_xs, xs.data(), sizeof(float) * xs.size(), hipMemcpyHostToDevice, sg->stream));
+    HIP_TRY(hipMemcpyAsync(sg->d_ts, ts.data(), sizeof(float) * ts.size(), hipMemcpyHostToDevice, sg->stream));
+    sg->n_set = n;
+    sg->set_norm(sums, n);
+    rc = sg->upload_norm();
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    return EMH_OK;
+}
+
+int emh_surrogate_get_training_set(emh_surrogate* sg, uint64_t first, uint64_t n, float* x, float* y) {
+    if (!sg || (n && (!x || !y))) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (first + n > sg->n_set) return fail(EMH_ERR_INVALID_ARG, "range beyond the training set");
+    if (n == 0) return EMH_OK;
+    std::vector<float> xs(n * 8), ts(n * 4);
+    HIP_TRY(hipMemcpyAsync(xs.data(), sg->d_xs + first * 8, sizeof(float) * xs.size(), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipMemcpyAsync(ts.data(), sg->d_ts + first * 4, sizeof(float) * ts.size(), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int k = 0; k < kSgIn; ++k) x[i * kSgIn + k] = xs[i * 8 + k];
+        for (int k = 0; k < kSgOut; ++k) y[i * kSgOut + k] = ts[i * 4 + k];
+    }
+    return EMH_OK;
+}
+
+int emh_surrogate_train(emh_surrogate* sg, const emh_surrogate_train_options* o, float* epoch_loss,
+                        uint32_t n_epoch_loss) {
+    if (!sg || !o) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (sg->n_set == 0) return fail(EMH_ERR_STATE, "no training set");
+    if (o->epochs == 0 || o->batch == 0) return fail(EMH_ERR_INVALID_ARG, "epochs and batch must be > 0");
+    if (!(o->lr > 0.0f) || !(o->lr_final >= 0.0f) || !(o->beta1 >= 0.0f && o->beta1 < 1.0f) ||
+        !(o->beta2 >= 0.0f && o->beta2 < 1.0f) || !(o->eps > 0.0f))
+        return fail(EMH_ERR_INVALID_ARG, "invalid Adam hyper-parameters");
+    const uint32_t nb = round_up_256(o->batch);
+    if (nb > sg->n_set) return fail(EMH_ERR_INVALID_ARG, "batch larger than the training set");
+    int rc = sg->ensure_scratch(nb);
+    if (rc) return rc;
+    const uint64_t per_epoch = sg->n_set / nb;
+    uint64_t total = (uint64_t)o->epochs * per_epoch;
+    if (o->max_steps && o->max_steps < total) total = o->max_steps;
+    uint64_t rng = o->seed;
+    SgBatch b = sg->batch(sg->d_xs, sg->d_ts, sg->n_set, nb, nb);
+    uint64_t step = 0;
+    for (uint32_t e = 0; e < o->epochs && step < total; ++e) {
+        // this epoch's visiting order: q -> (a q + c) mod n, a coprime to n
+        uint64_t a = 1 + mix64(rng) % (sg->n_set - 1 > 0 ? sg->n_set - 1 : 1);
+        while (gcd_u64(a, sg->n_set) != 1) a = a % sg->n_set + 1;
+        b.perm_a = a;
+        b.perm_b = mix64(rng) % sg->n_set;
+        HIP_TRY(hipMemsetAsync(sg->d_loss, 0, sizeof(float), sg->stream));
+        uint64_t done = 0;
+        for (uint64_t k = 0; k < per_epoch && step < total; ++k, ++step, ++done) {
+            b.first = k * nb;
+            const double frac = total > 1 ? (double)step / (double)(total - 1) : 1.0;
+            SgAdam h{};
+            h.lr = (float)(o->lr_final + 0.5 * (o->lr - o->lr_final) * (1.0 + std::cos(3.141592653589793 * frac)));
+            h.beta1 = o->beta1;
+            h.beta2 = o->beta2;
+            h.eps = o->eps;
+            ++sg->adam_t;
+            h.c1 = (float)(1.0 / (1.0 - std::pow((double)o->beta1, (double)sg->adam_t)));
+            h.c2 = (float)(1.0 / (1.0 - std::pow((double)o->beta2, (double)sg->adam_t)));
+            HIP_TRY(sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, &h, sg->stream));
+        }
+        float l = 0.0f;
+        HIP_TRY(hipMemcpyAsync(&l, sg->d_loss, sizeof(float), hipMemcpyDeviceToHost, sg->stream));
+        HIP_TRY(hipStreamSynchronize(sg->stream));
+        if (epoch_loss && e < n_epoch_loss) epoch_loss[e] = done ? l / (float)(done * nb) : 0.0f;
+    }
+    return EMH_OK;
+}
+
+int emh_surrogate_gradient(emh_surrogate* sg, const float* x, const float* y, uint32_t n, float* grad, float* loss) {
+    if (!sg || !x || !y || n == 0) return fail(EMH_ERR_INVALID_ARG, "null or empty argument");
+    const uint32_t nb = round_up_256(n);
+    int rc = sg->ensure_scratch(nb);
+    if (rc) return rc;
+    std::vector<float> xs((size_t)n * 8, 0.0f), ts((size_t)n * 4, 0.0f);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < kSgIn; ++k) xs[(size_t)i * 8 + k] = x[(size_t)i * kSgIn + k];
+        for (int k = 0; k < kSgOut; ++k) ts[(size_t)i * 4 + k] = y[(size_t)i * kSgOut + k];
+    }
+    float* d_x = dalloc<float>(xs.size());
+    float* d_t = dalloc<float>(ts.size());
+    rc = EMH_OK;
+    if (!d_x || !d_t) rc = fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+    hipError_t e = hipSuccess;
+    float l = 0.0f;
+    if (!rc) {
+        const SgBatch b = sg->batch(d_x, d_t, n, nb, n);
+        e = hipMemcpyAsync(d_x, xs.data(), sizeof(float) * xs.size(), hipMemcpyHostToDevice, sg->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_t, ts.data(), sizeof(float) * ts.size(), hipMemcpyHostToDevice, sg->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(sg->d_grad, 0, sizeof(float) * kSgParams, sg->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(sg->d_loss, 0, sizeof(float), sg->stream);
+        if (e == hipSuccess) e = sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, nullptr, sg->stream);
+        if (e == hipSuccess && grad)
+            e = hipMemcpyAsync(grad, sg->d_grad, sizeof(float) * kSgParams, hipMemcpyDeviceToHost, sg->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&l, sg->d_loss, sizeof(float), hipMemcpyDeviceToHost, sg->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(sg->d_grad, 0, sizeof(float) * kSgParams, sg->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(sg->stream);
+        if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "gradient: %s", hipGetErrorString(e));
+    }
+    if (d_x) (void)hipFree(d_x);
+    if (d_t) (void)hipFree(d_t);
+    if (!rc && loss) *loss = l / (float)n;
+    return rc;
+}
+
+int emh_surrogate_infer(emh_surrogate* sg, const float* x, uint64_t n, float* y) {
+    if (!sg || (n && (!x || !y))) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return EMH_OK;
+    float* d_x = dalloc<float>(n * kSgIn);
+    float* d_y = dalloc<float>(n * kSgOut);
+    int rc = EMH_OK;
+    if (!d_x || !d_y) rc = fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(d_x, x, sizeof(float) * n * kSgIn, hipMemcpyHostToDevice, sg->stream);
+        if (e == hipSuccess) e = sg_launch_infer(d_x, n, d_y, sg->dev(false), sg->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(y, d_y, sizeof(float) * n * kSgOut, hipMemcpyDeviceToHost, sg->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(sg->stream);
+        if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "infer: %s", hipGetErrorString(e));
+    }
+    if (d_x) (void)hipFree(d_x);
+    if (d_y) (void)hipFree(d_y);
+    return rc;
+}
+
+int emh_surrogate_get_counts(emh_surrogate* sg, emh_surrogate_counts* out, int reset) {
+    if (!sg || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    unsigned long long c[3];
+    HIP_TRY(hipMemcpyAsync(c, sg->d_counts, sizeof(c), hipMemcpyDeviceToHost, sg->stream));
+    if (reset) HIP_TRY(hipMemsetAsync(sg->d_counts, 0, sizeof(c), sg->stream));
+    HIP_TRY(hipStreamSynchronize(sg->stream));
+    out->yielding = c[0];
+    out->surrogate = c[1];
+    out->fallback = c[2];
+    return EMH_OK;
+}
+
+int emh_nacc_return_batch(emh_surrogate* sg, const emh_material_params* p, const emh_material_ext* ext,
+                          const float* f_trial, const float* alpha, uint64_t n, float* f_out, float* alpha_out,
+                          uint32_t reps, float* ms_per_rep) {
+    if (!p || (n && (!f_trial || !alpha || !f_out || !alpha_out))) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (p->model != EMH_MODEL_NACC) return fail(EMH_ERR_INVALID_ARG, "material is not NACC");
+    emh_material_ext e{};
+    if (ext)
+        e = *ext;
+    else
+        ext_defaults(&e);
+    const DevMat m = make_dev_mat(*p, e);
+    hipStream_t st = nullptr;
+    bool own = false;
+    if (sg) {
+        st = sg->stream;
+    } else {
+        if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own = true;
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = EMH_OK;
+    hipError_t err = hipEventCreate(&ev[0]);
+    if (err == hipSuccess) err = hipEventCreate(&ev[1]);
+    const uint32_t r = std::max<uint32_t>(reps, 1u);
+    if (err == hipSuccess) err = hipEventRecord(ev[0], st);
+    for (uint32_t k = 0; k < r && err == hipSuccess; ++k) {
+        if (sg)
+            err = sg_launch_return_surrogate(m, reinterpret_cast<const float4*>(f_trial), alpha, n,
+                                             reinterpret_cast<float4*>(f_out), alpha_out, sg->dev(false), st);
+        else
+            err = sg_launch_return_plain(m, reinterpret_cast<const float4*>(f_trial), alpha, n,
+                                         reinterpret_cast<float4*>(f_out), alpha_out, st);
+    }
+    if (err == hipSuccess) err = hipEventRecord(ev[1], st);
+    if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (err != hipSuccess) rc = fail(EMH_ERR_DEVICE, "return batch: %s", hipGetErrorString(err));
+    if (ms_per_rep) *ms_per_rep = ms / (float)r;
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (own) (void)hipStreamDestroy(st);
     return rc;
 }
 

@@ -572,48 +572,63 @@ EMH_DEV bool dp_project(const DevMat& m, float s1, float s2, float lvs, float q,
 }
 
 // NaccMaterial::project  nacc.rs:129-223
-EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, float& ao) {
+// NaccMaterial::project (nacc.rs:129-223) in singular-value space, split at
+// the yield-function check: nacc_trial evaluates the trial state and the
+// check, nacc_return the projection of a state that fails it.  nacc_project
+// chains them; the learned surrogate replaces nacc_return.
+struct NaccTrial {
+    float q1, q2, qt, p0, jtr, st1, st2, ptr, y0, y1;
+    int yield;  // 0 inside the yield surface (F unchanged), 1 Case A, 2 Case B, 3 Case C
+};
+EMH_DEV NaccTrial nacc_trial(const DevMat& m, float2 sg, float alpha) {
     const float xi = m.na_xi, beta = m.na_beta, mm = m.na_m, kappa = m.na_kappa;
+    NaccTrial t;
+    t.q1 = sg.x * sg.x;
+    t.q2 = sg.y * sg.y;
+    t.qt = t.q1 + t.q2;
+    t.p0 = kappa * (1e-5f + sinhf(xi * rmax(-alpha, 0.0f)));
+    t.jtr = rmax(sg.x * sg.y, 1.0e-6f);
+    float mu_inv = m.mu * (1.0f / t.jtr);
+    t.st1 = mu_inv * (t.q1 - t.qt * 0.5f);
+    t.st2 = mu_inv * (t.q2 - t.qt * 0.5f);
+    float psi = (kappa * 0.5f) * (t.jtr - 1.0f / t.jtr);
+    t.ptr = -psi * t.jtr;
+    t.y0 = 0.0f;
+    t.y1 = 0.0f;
+    if (t.ptr > t.p0) {
+        t.yield = 1;
+        return t;
+    }
+    if (t.ptr < -beta * t.p0) {
+        t.yield = 2;
+        return t;
+    }
+    t.y0 = (1.0f + 2.0f * beta) * 2.0f;
+    t.y1 = mm * mm * (t.ptr + beta * t.p0) * (t.ptr - t.p0);
+    float sn2 = t.st1 * t.st1 + t.st2 * t.st2;
+    float y = t.y0 * sn2 + t.y1;
+    t.yield = y < 1.0e-4f ? 0 : 3;
+    return t;
+}
+// Projected singular values (s1, s2) and hardening state of a trial state
+// with t.yield != 0.
+EMH_DEV void nacc_return(const DevMat& m, const NaccTrial& t, float alpha, float& s1, float& s2, float& ao) {
+    const float beta = m.na_beta, mm = m.na_m, kappa = m.na_kappa;
     const bool hard = (m.flags & kMatNaccHardening) != 0u;
-    M2 u, vt;
-    float2 sg;
-    svd2(f, u, sg, vt);
-    float q1 = sg.x * sg.x, q2 = sg.y * sg.y;
-    float qt = q1 + q2;
-    float p0 = kappa * (1e-5f + sinhf(xi * rmax(-alpha, 0.0f)));
-    float jtr = rmax(sg.x * sg.y, 1.0e-6f);
-    float mu_inv = m.mu * (1.0f / jtr);
-    float st1 = mu_inv * (q1 - qt * 0.5f), st2 = mu_inv * (q2 - qt * 0.5f);
-    float psi = (kappa * 0.5f) * (jtr - 1.0f / jtr);
-    float ptr = -psi * jtr;
-    if (ptr > p0) {
-        float jn = sqrtf(rmax(-2.0f * p0 / kappa + 1.0f, 1.0e-8f));
+    const float p0 = t.p0, jtr = t.jtr, ptr = t.ptr;
+    if (t.yield == 1 || t.yield == 2) {
+        float jn = t.yield == 1 ? sqrtf(rmax(-2.0f * p0 / kappa + 1.0f, 1.0e-8f))
+                                : sqrtf(rmax(2.0f * beta * p0 / kappa + 1.0f, 1.0e-8f));
         float sv = powf(jn, 0.5f);
         if (hard) alpha = alpha + logf(jtr / jn);
-        fo = usv(u, sv, sv, vt);
-        ao = alpha;
-        return;
-    }
-    if (ptr < -beta * p0) {
-        float jn = sqrtf(rmax(2.0f * beta * p0 / kappa + 1.0f, 1.0e-8f));
-        float sv = powf(jn, 0.5f);
-        if (hard) alpha = alpha + logf(jtr / jn);
-        fo = usv(u, sv, sv, vt);
-        ao = alpha;
-        return;
-    }
-    float y0 = (1.0f + 2.0f * beta) * 2.0f;
-    float y1 = mm * mm * (ptr + beta * p0) * (ptr - p0);
-    float sn2 = st1 * st1 + st2 * st2;
-    float y = y0 * sn2 + y1;
-    if (y < 1.0e-4f) {
-        fo = f;
+        s1 = sv;
+        s2 = sv;
         ao = alpha;
         return;
     }
     if (hard && p0 > 1.0e-4f && ptr < p0 - 1.0e-4f && ptr > -beta * p0 + 1.0e-4f) {
         float pc = (1.0f - beta) * p0 * 0.5f;
-        float qtr = sqrtf(2.0f) * len2(st1, st2);
+        float qtr = sqrtf(2.0f) * len2(t.st1, t.st2);
         float dx = pc - ptr, dy = -qtr;
         float rcp = 1.0f / len2(dx, dy);
         if (__builtin_isfinite(rcp) && rcp > 0.0f) {
@@ -635,8 +650,8 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
         float jex = sqrtf(rmax(fabsf(-2.0f * px / kappa + 1.0f), 1.0e-8f));
         if (jex > 1.0e-4f) alpha = alpha + logf(jtr / jex);
     }
-    float sc = sqrtf(rmax(-y1 / rmax(y0, 1.0e-10f), 0.0f)) * (jtr / m.mu);
-    float nx = st1, ny = st2;
+    float sc = sqrtf(rmax(-t.y1 / rmax(t.y0, 1.0e-10f), 0.0f)) * (jtr / m.mu);
+    float nx = t.st1, ny = t.st2;
     float rcp = 1.0f / len2(nx, ny);
     if (__builtin_isfinite(rcp) && rcp > 0.0f) {
         nx = nx * rcp;
@@ -645,10 +660,25 @@ EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, flo
         nx = 0.0f;
         ny = 0.0f;
     }
-    float b1 = sc * nx + qt / 2.0f;
-    float b2 = sc * ny + qt / 2.0f;
-    fo = usv(u, sqrtf(rmax(b1, 1.0e-8f)), sqrtf(rmax(b2, 1.0e-8f)), vt);
+    float b1 = sc * nx + t.qt / 2.0f;
+    float b2 = sc * ny + t.qt / 2.0f;
+    s1 = sqrtf(rmax(b1, 1.0e-8f));
+    s2 = sqrtf(rmax(b2, 1.0e-8f));
     ao = alpha;
+}
+EMH_DEV void nacc_project(const DevMat& m, const M2& f, float alpha, M2& fo, float& ao) {
+    M2 u, vt;
+    float2 sg;
+    svd2(f, u, sg, vt);
+    const NaccTrial t = nacc_trial(m, sg, alpha);
+    if (t.yield == 0) {
+        fo = f;
+        ao = alpha;
+        return;
+    }
+    float s1, s2;
+    nacc_return(m, t, alpha, s1, s2, ao);
+    fo = usv(u, s1, s2, vt);
 }
 
 EMH_DEV void sync_volume(PState& s) {

@@ -44,6 +44,7 @@
 #include <cstring>
 
 #include "kernels.hpp"
+#include "surrogate_kernels.hpp"
 
 namespace emh {
 
@@ -727,6 +728,8 @@ __global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
 constexpr int kModeG2P = 1;   // gather + update the stored state (a substep)
 constexpr int kModeP2G = 2;   // scatter mass, momentum and stress moment
 constexpr int kModeMass = 4;  // scatter mass only (density re-estimate)
+constexpr int kModeHarvest = 8;     // record NACC trial states (training data of the surrogate)
+constexpr int kModeSurrogate = 16;  // NACC return mapping through the learned surrogate
 
 // Bounding box of the chunk's base cells (positions projected when the state
 // is flagged), widened by `margin` cells.
@@ -955,11 +958,61 @@ __device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCel
 // The substep kernel (MODE = G2P | P2G or G2P | Mass) and the prologue of a
 // freshly uploaded state (MODE = P2G or Mass: no gather, no store).  ONLY:
 // the one constitutive model every particle uses, or -1 for any mix.
+// NACC trial state (F_trial as update_particle forms it, alpha and the
+// material's parameters) of each particle into the harvest buffer; records
+// beyond its capacity are counted and dropped.
+__device__ __forceinline__ void harvest_nacc(const SgDev& sg, const DevMat& m, const PState& s, float dt) {
+    const bool want = sg.harvest != nullptr && m.model == kModelNacc;
+    const uint64_t act = __ballot(want);
+    if (act == 0ull) return;
+    M2 fin = s.f;
+    if (m.flags & kMatNaccAdvance) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
+    const uint32_t lane = lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+    unsigned long long at = 0;
+    if (lane == leader) at = atomicAdd(sg.harvest_count, (unsigned long long)__popcll(act));
+    at = __shfl(at, (int)leader);
+    if (!want) return;
+    at += (unsigned long long)__popcll(act & ((1ull << lane) - 1ull));
+    if (at >= sg.harvest_cap) return;
+    float4* r = reinterpret_cast<float4*>(sg.harvest + at * 12ull);
+    r[0] = make_float4(fin.a, fin.b, fin.c, fin.d);
+    r[1] = make_float4(s.lvs, m.mu, m.na_kappa, m.na_m);
+    r[2] = make_float4(m.na_beta, m.na_xi, __uint_as_float(m.flags), 0.0f);
+}
+
+// update_particle with the NACC return mapping through the learned
+// surrogate (nacc.rs:255-269 otherwise unchanged).  Every lane of the wave
+// calls this.
+template <int ONLY>
+__device__ __forceinline__ void surrogate_update(const SgDev& sg, const DevMat& m, bool valid, PState& s, float dt,
+                                                 float* s_io) {
+    const bool nacc = valid && m.model == kModelNacc;
+    M2 fin = s.f;
+    if (nacc && (m.flags & kMatNaccAdvance)) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
+    M2 nf;
+    float na;
+    sg_return_wave(m, sg, nacc, fin, s.lvs, nf, na, s_io);
+    if (nacc) {
+        s.f = nf;
+        s.lvs = na;
+        float j = rmax(mdet(nf), kMinJ);
+        float vol = s.v0 * j;
+        s.vol = vol;
+        if (vol > kMinJ) s.dens = s.mass / vol;
+    } else if (valid) {
+        update_particle<ONLY>(m, s, dt);
+    }
+}
+
 template <int MODE, int ONLY>
-__global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_t seq) {
+__global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_substep(const Pipeline p, uint64_t seq) {
     constexpr bool G2P = (MODE & kModeG2P) != 0;
     constexpr bool P2G = (MODE & kModeP2G) != 0;
+    constexpr bool HARVEST = (MODE & kModeHarvest) != 0;
+    constexpr bool SURR = (MODE & kModeSurrogate) != 0;
     constexpr int NCH = P2G ? kAccChannels : 1;
+    __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgWaveLds : 1];
     __shared__ float s_acc[NCH][kLdsCells];
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];
     __shared__ uint32_t s_perm[kChunk];
@@ -1040,8 +1093,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
         }
         const uint32_t j = base + (valid ? s_perm[q] : q);
         PState sp = neutral_state();
+        PState s = neutral_state();
         if (valid) {
-            PState s = neutral_state();
             const float2 x = a.x[j];
             s.x = x.x;
             s.y = x.y;
@@ -1093,12 +1146,26 @@ __global__ __launch_bounds__(kBlock, 4) void k_substep(const Pipeline p, uint64_
                     s.y = ny;
                 }
                 // phase 2: update_particle (g2p.rs:333-340)
-                update_particle<ONLY>(mat_of(p, s.mat), s, dt);
+                if constexpr (HARVEST) harvest_nacc(p.ctrl->sg, mat_of(p, s.mat), s, dt);
+                if constexpr (!SURR) update_particle<ONLY>(mat_of(p, s.mat), s, dt);
             }
-            // 4. the state the next substep starts from: CFL, projection check
-            cfl_of<ONLY>(p, s, speed, mdt);
-            any_dirty |= needs_projection(s, sp, cfg);
-            if (G2P) {
+            if constexpr (!SURR) {
+                // 4. the state the next substep starts from: CFL, projection check
+                cfl_of<ONLY>(p, s, speed, mdt);
+                any_dirty |= needs_projection(s, sp, cfg);
+                if (G2P) {
+                    const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
+                    store_fields(o, dest, s, mask, full);
+                    if (reorder) o.id[dest] = a.id[j];
+                }
+            }
+        }
+        if constexpr (SURR) {
+            // phase 2 with the learned return mapping (every lane of the wave)
+            surrogate_update<ONLY>(p.ctrl->sg, mat_of(p, s.mat), valid, s, dt, s_sg[threadIdx.x >> 6]);
+            if (valid) {
+                cfl_of<ONLY>(p, s, speed, mdt);
+                any_dirty |= needs_projection(s, sp, cfg);
                 const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
                 store_fields(o, dest, s, mask, full);
                 if (reorder) o.id[dest] = a.id[j];
@@ -1475,7 +1542,7 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream) {
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode) {
     const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
     if (p.cfg.density_pass) {
         k_density_p2g<<<nb, kBlock, 0, stream>>>(p, seq);
@@ -1487,7 +1554,14 @@ hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream) {
             case kModelCorot: k_substep<kFused, kModelCorot><<<nb, kBlock, 0, stream>>>(p, seq); break;
             case kModelSnow: k_substep<kFused, kModelSnow><<<nb, kBlock, 0, stream>>>(p, seq); break;
             case kModelDP: k_substep<kFused, kModelDP><<<nb, kBlock, 0, stream>>>(p, seq); break;
-            case kModelNacc: k_substep<kFused, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq); break;
+            case kModelNacc:
+                if (sg_mode == 1u)
+                    k_substep<kFused | kModeHarvest, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq);
+                else if (sg_mode == 2u)
+                    k_substep<kFused | kModeSurrogate, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq);
+                else
+                    k_substep<kFused, kModelNacc><<<nb, kBlock, 0, stream>>>(p, seq);
+                break;
             default: k_substep<kFused, -1><<<nb, kBlock, 0, stream>>>(p, seq); break;
         }
     }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "device_math.hpp"
+#include "surrogate.hpp"
 
 namespace emh {
 
@@ -103,6 +104,7 @@ struct Ctrl {
     CtrlRed red[3];
     uint32_t clamp_count[kStepRing];  // G2P velocity clamps of each step
     uint32_t proj_count[kStepRing];   // state projections of each step
+    SgDev sg;  // learned NACC return mapping (read by the harvest and surrogate substeps only)
 };
 constexpr uint32_t kInfBits = 0x7f800000u;
 
@@ -122,7 +124,9 @@ struct Pipeline {
 hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream);
 hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream);
 hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream);
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream);
+// sg_mode: 0 plain, 1 harvest NACC trial states, 2 NACC return mapping through
+// the learned surrogate (Ctrl::sg); modes 1 and 2 need a NACC-only scene.
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode = 0);
 
 // Device accumulator of the diagnostics snapshot (see launch_snapshot).
 struct DevSnapshot {

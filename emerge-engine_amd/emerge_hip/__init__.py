@@ -97,6 +97,17 @@ class CSpawnRegion(ctypes.Structure):
                 ("mass_override", ctypes.c_float)]
 
 
+class CTrainOptions(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("pad", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+                ("max_steps", ctypes.c_uint64)]
+
+
+class CSurrogateCounts(ctypes.Structure):
+    _fields_ = [("yielding", ctypes.c_uint64), ("surrogate", ctypes.c_uint64), ("fallback", ctypes.c_uint64)]
+
+
 class CKernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
 
@@ -237,6 +248,28 @@ def load_library() -> ctypes.CDLL:
         "emh_sim_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "emh_sim_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
                                                 P(ctypes.c_uint32)]),
+        "emh_surrogate_create": (ctypes.c_int, [ctypes.c_uint64, P(ctypes.c_void_p)]),
+        "emh_surrogate_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+        "emh_surrogate_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+        "emh_surrogate_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+        "emh_sim_set_surrogate_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                      ctypes.c_uint64]),
+        "emh_surrogate_harvest_count": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]),
+        "emh_surrogate_build_training_set": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64)]),
+        "emh_surrogate_get_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                                     ctypes.c_void_p]),
+        "emh_surrogate_set_training_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                          ctypes.c_uint64]),
+        "emh_surrogate_get_training_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                                          ctypes.c_void_p, ctypes.c_void_p]),
+        "emh_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, P(CTrainOptions), ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_surrogate_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                  ctypes.c_void_p, P(ctypes.c_float)]),
+        "emh_surrogate_infer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+        "emh_surrogate_get_counts": (ctypes.c_int, [ctypes.c_void_p, P(CSurrogateCounts), ctypes.c_int]),
+        "emh_nacc_return_batch": (ctypes.c_int, [ctypes.c_void_p, P(CMaterialParams), P(CMaterialExt),
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_float)]),
         "emh_cell_key": (ctypes.c_uint32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32]),
         "emh_sort_cell_keys": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
                                               ctypes.c_void_p]),
@@ -840,6 +873,15 @@ class Simulation:
             d[name] = tuple(v) if isinstance(v, ctypes.Array) else v
         return d
 
+    def set_surrogate_mode(self, surrogate: Optional["NaccSurrogate"], mode: str,
+                           harvest_capacity: int = 0) -> None:
+        """NACC return mapping of the following substeps: "plain", "harvest" (plain,
+        recording every trial state into the surrogate's buffer) or "surrogate"."""
+        code = {"plain": 0, "harvest": 1, "surrogate": 2}[mode]
+        _check(load_library().emh_sim_set_surrogate_mode(self._h, surrogate._h if surrogate else None, code,
+                                                          int(harvest_capacity)))
+        self._surrogate = surrogate  # keeps it alive while the sim uses it
+
     def set_gravity(self, gravity) -> None:
         """Simulation::set_gravity (lifecycle.rs:403-405)."""
         _check(load_library().emh_sim_set_gravity(self._h, float(gravity[0]), float(gravity[1])))
@@ -870,3 +912,114 @@ class Simulation:
         mom = np.zeros((r * r, 2), dtype=np.float32)
         _check(load_library().emh_sim_get_grid(self._h, m.ctypes.data, mom.ctypes.data, r * r))
         return m, mom
+
+
+# -------------------------------------------------------------- NACC surrogate
+SURROGATE_PARAMS = 8963
+SURROGATE_NORM = 20
+
+
+class NaccSurrogate:
+    """Learned NACC return mapping (a 6-64-64-64-3 SiLU MLP on BF16 MFMA) on one GPU."""
+
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        _check(load_library().emh_surrogate_create(int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            load_library().emh_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> Tuple[np.ndarray, np.ndarray]:
+        w = np.zeros(SURROGATE_PARAMS, dtype=np.float32)
+        nm = np.zeros(SURROGATE_NORM, dtype=np.float32)
+        _check(load_library().emh_surrogate_get_params(self._h, w.ctypes.data, nm.ctypes.data))
+        return w, nm
+
+    def set_params(self, params: Optional[np.ndarray] = None, norm: Optional[np.ndarray] = None) -> None:
+        w = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, dtype=np.float32)
+        if w is not None and w.size != SURROGATE_PARAMS:
+            raise ValueError("params has the wrong size")
+        if nm is not None and nm.size != SURROGATE_NORM:
+            raise ValueError("norm has the wrong size")
+        _check(load_library().emh_surrogate_set_params(self._h, None if w is None else w.ctypes.data,
+                                                       None if nm is None else nm.ctypes.data))
+
+    def harvest_count(self) -> Tuple[int, int]:
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load_library().emh_surrogate_harvest_count(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def records(self, first: int = 0, n: Optional[int] = None) -> np.ndarray:
+        """Harvested trial states [n, 12]: F_trial (a, b, c, d), alpha, mu, kappa, M, beta, xi, flags, 0."""
+        if n is None:
+            n = self.harvest_count()[1] - first
+        out = np.zeros((n, 12), dtype=np.float32)
+        _check(load_library().emh_surrogate_get_records(self._h, first, n, out.ctypes.data))
+        return out
+
+    def build_training_set(self) -> int:
+        n = ctypes.c_uint64(0)
+        _check(load_library().emh_surrogate_build_training_set(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_training_set(self, x: np.ndarray, y: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 3)
+        if len(x) != len(y):
+            raise ValueError("x and y differ in length")
+        _check(load_library().emh_surrogate_set_training_set(self._h, x.ctypes.data, y.ctypes.data, len(x)))
+
+    def training_set(self, first: int, n: int) -> Tuple[np.ndarray, np.ndarray]:
+        x = np.zeros((n, 6), dtype=np.float32)
+        y = np.zeros((n, 3), dtype=np.float32)
+        _check(load_library().emh_surrogate_get_training_set(self._h, first, n, x.ctypes.data, y.ctypes.data))
+        return x, y
+
+    def train(self, epochs: int, batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5, beta1: float = 0.9,
+              beta2: float = 0.999, eps: float = 1e-8, seed: int = 1, max_steps: int = 0) -> np.ndarray:
+        o = CTrainOptions(epochs, batch, lr, lr_final, beta1, beta2, eps, 0, seed, max_steps)
+        loss = np.zeros(epochs, dtype=np.float32)
+        _check(load_library().emh_surrogate_train(self._h, ctypes.byref(o), loss.ctypes.data, epochs))
+        return loss
+
+    def gradient(self, x: np.ndarray, y: np.ndarray) -> Tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(SURROGATE_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        _check(load_library().emh_surrogate_gradient(self._h, x.ctypes.data, y.ctypes.data, len(x), g.ctypes.data,
+                                                     ctypes.byref(loss)))
+        return g, loss.value
+
+    def infer(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        y = np.zeros((len(x), 3), dtype=np.float32)
+        _check(load_library().emh_surrogate_infer(self._h, x.ctypes.data, len(x), y.ctypes.data))
+        return y
+
+    def counts(self, reset: bool = False) -> dict:
+        c = CSurrogateCounts()
+        _check(load_library().emh_surrogate_get_counts(self._h, ctypes.byref(c), int(reset)))
+        return {"yielding": c.yielding, "surrogate": c.surrogate, "fallback": c.fallback}
+
+
+def nacc_return_batch(material: "NaccMaterial", f_ptr: int, alpha_ptr: int, n: int, f_out_ptr: int,
+                      alpha_out_ptr: int, surrogate: Optional[NaccSurrogate] = None, reps: int = 1) -> float:
+    """NACC return mapping of n device-resident trial states (plain, or through
+    the surrogate); returns the milliseconds per launch."""
+    ms = ctypes.c_float(0.0)
+    _check(load_library().emh_nacc_return_batch(surrogate._h if surrogate else None, ctypes.byref(material.params),
+                                                ctypes.byref(material.ext), ctypes.c_void_p(f_ptr),
+                                                ctypes.c_void_p(alpha_ptr), n, ctypes.c_void_p(f_out_ptr),
+                                                ctypes.c_void_p(alpha_out_ptr), reps, ctypes.byref(ms)))
+    return ms.value

@@ -427,6 +427,89 @@ uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
                        uint32_t* keys_out, uint32_t* perm_out);
 
+/* ---------------------------------------------------------------------
+ * Learned NACC return-mapping surrogate.
+ *
+ * No counterpart in the reference: the crate's return mapping is
+ * NaccMaterial::project (materials/nacc.rs:129-223), applied by
+ * update_particle (nacc.rs:255-269); these entry points add the learned
+ * replacement for it.  A 6-64-64-64-3 SiLU MLP maps the trial state
+ * (ln s1, ln s2, alpha, M, beta, xi) -- s the singular values of F_trial,
+ * alpha the hardening state (Particle::log_volume_strain slot), M / beta /
+ * xi the material's stretch_limit / compression_limit / hardening_exponent
+ * fields -- to the projected (ln s1', ln s2', alpha').  Weights are BF16 on
+ * MFMA with FP32 accumulation; training (forward, backward, Adam) runs on
+ * the device.  Each particle is gated by the exact yield-function check:
+ * a state inside the yield surface keeps F, a state outside it goes through
+ * the network unless its inputs leave the training range (more than z_max
+ * standard deviations) or s2 <= 0, in which case the plain return mapping
+ * runs instead.
+ * ------------------------------------------------------------------- */
+typedef struct emh_surrogate emh_surrogate;
+
+#define EMH_SURROGATE_PARAMS 8963u /* W1[64][6] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3] */
+#define EMH_SURROGATE_NORM   20u   /* in_mu[6] in_sd[6] out_mu[3] out_sd[3] z_max pad */
+
+typedef struct emh_surrogate_train_options {
+    uint32_t epochs;
+    uint32_t batch;      /* rounded up to a multiple of 256 */
+    float    lr;         /* cosine schedule from lr to lr_final over the run */
+    float    lr_final;
+    float    beta1, beta2, eps;
+    uint32_t pad;
+    uint64_t seed;       /* per-epoch sample order */
+    uint64_t max_steps;  /* 0: epochs * (n_set / batch) */
+} emh_surrogate_train_options;
+
+typedef struct emh_surrogate_counts {
+    uint64_t yielding;   /* trial states that failed the yield check */
+    uint64_t surrogate;  /* ... returned through the network */
+    uint64_t fallback;   /* ... returned by the plain mapping (out of range, s2 <= 0) */
+} emh_surrogate_counts;
+
+/* Glorot-uniform weights from `seed`, zero biases, identity normalisation. */
+int emh_surrogate_create(uint64_t seed, emh_surrogate** out);
+int emh_surrogate_destroy(emh_surrogate* sg);
+/* FP32 master weights (EMH_SURROGATE_PARAMS floats) and the normalisation
+ * block (EMH_SURROGATE_NORM floats); either pointer may be NULL.  set_params
+ * resets the Adam moments. */
+int emh_surrogate_get_params(emh_surrogate* sg, float* params, float* norm);
+int emh_surrogate_set_params(emh_surrogate* sg, const float* params, const float* norm);
+/* mode 0: plain return mapping; 1: plain, and record every NACC particle's
+ * trial state (F_trial as update_particle forms it, alpha, material) into
+ * sg's buffer of harvest_capacity records; 2: return mapping through sg.
+ * Modes 1 and 2 need a scene whose particles are all NACC (the next step
+ * fails with EMH_ERR_UNSUPPORTED otherwise).  sg must outlive its use by the
+ * sim (or be detached with mode 0). */
+int emh_sim_set_surrogate_mode(emh_sim* sim, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity);
+int emh_surrogate_harvest_count(emh_surrogate* sg, uint64_t* offered, uint64_t* stored);
+/* Harvested records [first, first + n), 12 floats each: F_trial (column-major
+ * a, b, c, d), alpha, mu, kappa, M, beta, xi, flags (uint32 bits), 0. */
+int emh_surrogate_get_records(emh_surrogate* sg, uint64_t first, uint64_t n, float* out);
+/* Training set from the harvested records: those failing the yield check
+ * with s2 > 0, labelled by the plain return mapping; sets the normalisation. */
+int emh_surrogate_build_training_set(emh_surrogate* sg, uint64_t* n_samples);
+/* Host samples x[n][6] (raw inputs), y[n][3] (raw targets); sets the normalisation. */
+int emh_surrogate_set_training_set(emh_surrogate* sg, const float* x, const float* y, uint64_t n);
+int emh_surrogate_get_training_set(emh_surrogate* sg, uint64_t first, uint64_t n, float* x, float* y);
+/* Adam on the training set; epoch_loss[e] = mean over samples of the summed
+ * squared error of the normalised residual outputs. */
+int emh_surrogate_train(emh_surrogate* sg, const emh_surrogate_train_options* o, float* epoch_loss,
+                        uint32_t n_epoch_loss);
+/* Gradient of that loss over the given samples (no update): grad has
+ * EMH_SURROGATE_PARAMS floats; loss is the mean. */
+int emh_surrogate_gradient(emh_surrogate* sg, const float* x, const float* y, uint32_t n, float* grad, float* loss);
+/* Network outputs y[n][3] = x[:, 0:3] + denormalised residual for raw x[n][6]. */
+int emh_surrogate_infer(emh_surrogate* sg, const float* x, uint64_t n, float* y);
+int emh_surrogate_get_counts(emh_surrogate* sg, emh_surrogate_counts* out, int reset);
+/* NACC return mapping of n trial states on the device (device pointers:
+ * f_trial / f_out [n][4] column-major 2x2, alpha / alpha_out [n]), through
+ * sg (gated as above) or, with sg NULL, the plain mapping; `reps` back-to-back
+ * launches timed with HIP events on the launching stream. */
+int emh_nacc_return_batch(emh_surrogate* sg, const emh_material_params* p, const emh_material_ext* ext,
+                          const float* f_trial, const float* alpha, uint64_t n, float* f_out, float* alpha_out,
+                          uint32_t reps, float* ms_per_rep);
+
 #ifdef __cplusplus
 }
 #endif
