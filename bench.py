@@ -43,8 +43,8 @@ ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 # HBM bytes per k_substep launch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (one run each, mean over launches) and scaled by
 # the same counters' reading for a 256 MiB device-to-device copy
-# (profiles/k_substep_sand_1m_512.md): 26861 KiB x 2.000 + 78440 KiB x 1.000.
-MEASURED_TRAFFIC = {"k_substep": (26861.0 * 2.000 + 78440.0 * 1.000) * 1024.0}
+# (profiles/k_substep_sand_1m_512.md): 27042 KiB x 2.000 + 78950 KiB x 1.000.
+MEASURED_TRAFFIC = {"k_substep": (27042.1 * 2.000 + 78949.8 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu"):
@@ -82,6 +82,155 @@ def cpu_baseline(target_s: float, first_dt: float) -> dict:
                       f"substep; oracle f32, 1 thread, {t:.1f} s"}
 
 
+# ------------------------------------------------------------ NACC surrogate
+# Algorithmic MFMA work of one network evaluation: 2 * (6*64 + 64*64 + 64*64 + 64*3).
+SURROGATE_FLOPS = 2 * (6 * 64 + 64 * 64 + 64 * 64 + 64 * 3)
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+
+
+def build_nacc_slope(seed: int):
+    """NACC soil (E = 2e5, nu = 0.3, M = 1.85, beta = 0.5, xi = 0.8, F advanced
+    before the projection) in 24 seeded random blocks (spacing 0.25, 16 per
+    cell, about 2.0M particles) above a heightmap slope (terrain 120 -> 20
+    cells across the 512^2 grid, terrain friction 0.3); gravity (0, -9.81),
+    step dt 1/60."""
+    cfg = eh.SimConfig.standard(512, 1.0 / 60.0, (0.0, -9.81))
+    mat = eh.NaccMaterial.from_young_modulus(2e5, 0.3, 1.85, 0.5, 0.8, advance_f=True)
+    sim = eh.Simulation.empty(cfg).with_default_material(mat)
+    sim.clear_boundaries()
+    sim.with_boundary(eh.HeightmapBoundary.new([20.0 + 100.0 * (1.0 - x / 511.0) for x in range(512)], 0.3, 2))
+    rng = np.random.default_rng(seed)
+    k = 0
+    for row in range(4):
+        for col in range(6):
+            w, h = int(rng.integers(62, 79)), int(rng.integers(66, 85))
+            cx = 16.0 + 80.0 * col + 40.0 + rng.uniform(-0.5, 0.5) * (80 - w - 2)
+            cy = 150.0 + 87.5 * row + 43.75 + rng.uniform(-0.5, 0.5) * (87.5 - h - 2)
+            sim.add_body(eh.SpawnRegion(spacing=0.25, box_size=(w, h), box_center=(float(cx), float(cy)),
+                                        position_jitter=1.0, rng_seed=seed * 100 + k))
+            k += 1
+    return sim, mat
+
+
+def surrogate_section(args) -> dict:
+    """Learned NACC return mapping on the slope scene: train on trial states
+    harvested from a plain run (seed 101), evaluate on every trial state of
+    one substep of another run (seed 202) after it has landed."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import surrogate_ref  # the checker: Kirchhoff stress of a returned F
+
+    t_all = time.perf_counter()
+    sg = eh.NaccSurrogate(7)
+    train_sim, mat = build_nacc_slope(101)
+    n_train = train_sim.num_particles()
+    train_sim.step_n(args.sg_warm_steps)
+    cap = args.sg_records
+    stride = max(1, int(n_train * 10 * args.sg_harvest_steps // cap))  # about 10 substeps per step
+    train_sim.set_surrogate_mode(sg, "harvest", cap, stride)
+    train_sim.step_n(args.sg_harvest_steps)
+    train_sim.synchronize()
+    train_sim.set_surrogate_mode(None, "plain")
+    offered, stored = sg.harvest_count()
+    train_sim.close()
+    n_set = sg.build_training_set()
+    t0 = time.perf_counter()
+    loss = sg.train(epochs=args.sg_epochs, batch=args.sg_batch, lr=args.sg_lr, lr_final=args.sg_lr * 1e-2, seed=3)
+    train_s = time.perf_counter() - t0
+
+    ev, mat = build_nacc_slope(202)
+    n = ev.num_particles()
+    ev.step_n(args.sg_eval_steps)
+    ev.set_surrogate_mode(sg, "harvest", 2 * n, 1)
+    ev.step_n(1)
+    ev.synchronize()
+    ev.set_surrogate_mode(None, "plain")
+    rec = sg.records(0, n)  # every particle's trial state in the step's first substep
+    reps = args.sg_reps
+    ms_plain, fp, ap = sg.eval_records(mat, 0, n, False, 1)
+    sg.counts(reset=True)
+    ms_sg, fs, as_ = sg.eval_records(mat, 0, n, True, 1)
+    counts = sg.counts(reset=True)
+    ms_plain, _, _ = sg.eval_records(mat, 0, n, False, reps, outputs=False)
+    ms_sg, _, _ = sg.eval_records(mat, 0, n, True, reps, outputs=False)
+    sg.counts(reset=True)
+
+    f_trial = rec[:, :4]
+    yl = ~np.all(fp == f_trial, axis=1)
+    mu, kappa = float(rec[0, 5]), float(rec[0, 6])
+    tp = surrogate_ref.nacc_kirchhoff(fp[yl], mu, kappa)
+    ts = surrogate_ref.nacc_kirchhoff(fs[yl], mu, kappa)
+    tt = surrogate_ref.nacc_kirchhoff(f_trial[yl], mu, kappa)
+    agg = float(np.linalg.norm(ts - tp) / np.linalg.norm(tp))
+    agg_trial = float(np.linalg.norm(tt - tp) / np.linalg.norm(tp))
+    # per-particle error in units of the RMS stress of the yielding states
+    per = np.linalg.norm(ts - tp, axis=1) / np.sqrt(np.mean(np.sum(tp * tp, axis=1)))
+
+    # FP64 oracle on a sample of the yielding states
+    import oracle_py
+    oracle_py.load()
+    om = oracle_py.material("nacc", mu, kappa, mat.params.stretch_limit, mat.params.compression_limit,
+                            mat.params.hardening_exponent)
+    om.ext.nacc_advance_f = 1
+    idx = np.flatnonzero(yl)
+    idx = idx[np.random.default_rng(0).permutation(len(idx))[:args.sg_fp64_sample]]
+    f64, _ = oracle_py.nacc_project_batch(om, f_trial[idx], rec[idx, 4], 64)
+    t64 = surrogate_ref.nacc_kirchhoff(f64, mu, kappa)
+    e64 = lambda t: float(np.linalg.norm(t - t64) / np.linalg.norm(t64))
+    e_plain64 = e64(surrogate_ref.nacc_kirchhoff(fp[idx], mu, kappa))
+    e_sg64 = e64(surrogate_ref.nacc_kirchhoff(fs[idx], mu, kappa))
+
+    # the whole substep with either return mapping (the scene continues)
+    def substep_us(mode):
+        ev.set_surrogate_mode(sg if mode == "surrogate" else None, mode)
+        ev.set_profiling(True)
+        ev.step_n(args.sg_substep_steps)
+        ev.synchronize()
+        t = ev.kernel_times().get("k_substep", (0, 0.0))
+        ev.set_profiling(False)
+        return t[1] / max(t[0], 1) * 1e3
+    sub_plain = substep_us("plain")
+    sub_sg = substep_us("surrogate")
+    ev.set_surrogate_mode(None, "plain")
+    ev.close()
+
+    # CPU: the oracle's f32 return mapping, one thread, on a sample of the same states
+    m_cpu = min(n, args.sg_cpu_sample)
+    t0 = time.perf_counter()
+    oracle_py.nacc_project_batch(om, f_trial[:m_cpu], rec[:m_cpu, 4], 32)
+    cpu_s = time.perf_counter() - t0
+
+    inferred = counts["surrogate"]
+    tflops = SURROGATE_FLOPS * inferred / (ms_sg * 1e-3) / 1e12
+    sg.close()
+    return {
+        "workload": "nacc_slope_2m_512",
+        "particles": n,
+        "speedup": ms_plain / ms_sg,
+        "plain_ms": ms_plain,
+        "surrogate_ms": ms_sg,
+        "yielding": int(yl.sum()),
+        "counts": counts,
+        "stress_rel_err_vs_plain": agg,
+        "stress_err_per_particle_p50": float(np.percentile(per, 50)),
+        "stress_err_per_particle_p99": float(np.percentile(per, 99)),
+        "stress_rel_err_trial_vs_plain": agg_trial,
+        "stress_rel_err_vs_fp64": {"surrogate": e_sg64, "plain_f32": e_plain64, "sample": int(len(idx))},
+        "target_err": 1e-3,
+        "meets_target": bool(agg <= 1e-3 and ms_plain / ms_sg >= 1.5),
+        "substep_us": {"plain": sub_plain, "surrogate": sub_sg, "speedup": sub_plain / sub_sg},
+        "roofline": {"bound": "mfma", "kernel": "k_nacc_return_surrogate", "achieved": tflops,
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                     "flops_per_inference": SURROGATE_FLOPS},
+        "training": {"records_offered": offered, "records_stored": stored, "harvest_stride": stride,
+                     "samples": n_set, "epochs": args.sg_epochs, "batch": args.sg_batch, "seconds": train_s,
+                     "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
+                     "scene": "nacc_slope seed 101", "particles": n_train},
+        "cpu_baseline": {"value": m_cpu / cpu_s, "unit": "return-mappings/s", "cores": 1, "kind": "port",
+                         "sample": f"{m_cpu} trial states of the evaluation substep, oracle f32 NaccMaterial::project"},
+        "seconds": time.perf_counter() - t_all,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +239,18 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the NACC surrogate section")
+    ap.add_argument("--sg-warm-steps", type=int, default=200)
+    ap.add_argument("--sg-harvest-steps", type=int, default=200)
+    ap.add_argument("--sg-records", type=int, default=24_000_000)
+    ap.add_argument("--sg-epochs", type=int, default=400)
+    ap.add_argument("--sg-batch", type=int, default=16384)
+    ap.add_argument("--sg-lr", type=float, default=1e-2)
+    ap.add_argument("--sg-eval-steps", type=int, default=400)
+    ap.add_argument("--sg-reps", type=int, default=20)
+    ap.add_argument("--sg-substep-steps", type=int, default=10)
+    ap.add_argument("--sg-fp64-sample", type=int, default=50_000)
+    ap.add_argument("--sg-cpu-sample", type=int, default=200_000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the bench line, run three device-to-device copies of this size (a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -168,6 +329,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, first_dt)
     else:
         out["cpu_baseline"] = None
+    sim.close()
+    out["nacc_surrogate"] = None if args.no_surrogate else surrogate_section(args)
     print(json.dumps(out), flush=True)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)

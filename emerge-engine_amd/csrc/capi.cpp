@@ -1753,9 +1753,12 @@ struct emh_surrogate {
     float* d_grad = nullptr;  // gradient accumulator (cleared by each Adam step)
     float* d_norm = nullptr;  // kSgNorm
     uint4* d_pk = nullptr;    // kPkTiles * 64 packed operand lanes
-    unsigned long long* d_counts = nullptr;  // [0..2] return-mapping counts, [3] harvest offers
+    float* d_bias = nullptr;  // compact bias block (kSgBias)
+    unsigned long long* d_counts = nullptr;  // 3 x kSgCountSlots return-mapping counts, then the harvest offers
     float* d_rec = nullptr;   // harvested records, 12 floats each
-    uint64_t rec_cap = 0;
+    uint64_t rec_cap = 0;     // allocated records
+    uint64_t rec_used = 0;    // capacity of the current harvest
+    uint64_t rec_stride = 1;  // its stride
     float* d_xs = nullptr;    // training inputs [n][8]
     float* d_ts = nullptr;    // training targets [n][4]
     uint64_t n_set = 0, set_cap = 0;
@@ -1770,7 +1773,7 @@ struct emh_surrogate {
     ~emh_surrogate() { release(); }
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* q : {(void*)d_w, (void*)d_m, (void*)d_v, (void*)d_grad, (void*)d_norm, (void*)d_pk,
+        for (void* q : {(void*)d_w, (void*)d_m, (void*)d_v, (void*)d_grad, (void*)d_norm, (void*)d_pk, (void*)d_bias,
                         (void*)d_counts, (void*)d_rec, (void*)d_xs, (void*)d_ts, (void*)d_scratch, (void*)d_sums,
                         (void*)d_loss})
             if (q) (void)hipFree(q);
@@ -1780,12 +1783,12 @@ struct emh_surrogate {
     SgDev dev(bool harvest) const {
         SgDev d{};
         d.pk = d_pk;
-        d.prm = d_w;
+        d.bias = d_bias;
         d.norm = d_norm;
         d.counts = d_counts;
         if (harvest) {
             d.harvest = d_rec;
-            d.harvest_count = d_counts + 3;
+            d.harvest_count = d_counts + 3 * kSgCountSlots;
             d.harvest_cap = rec_cap;
         }
         return d;
@@ -1795,7 +1798,7 @@ struct emh_surrogate {
         return EMH_OK;
     }
     int pack() {
-        HIP_TRY(sg_launch_pack(d_w, d_pk, stream));
+        HIP_TRY(sg_launch_pack(d_w, d_pk, d_bias, stream));
         return EMH_OK;
     }
     // Normalisation from the sums of x (6), residual y - x[0:3] (3), and their squares.
@@ -1859,7 +1862,7 @@ struct emh_surrogate {
         b.n_valid = n_valid;
         b.inv_n = 1.0f / (float)std::max<uint32_t>(n_valid, 1u);
         b.pk = d_pk;
-        b.prm = d_w;
+        b.bias = d_bias;
         b.norm = d_norm;
         return b;
     }
@@ -1903,10 +1906,11 @@ int emh_surrogate_create(uint64_t seed, emh_surrogate** out) {
     sg->d_grad = dalloc<float>(kSgParams);
     sg->d_norm = dalloc<float>(kSgNorm);
     sg->d_pk = dalloc<uint4>((size_t)kPkTiles * 64);
-    sg->d_counts = dalloc<unsigned long long>(4);
+    sg->d_bias = dalloc<float>(kSgBias);
+    sg->d_counts = dalloc<unsigned long long>(3 * kSgCountSlots + 1);
     sg->d_sums = dalloc<double>(18);
     sg->d_loss = dalloc<float>(1);
-    if (!sg->d_w || !sg->d_m || !sg->d_v || !sg->d_grad || !sg->d_norm || !sg->d_pk || !sg->d_counts || !sg->d_sums ||
+    if (!sg->d_w || !sg->d_m || !sg->d_v || !sg->d_grad || !sg->d_norm || !sg->d_pk || !sg->d_bias || !sg->d_counts || !sg->d_sums ||
         !sg->d_loss)
         return bail(fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed"));
     // uniform(-1, 1) * sqrt(6 / (fan_in + fan_out)) weights, zero biases
@@ -1931,7 +1935,7 @@ int emh_surrogate_create(uint64_t seed, emh_surrogate** out) {
         hipMemsetAsync(sg->d_m, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
         hipMemsetAsync(sg->d_v, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
         hipMemsetAsync(sg->d_grad, 0, sizeof(float) * kSgParams, s) != hipSuccess ||
-        hipMemsetAsync(sg->d_counts, 0, sizeof(unsigned long long) * 4, s) != hipSuccess)
+        hipMemsetAsync(sg->d_counts, 0, sizeof(unsigned long long) * (3 * kSgCountSlots + 1), s) != hipSuccess)
         return bail(fail(EMH_ERR_DEVICE, "surrogate initialisation failed"));
     int rc = sg->upload_norm();
     if (!rc) rc = sg->pack();
@@ -1977,7 +1981,8 @@ int emh_surrogate_set_params(emh_surrogate* sg, const float* params, const float
     return EMH_OK;
 }
 
-int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity) {
+int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity,
+                               uint32_t harvest_stride) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     if (mode > 2) return fail(EMH_ERR_INVALID_ARG, "mode must be 0 (plain), 1 (harvest) or 2 (surrogate)");
     if (mode != 0 && !sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
@@ -1990,13 +1995,18 @@ int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uin
             sg->rec_cap = sg->d_rec ? harvest_capacity : 0;
             if (!sg->d_rec) return fail(EMH_ERR_OUT_OF_MEMORY, "harvest buffer allocation failed");
         }
-        HIP_TRY(hipMemsetAsync(sg->d_counts + 3, 0, sizeof(unsigned long long), sg->stream));
+        HIP_TRY(hipMemsetAsync(sg->d_counts + 3 * kSgCountSlots, 0, sizeof(unsigned long long), sg->stream));
         HIP_TRY(hipStreamSynchronize(sg->stream));
     }
     // drain queued work before the descriptor changes under it
     if (s->stream) HIP_TRY(hipStreamSynchronize(s->stream));
     s->sg_host = mode == 0 ? SgDev{} : sg->dev(mode == 1);
-    if (mode == 1) s->sg_host.harvest_cap = harvest_capacity;
+    if (mode == 1) {
+        s->sg_host.harvest_cap = harvest_capacity;
+        s->sg_host.harvest_stride = std::max<uint32_t>(harvest_stride, 1u);
+        sg->rec_stride = s->sg_host.harvest_stride;
+        sg->rec_used = harvest_capacity;
+    }
     s->sg_mode = mode;
     if (s->d_ctrl) {
         HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(s->d_ctrl) + offsetof(Ctrl, sg), &s->sg_host, sizeof(SgDev),
@@ -2009,10 +2019,10 @@ int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uin
 int emh_surrogate_harvest_count(emh_surrogate* sg, uint64_t* offered, uint64_t* stored) {
     if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
     unsigned long long c = 0;
-    HIP_TRY(hipMemcpyAsync(&c, sg->d_counts + 3, sizeof(c), hipMemcpyDeviceToHost, sg->stream));
+    HIP_TRY(hipMemcpyAsync(&c, sg->d_counts + 3 * kSgCountSlots, sizeof(c), hipMemcpyDeviceToHost, sg->stream));
     HIP_TRY(hipStreamSynchronize(sg->stream));
     if (offered) *offered = c;
-    if (stored) *stored = std::min<uint64_t>(c, sg->rec_cap);
+    if (stored) *stored = std::min<uint64_t>((c + sg->rec_stride - 1) / sg->rec_stride, sg->rec_used);
     return EMH_OK;
 }
 
@@ -2028,6 +2038,30 @@ int emh_surrogate_get_records(emh_surrogate* sg, uint64_t first, uint64_t n, flo
     return EMH_OK;
 }
 
+int emh_surrogate_eval_records(emh_surrogate* sg, const emh_material_params* p, const emh_material_ext* ext,
+                               uint64_t first, uint64_t n, int use_surrogate, uint32_t reps, float* ms_per_rep,
+                               float* f_out, float* alpha_out) {
+    if (!sg || !p) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    uint64_t stored = 0;
+    int rc = emh_surrogate_harvest_count(sg, nullptr, &stored);
+    if (rc) return rc;
+    if (n == 0 || first + n > stored) return fail(EMH_ERR_INVALID_ARG, "range beyond the harvested records");
+    float* d = dalloc<float>(n * 10);  // trial F, alpha, then the outputs
+    if (!d) return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
+    float *df = d, *da = d + 4 * n, *dfo = d + 5 * n, *dao = d + 9 * n;
+    const float* src = sg->d_rec + first * 12;
+    hipError_t e = hipMemcpy2DAsync(df, 16, src, 48, 16, n, hipMemcpyDeviceToDevice, sg->stream);
+    if (e == hipSuccess) e = hipMemcpy2DAsync(da, 4, src + 4, 48, 4, n, hipMemcpyDeviceToDevice, sg->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sg->stream);
+    if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "staging: %s", hipGetErrorString(e));
+    if (!rc) rc = emh_nacc_return_batch(use_surrogate ? sg : nullptr, p, ext, df, da, n, dfo, dao, reps, ms_per_rep);
+    if (!rc && f_out) e = hipMemcpy(f_out, dfo, sizeof(float) * 4 * n, hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess && alpha_out) e = hipMemcpy(alpha_out, dao, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (!rc && e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "copy-out: %s", hipGetErrorString(e));
+    (void)hipFree(d);
+    return rc;
+}
+
 int emh_surrogate_build_training_set(emh_surrogate* sg, uint64_t* n_samples) {
     if (!sg) return fail(EMH_ERR_INVALID_ARG, "null surrogate");
     uint64_t offered = 0, n_rec = 0;
@@ -2036,15 +2070,15 @@ int emh_surrogate_build_training_set(emh_surrogate* sg, uint64_t* n_samples) {
     if (n_rec == 0) return fail(EMH_ERR_STATE, "no harvested trial states");
     rc = sg->ensure_set(n_rec);
     if (rc) return rc;
-    unsigned long long* d_n = sg->d_counts;  // reuse slot 0 as the output counter, reset below
-    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long) * 3, sg->stream));
+    unsigned long long* d_n = sg->d_counts;  // reuse count slot 0 as the output counter, reset below
+    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long), sg->stream));
     HIP_TRY(hipMemsetAsync(sg->d_sums, 0, sizeof(double) * 18, sg->stream));
     HIP_TRY(sg_launch_prepare(sg->d_rec, n_rec, sg->d_xs, sg->d_ts, d_n, sg->d_sums, sg->stream));
     unsigned long long n = 0;
     double sums[18];
     HIP_TRY(hipMemcpyAsync(&n, d_n, sizeof(n), hipMemcpyDeviceToHost, sg->stream));
     HIP_TRY(hipMemcpyAsync(sums, sg->d_sums, sizeof(sums), hipMemcpyDeviceToHost, sg->stream));
-    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long) * 3, sg->stream));
+    HIP_TRY(hipMemsetAsync(d_n, 0, sizeof(unsigned long long), sg->stream));
     HIP_TRY(hipStreamSynchronize(sg->stream));
     sg->n_set = n;
     if (n == 0) return fail(EMH_ERR_STATE, "no harvested trial state fails the yield check");
@@ -2139,7 +2173,7 @@ int emh_surrogate_train(emh_surrogate* sg, const emh_surrogate_train_options* o,
             ++sg->adam_t;
             h.c1 = (float)(1.0 / (1.0 - std::pow((double)o->beta1, (double)sg->adam_t)));
             h.c2 = (float)(1.0 / (1.0 - std::pow((double)o->beta2, (double)sg->adam_t)));
-            HIP_TRY(sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, &h, sg->stream));
+            HIP_TRY(sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, &h, sg->d_bias, sg->stream));
         }
         float l = 0.0f;
         HIP_TRY(hipMemcpyAsync(&l, sg->d_loss, sizeof(float), hipMemcpyDeviceToHost, sg->stream));
@@ -2171,7 +2205,7 @@ int emh_surrogate_gradient(emh_surrogate* sg, const float* x, const float* y, ui
         if (e == hipSuccess) e = hipMemcpyAsync(d_t, ts.data(), sizeof(float) * ts.size(), hipMemcpyHostToDevice, sg->stream);
         if (e == hipSuccess) e = hipMemsetAsync(sg->d_grad, 0, sizeof(float) * kSgParams, sg->stream);
         if (e == hipSuccess) e = hipMemsetAsync(sg->d_loss, 0, sizeof(float), sg->stream);
-        if (e == hipSuccess) e = sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, nullptr, sg->stream);
+        if (e == hipSuccess) e = sg_launch_train_step(b, sg->bufs, sg->d_w, sg->d_m, sg->d_v, sg->d_grad, nullptr, sg->d_bias, sg->stream);
         if (e == hipSuccess && grad)
             e = hipMemcpyAsync(grad, sg->d_grad, sizeof(float) * kSgParams, hipMemcpyDeviceToHost, sg->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(&l, sg->d_loss, sizeof(float), hipMemcpyDeviceToHost, sg->stream);
@@ -2206,13 +2240,17 @@ int emh_surrogate_infer(emh_surrogate* sg, const float* x, uint64_t n, float* y)
 
 int emh_surrogate_get_counts(emh_surrogate* sg, emh_surrogate_counts* out, int reset) {
     if (!sg || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
-    unsigned long long c[3];
-    HIP_TRY(hipMemcpyAsync(c, sg->d_counts, sizeof(c), hipMemcpyDeviceToHost, sg->stream));
-    if (reset) HIP_TRY(hipMemsetAsync(sg->d_counts, 0, sizeof(c), sg->stream));
+    std::vector<unsigned long long> c(3 * kSgCountSlots);
+    HIP_TRY(hipMemcpyAsync(c.data(), sg->d_counts, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost,
+                           sg->stream));
+    if (reset) HIP_TRY(hipMemsetAsync(sg->d_counts, 0, sizeof(unsigned long long) * c.size(), sg->stream));
     HIP_TRY(hipStreamSynchronize(sg->stream));
-    out->yielding = c[0];
-    out->surrogate = c[1];
-    out->fallback = c[2];
+    uint64_t sum[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < kSgCountSlots; ++i) sum[k] += c[(size_t)k * kSgCountSlots + i];
+    out->yielding = sum[0];
+    out->surrogate = sum[1];
+    out->fallback = sum[2];
     return EMH_OK;
 }
 

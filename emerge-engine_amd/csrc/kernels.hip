@@ -42,6 +42,7 @@
 
 #include <climits>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "surrogate_kernels.hpp"
@@ -959,8 +960,8 @@ __device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCel
 // freshly uploaded state (MODE = P2G or Mass: no gather, no store).  ONLY:
 // the one constitutive model every particle uses, or -1 for any mix.
 // NACC trial state (F_trial as update_particle forms it, alpha and the
-// material's parameters) of each particle into the harvest buffer; records
-// beyond its capacity are counted and dropped.
+// material's parameters) of every stride-th offer into the harvest buffer;
+// offers beyond its capacity are counted and dropped.
 __device__ __forceinline__ void harvest_nacc(const SgDev& sg, const DevMat& m, const PState& s, float dt) {
     const bool want = sg.harvest != nullptr && m.model == kModelNacc;
     const uint64_t act = __ballot(want);
@@ -974,6 +975,8 @@ __device__ __forceinline__ void harvest_nacc(const SgDev& sg, const DevMat& m, c
     at = __shfl(at, (int)leader);
     if (!want) return;
     at += (unsigned long long)__popcll(act & ((1ull << lane) - 1ull));
+    if (at % sg.harvest_stride != 0ull) return;
+    at /= sg.harvest_stride;
     if (at >= sg.harvest_cap) return;
     float4* r = reinterpret_cast<float4*>(sg.harvest + at * 12ull);
     r[0] = make_float4(fin.a, fin.b, fin.c, fin.d);
@@ -985,14 +988,14 @@ __device__ __forceinline__ void harvest_nacc(const SgDev& sg, const DevMat& m, c
 // surrogate (nacc.rs:255-269 otherwise unchanged).  Every lane of the wave
 // calls this.
 template <int ONLY>
-__device__ __forceinline__ void surrogate_update(const SgDev& sg, const DevMat& m, bool valid, PState& s, float dt,
-                                                 float* s_io) {
+__device__ __forceinline__ void surrogate_update(const SgDev& sg, const SgLds& net, const DevMat& m, bool valid,
+                                                 PState& s, float dt, float* s_io) {
     const bool nacc = valid && m.model == kModelNacc;
     M2 fin = s.f;
     if (nacc && (m.flags & kMatNaccAdvance)) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
     M2 nf;
     float na;
-    sg_return_wave(m, sg, nacc, fin, s.lvs, nf, na, s_io);
+    sg_return_wave(m, sg, net, nacc, fin, s.lvs, nf, na, s_io);
     if (nacc) {
         s.f = nf;
         s.lvs = na;
@@ -1013,6 +1016,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
     constexpr bool SURR = (MODE & kModeSurrogate) != 0;
     constexpr int NCH = P2G ? kAccChannels : 1;
     __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgWaveLds : 1];
+    __shared__ std::conditional_t<SURR, SgLds, char> s_net;
     __shared__ float s_acc[NCH][kLdsCells];
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];
     __shared__ uint32_t s_perm[kChunk];
@@ -1040,6 +1044,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
         reset_red(c->red[seq_mod3(seq + 1u)]);
     }
     if (G2P && !N.active) return;
+    if constexpr (SURR) sg_stage(c->sg.pk, c->sg.bias, s_net);
     CtrlRed* red = &c->red[seq_mod3(seq)];
     const int R = (int)cfg.grid_res;
     const uint32_t cur = N.cur_grid;
@@ -1162,7 +1167,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
         }
         if constexpr (SURR) {
             // phase 2 with the learned return mapping (every lane of the wave)
-            surrogate_update<ONLY>(p.ctrl->sg, mat_of(p, s.mat), valid, s, dt, s_sg[threadIdx.x >> 6]);
+            surrogate_update<ONLY>(p.ctrl->sg, s_net, mat_of(p, s.mat), valid, s, dt, s_sg[threadIdx.x >> 6]);
             if (valid) {
                 cfl_of<ONLY>(p, s, speed, mdt);
                 any_dirty |= needs_projection(s, sp, cfg);

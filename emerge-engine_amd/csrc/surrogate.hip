@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace emh {
 namespace {
 
@@ -45,8 +47,20 @@ __device__ float pk_value(const float* __restrict__ w, int tile, int lane, int j
     return w[(l3 ? kSgW3 : kSgW2) + chained(kb) * kSgHid + it * 16 + c];
 }
 
-__global__ void k_sg_pack(const float* __restrict__ w, uint32_t* __restrict__ pk) {
+__global__ void k_sg_pack(const float* __restrict__ w, uint32_t* __restrict__ pk, float* __restrict__ bias) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (tile, lane, pair)
+    if (i < kSgBias) {
+        float b = 0.0f;
+        if (i < 64)
+            b = w[kSgB1 + i];
+        else if (i < 128)
+            b = w[kSgB2 + i - 64];
+        else if (i < 192)
+            b = w[kSgB3 + i - 128];
+        else if (i < 192 + kSgOut)
+            b = w[kSgB4 + i - 192];
+        bias[i] = b;
+    }
     if (i >= kPkTiles * 64 * 4) return;
     const int pair = i & 3, lane = (i >> 2) & 63, tile = i >> 8;
     pk[i] = sg_bf16_bits(pk_value(w, tile, lane, 2 * pair)) | (sg_bf16_bits(pk_value(w, tile, lane, 2 * pair + 1)) << 16);
@@ -67,9 +81,10 @@ __device__ DevMat record_mat(const float* r) {
     return m;
 }
 
-// Keeps the records that fail the yield check (the states the network
-// replaces the return mapping for) with positive singular values, labels
-// them with the plain return mapping, and accumulates the normalisation sums.
+// Keeps the records whose return is the projection onto the yield surface
+// (Case C, the states the network replaces the return mapping for) with
+// positive singular values, labels them with the plain return mapping, and
+// accumulates the normalisation sums.
 __global__ void k_sg_prepare(const float* __restrict__ rec, unsigned long long n_rec, float* __restrict__ xs,
                              float* __restrict__ ts, unsigned long long* __restrict__ n_out, double* __restrict__ sums) {
     const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +99,7 @@ __global__ void k_sg_prepare(const float* __restrict__ rec, unsigned long long n
         svd2(f, u, sg, vt);
         if (sg.y > 0.0f && sg.x > 0.0f) {
             const NaccTrial tr = nacc_trial(m, sg, r[4]);
-            if (tr.yield != 0) {
+            if (tr.yield == 3) {
                 float s1, s2, ao;
                 nacc_return(m, tr, r[4], s1, s2, ao);
                 x[0] = logf(sg.x);
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(kSgBlock) void k_sg_fwd_bwd(SgBatch b, SgTrainBufs 
     }
     const bf16x8 xb = sg_pack8(x[0], x[1], x[2], x[3], x[4], x[5], 0.0f, 0.0f);
     f32x4 zs[3][4];
-    const f32x4 out = sg_forward(b.pk, b.prm, xb, zs);
+    const f32x4 out = sg_forward(b.pk, b.bias, xb, zs);
     f32x4 d_out = {0.0f, 0.0f, 0.0f, 0.0f};
     float se = 0.0f;
     if (valid && g == 0) {
@@ -274,26 +289,31 @@ __global__ void k_nacc_return_plain(DevMat m, const float4* __restrict__ f, cons
     ao[i] = na;
 }
 
-__global__ __launch_bounds__(kSgBlock) void k_nacc_return_surrogate(DevMat m, const float4* __restrict__ f,
+__global__ __launch_bounds__(kSgBlock, 4) void k_nacc_return_surrogate(DevMat m, const float4* __restrict__ f,
                                                                     const float* __restrict__ alpha,
                                                                     unsigned long long n, float4* __restrict__ fo,
                                                                     float* __restrict__ ao, SgDev sg) {
     __shared__ float s_io[kSgBlock / 64][kSgWaveLds];
-    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    M2 fin{1.0f, 0.0f, 0.0f, 1.0f};
-    float a = 0.0f;
-    if (valid) {
-        const float4 q = f[i];
-        fin = M2{q.x, q.y, q.z, q.w};
-        a = alpha[i];
-    }
-    M2 nf;
-    float na;
-    sg_return_wave(m, sg, valid, fin, a, nf, na, s_io[threadIdx.x >> 6]);
-    if (valid) {
-        fo[i] = make_float4(nf.a, nf.b, nf.c, nf.d);
-        ao[i] = na;
+    __shared__ SgLds s_net;
+    sg_stage(sg.pk, sg.bias, s_net);  // once per workgroup: the grid strides over the particles
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const unsigned long long i = b0 + threadIdx.x;
+        const bool valid = i < n;
+        M2 fin{1.0f, 0.0f, 0.0f, 1.0f};
+        float a = 0.0f;
+        if (valid) {
+            const float4 q = f[i];
+            fin = M2{q.x, q.y, q.z, q.w};
+            a = alpha[i];
+        }
+        M2 nf;
+        float na;
+        sg_return_wave(m, sg, s_net, valid, fin, a, nf, na, s_io[threadIdx.x >> 6]);
+        if (valid) {
+            fo[i] = make_float4(nf.a, nf.b, nf.c, nf.d);
+            ao[i] = na;
+        }
     }
 }
 
@@ -307,7 +327,7 @@ __global__ __launch_bounds__(kSgBlock) void k_sg_infer(const float* __restrict__
     float x[kSgIn] = {};
     if (s < n && g == 0)
         for (int k = 0; k < kSgIn; ++k) x[k] = (xs[s * kSgIn + k] - sg.norm[kSgInMu + k]) / sg.norm[kSgInSd + k];
-    const f32x4 out = sg_forward(sg.pk, sg.prm, sg_pack8(x[0], x[1], x[2], x[3], x[4], x[5], 0.0f, 0.0f), nullptr);
+    const f32x4 out = sg_forward(sg.pk, sg.bias, sg_pack8(x[0], x[1], x[2], x[3], x[4], x[5], 0.0f, 0.0f), nullptr);
     if (s < n && g == 0)
         for (int k = 0; k < kSgOut; ++k)
             ys[s * kSgOut + k] = xs[s * kSgIn + k] + (out[k] * sg.norm[kSgOutSd + k] + sg.norm[kSgOutMu + k]);
@@ -315,8 +335,8 @@ __global__ __launch_bounds__(kSgBlock) void k_sg_infer(const float* __restrict__
 
 }  // namespace
 
-hipError_t sg_launch_pack(const float* w, uint4* pk, hipStream_t st) {
-    k_sg_pack<<<div_up_u(kPkTiles * 64 * 4, kSgBlock), kSgBlock, 0, st>>>(w, reinterpret_cast<uint32_t*>(pk));
+hipError_t sg_launch_pack(const float* w, uint4* pk, float* bias, hipStream_t st) {
+    k_sg_pack<<<div_up_u(kPkTiles * 64 * 4, kSgBlock), kSgBlock, 0, st>>>(w, reinterpret_cast<uint32_t*>(pk), bias);
     return hipGetLastError();
 }
 hipError_t sg_launch_prepare(const float* rec, unsigned long long n_rec, float* xs, float* ts,
@@ -326,15 +346,15 @@ hipError_t sg_launch_prepare(const float* rec, unsigned long long n_rec, float* 
     return hipGetLastError();
 }
 hipError_t sg_launch_train_step(const SgBatch& b, const SgTrainBufs& o, float* w, float* m, float* v, float* grad,
-                                const SgAdam* adam, hipStream_t st) {
+                                const SgAdam* adam, float* bias, hipStream_t st) {
     const uint32_t groups = b.nb / 16u;
     k_sg_fwd_bwd<<<div_up_u(groups, kSgBlock / 64), kSgBlock, 0, st>>>(b, o);
     const uint32_t waves = (uint32_t)kSgTiles * (b.nb / kSgSlice);
     k_sg_wgrad<<<div_up_u(waves, kSgBlock / 64), kSgBlock, 0, st>>>(o, b.nb, grad);
     if (adam) {
         k_sg_adam<<<div_up_u(kSgParams, kSgBlock), kSgBlock, 0, st>>>(w, m, v, grad, *adam);
-        k_sg_pack<<<div_up_u(kPkTiles * 64 * 4, kSgBlock), kSgBlock, 0, st>>>(w, reinterpret_cast<uint32_t*>(
-                                                                                      const_cast<uint4*>(b.pk)));
+        k_sg_pack<<<div_up_u(kPkTiles * 64 * 4, kSgBlock), kSgBlock, 0, st>>>(
+            w, reinterpret_cast<uint32_t*>(const_cast<uint4*>(b.pk)), bias);
     }
     return hipGetLastError();
 }
@@ -347,8 +367,9 @@ hipError_t sg_launch_return_plain(const DevMat& m, const float4* f, const float*
 hipError_t sg_launch_return_surrogate(const DevMat& m, const float4* f, const float* alpha, unsigned long long n,
                                       float4* fo, float* ao, const SgDev& sg, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    k_nacc_return_surrogate<<<(unsigned)((n + kSgBlock - 1) / kSgBlock), kSgBlock, 0, st>>>(m, f, alpha, n, fo, ao,
-                                                                                            sg);
+    // about four resident workgroups per CU (256 CUs), each staging the network once
+    const unsigned long long blocks = std::min<unsigned long long>((n + kSgBlock - 1) / kSgBlock, 1024ull);
+    k_nacc_return_surrogate<<<(unsigned)blocks, kSgBlock, 0, st>>>(m, f, alpha, n, fo, ao, sg);
     return hipGetLastError();
 }
 hipError_t sg_launch_infer(const float* xs, unsigned long long n, float* ys, const SgDev& sg, hipStream_t st) {

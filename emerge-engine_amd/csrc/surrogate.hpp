@@ -37,6 +37,9 @@ constexpr int kSgInMu = 0, kSgInSd = 6, kSgOutMu = 12, kSgOutSd = 15, kSgZMax = 
 // k-blocks), L4 (2 k-blocks); backward-data L4^T (4 in tiles), L3^T / L2^T
 // (4 in tiles x 2 k-blocks)
 constexpr int kPkF1 = 0, kPkF2 = 4, kPkF3 = 12, kPkF4 = 20, kPkB4 = 22, kPkB3 = 26, kPkB2 = 34, kPkTiles = 42;
+constexpr int kPkFwdTiles = kPkB4;  // the forward pass reads tiles 0..21
+// compact bias block the forward pass reads: b1 at 0, b2 at 64, b3 at 128, b4 at 192
+constexpr int kSgBias = 196;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -100,7 +103,7 @@ __device__ __forceinline__ void sg_hidden(const uint4* __restrict__ pk, int t0, 
 // other slot zero).  Returns the output tile: lanes 0..15 hold outputs 0..2
 // of particle l in elements 0..2.  With zs, the pre-activations of the three
 // hidden layers are kept for the backward pass.
-__device__ __forceinline__ f32x4 sg_forward(const uint4* __restrict__ pk, const float* __restrict__ prm, bf16x8 x,
+__device__ __forceinline__ f32x4 sg_forward(const uint4* __restrict__ pk, const float* __restrict__ bias, bf16x8 x,
                                             f32x4 (*zs)[4]) {
     const int g = (threadIdx.x & 63) >> 4;
     f32x4 z[4], h[4];
@@ -109,7 +112,7 @@ __device__ __forceinline__ f32x4 sg_forward(const uint4* __restrict__ pk, const 
     for (int ot = 0; ot < 4; ++ot) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         acc = sg_mfma(sg_tile(pk, kPkF1 + ot), x, acc);
-        const float4 bb = *reinterpret_cast<const float4*>(prm + kSgB1 + ot * 16 + 4 * g);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * g);
         z[ot] = acc + f32x4{bb.x, bb.y, bb.z, bb.w};
     }
 #pragma unroll
@@ -123,8 +126,8 @@ __device__ __forceinline__ f32x4 sg_forward(const uint4* __restrict__ pk, const 
 #pragma unroll
             for (int r = 0; r < 4; ++r) h[t][r] = sg_silu(z[t][r]);
         sg_chain(h, b);
-        if (layer == 0) sg_hidden(pk, kPkF2, prm + kSgB2, b, z);
-        if (layer == 1) sg_hidden(pk, kPkF3, prm + kSgB3, b, z);
+        if (layer == 0) sg_hidden(pk, kPkF2, bias + 64, b, z);
+        if (layer == 1) sg_hidden(pk, kPkF3, bias + 128, b, z);
     }
     f32x4 o = {0.0f, 0.0f, 0.0f, 0.0f};
     o = sg_mfma(sg_tile(pk, kPkF4), b[0], o);
@@ -132,7 +135,48 @@ __device__ __forceinline__ f32x4 sg_forward(const uint4* __restrict__ pk, const 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int row = 4 * g + r;
-        o[r] = row < kSgOut ? o[r] + prm[kSgB4 + row] : 0.0f;
+        o[r] = row < kSgOut ? o[r] + bias[192 + row] : 0.0f;
+    }
+    return o;
+}
+
+// The same forward pass without saved pre-activations, the two hidden
+// layers in a rolled loop: each layer's operand loads then stay behind the
+// previous layer's arithmetic instead of all being hoisted (22 tiles would
+// hold 88 registers).
+__device__ __forceinline__ f32x4 sg_forward_infer(const uint4* __restrict__ pk, const float* __restrict__ bias,
+                                                  bf16x8 x) {
+    const int g = (threadIdx.x & 63) >> 4;
+    f32x4 z[4], h[4];
+    bf16x8 b[2];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = sg_mfma(sg_tile(pk, kPkF1 + ot), x, acc);
+        const float4 bb = *reinterpret_cast<const float4*>(bias + ot * 16 + 4 * g);
+        z[ot] = acc + f32x4{bb.x, bb.y, bb.z, bb.w};
+    }
+#pragma unroll 1
+    for (int layer = 0; layer < 2; ++layer) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h[t][r] = sg_silu(z[t][r]);
+        sg_chain(h, b);
+        sg_hidden(pk, kPkF2 + 8 * layer, bias + 64 * (layer + 1), b, z);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) h[t][r] = sg_silu(z[t][r]);
+    sg_chain(h, b);
+    f32x4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+    o = sg_mfma(sg_tile(pk, kPkF4), b[0], o);
+    o = sg_mfma(sg_tile(pk, kPkF4 + 1), b[1], o);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * g + r;
+        o[r] = row < kSgOut ? o[r] + bias[192 + row] : 0.0f;
     }
     return o;
 }
@@ -166,15 +210,31 @@ __device__ __forceinline__ void sg_backward(const uint4* __restrict__ pk, const 
     }
 }
 
+// The forward tiles and the bias block, staged in LDS by the whole workgroup
+// (ends with a barrier): inference kernels read them from there.
+struct SgLds {
+    uint4 pk[kPkFwdTiles * 64];
+    float bias[kSgBias];
+};
+__device__ __forceinline__ void sg_stage(const uint4* __restrict__ pk, const float* __restrict__ bias, SgLds& s) {
+    for (int i = threadIdx.x; i < kPkFwdTiles * 64; i += blockDim.x) s.pk[i] = pk[i];
+    for (int i = threadIdx.x; i < kSgBias; i += blockDim.x) s.bias[i] = bias[i];
+    __syncthreads();
+}
+
 // Host/device view of a surrogate's device state used by the substep kernel.
 struct SgDev {
     const uint4* pk;    // packed bf16 operand tiles (kPkTiles * 64 uint4)
-    const float* prm;   // FP32 master parameters (biases are read from here)
+    const float* bias;  // compact bias block (kSgBias floats)
     const float* norm;  // normalisation block
     float* harvest;     // NACC trial-state records (12 floats each), or null
-    unsigned long long* harvest_count;
-    unsigned long long harvest_cap;
-    unsigned long long* counts;  // [0] gated (yielding) trial states, [1] surrogate returns, [2] fallbacks
+    unsigned long long* harvest_count;  // trial states offered
+    unsigned long long harvest_cap;     // records the buffer holds
+    unsigned long long harvest_stride;  // offer k is recorded (as record k / stride) when k % stride == 0
+    // [c * kSgCountSlots + slot]: c = 0 yielding trial states, 1 surrogate
+    // returns, 2 plain fallbacks; a workgroup adds into slot blockIdx % kSgCountSlots
+    unsigned long long* counts;
 };
+constexpr int kSgCountSlots = 256;
 
 }  // namespace emh

@@ -15,14 +15,18 @@ constexpr int kSgWaveLds = 64 * 12;
 
 // NaccMaterial::project (nacc.rs:129-223) for the 64 particles of a wave,
 // with the exact yield-function check as the gate: a state inside the yield
-// surface keeps F; a state outside it goes through the network when its
-// singular values are positive and every normalised input lies within
-// z_max, and through the plain return mapping (nacc_return) otherwise.
+// surface keeps F; the tension and compression caps (Cases A and B, closed
+// forms of one sqrt and one pow) keep the plain return; a state outside the
+// yield surface between them (Case C, the projection onto the surface with
+// the hardening update) goes through the network when its singular values
+// are positive and every normalised input lies within z_max, and through the
+// plain return mapping (nacc_return) otherwise.
 // Every lane of the wave calls this (it runs MFMAs over compacted lanes and
 // synchronises the workgroup); `valid` marks the lanes with a particle.
-// s_io: this wave's kSgWaveLds floats of LDS.
-__device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg, bool valid, const M2& fin,
-                                               float alpha, M2& fo, float& ao, float* s_io) {
+// net: the network staged in LDS (sg_stage); s_io: this wave's kSgWaveLds
+// floats of LDS.
+__device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg, const SgLds& net, bool valid,
+                                               const M2& fin, float alpha, M2& fo, float& ao, float* s_io) {
     const uint32_t lane = threadIdx.x & 63u, c = lane & 15u, g = lane >> 4;
     M2 u{1.0f, 0.0f, 0.0f, 1.0f}, vt{1.0f, 0.0f, 0.0f, 1.0f};
     NaccTrial tr{};
@@ -33,7 +37,7 @@ __device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg,
         float2 sv;
         svd2(fin, u, sv, vt);
         tr = nacc_trial(m, sv, alpha);
-        if (tr.yield != 0 && sv.y > 0.0f) {  // svd2 orders s1 >= |s2|
+        if (tr.yield == 3 && sv.y > 0.0f) {  // svd2 orders s1 >= |s2|
             x[0] = logf(sv.x);
             x[1] = logf(sv.y);
             x[2] = alpha;
@@ -64,7 +68,7 @@ __device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg,
 #pragma unroll
             for (int k = 0; k < kSgIn; ++k) z[k] = s_io[slot * 12 + k];
         }
-        const f32x4 o = sg_forward(sg.pk, sg.prm, sg_pack8(z[0], z[1], z[2], z[3], z[4], z[5], 0.0f, 0.0f), nullptr);
+        const f32x4 o = sg_forward_infer(net.pk, net.bias, sg_pack8(z[0], z[1], z[2], z[3], z[4], z[5], 0.0f, 0.0f));
         if (mine) {
 #pragma unroll
             for (int k = 0; k < kSgOut; ++k) s_io[slot * 12 + 8 + k] = o[k];
@@ -88,12 +92,13 @@ __device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg,
             fo = usv(u, s1, s2, vt);
         }
     }
-    if (sg.counts) {
+    if (sg.counts) {  // spread over slots: one address per counter would serialise every wave's atomics
         const uint32_t yielding = (uint32_t)__popcll(__ballot(valid && tr.yield != 0));
         if (lane == 0 && yielding) {
-            atomicAdd(&sg.counts[0], (unsigned long long)yielding);
-            atomicAdd(&sg.counts[1], (unsigned long long)cnt);
-            atomicAdd(&sg.counts[2], (unsigned long long)(yielding - cnt));
+            unsigned long long* c = sg.counts + (blockIdx.x % kSgCountSlots);
+            atomicAdd(&c[0], (unsigned long long)yielding);
+            atomicAdd(&c[kSgCountSlots], (unsigned long long)cnt);
+            atomicAdd(&c[2 * kSgCountSlots], (unsigned long long)(yielding - cnt));
         }
     }
 }
@@ -107,7 +112,7 @@ struct SgBatch {
     uint32_t nb, n_valid;  // nb: multiple of 256
     float inv_n;           // 1 / n_valid
     const uint4* pk;
-    const float* prm;
+    const float* bias;
     const float* norm;
 };
 // Per-slot operands of the weight-gradient GEMMs ([slot][feature]) and the
@@ -120,13 +125,13 @@ struct SgAdam {
     float lr, beta1, beta2, eps, c1, c2;  // c1 = 1 / (1 - beta1^t), c2 = 1 / (1 - beta2^t)
 };
 
-hipError_t sg_launch_pack(const float* w, uint4* pk, hipStream_t st);
+hipError_t sg_launch_pack(const float* w, uint4* pk, float* bias, hipStream_t st);
 hipError_t sg_launch_prepare(const float* rec, unsigned long long n_rec, float* xs, float* ts,
                              unsigned long long* n_out, double* sums, hipStream_t st);
 // fwd/bwd + weight gradients (accumulated into grad); with adam, the update
 // and the re-packing of the operands as well.
 hipError_t sg_launch_train_step(const SgBatch& b, const SgTrainBufs& o, float* w, float* m, float* v, float* grad,
-                                const SgAdam* adam, hipStream_t st);
+                                const SgAdam* adam, float* bias, hipStream_t st);
 hipError_t sg_launch_return_plain(const DevMat& m, const float4* f, const float* alpha, unsigned long long n,
                                   float4* fo, float* ao, hipStream_t st);
 hipError_t sg_launch_return_surrogate(const DevMat& m, const float4* f, const float* alpha, unsigned long long n,

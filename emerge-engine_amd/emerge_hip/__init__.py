@@ -253,7 +253,10 @@ def load_library() -> ctypes.CDLL:
         "emh_surrogate_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "emh_surrogate_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "emh_sim_set_surrogate_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
-                                                      ctypes.c_uint64]),
+                                                      ctypes.c_uint64, ctypes.c_uint32]),
+        "emh_surrogate_eval_records": (ctypes.c_int, [ctypes.c_void_p, P(CMaterialParams), P(CMaterialExt),
+                                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32,
+                                                      P(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]),
         "emh_surrogate_harvest_count": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]),
         "emh_surrogate_build_training_set": (ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint64)]),
         "emh_surrogate_get_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
@@ -874,12 +877,12 @@ class Simulation:
         return d
 
     def set_surrogate_mode(self, surrogate: Optional["NaccSurrogate"], mode: str,
-                           harvest_capacity: int = 0) -> None:
+                           harvest_capacity: int = 0, harvest_stride: int = 1) -> None:
         """NACC return mapping of the following substeps: "plain", "harvest" (plain,
-        recording every trial state into the surrogate's buffer) or "surrogate"."""
+        recording every stride-th trial state into the surrogate's buffer) or "surrogate"."""
         code = {"plain": 0, "harvest": 1, "surrogate": 2}[mode]
         _check(load_library().emh_sim_set_surrogate_mode(self._h, surrogate._h if surrogate else None, code,
-                                                          int(harvest_capacity)))
+                                                          int(harvest_capacity), int(harvest_stride)))
         self._surrogate = surrogate  # keeps it alive while the sim uses it
 
     def set_gravity(self, gravity) -> None:
@@ -966,6 +969,17 @@ class NaccSurrogate:
         out = np.zeros((n, 12), dtype=np.float32)
         _check(load_library().emh_surrogate_get_records(self._h, first, n, out.ctypes.data))
         return out
+
+    def eval_records(self, material: "NaccMaterial", first: int, n: int, use_surrogate: bool, reps: int = 1,
+                     outputs: bool = True):
+        """Batch return mapping of harvested records on the device; (ms per launch, F [n, 4], alpha [n])."""
+        ms = ctypes.c_float(0.0)
+        f = np.zeros((n, 4), dtype=np.float32) if outputs else None
+        a = np.zeros(n, dtype=np.float32) if outputs else None
+        _check(load_library().emh_surrogate_eval_records(
+            self._h, ctypes.byref(material.params), ctypes.byref(material.ext), first, n, int(use_surrogate), reps,
+            ctypes.byref(ms), None if f is None else f.ctypes.data, None if a is None else a.ctypes.data))
+        return ms.value, f, a
 
     def build_training_set(self) -> int:
         n = ctypes.c_uint64(0)

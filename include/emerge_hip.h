@@ -440,10 +440,12 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
  * fields -- to the projected (ln s1', ln s2', alpha').  Weights are BF16 on
  * MFMA with FP32 accumulation; training (forward, backward, Adam) runs on
  * the device.  Each particle is gated by the exact yield-function check:
- * a state inside the yield surface keeps F, a state outside it goes through
- * the network unless its inputs leave the training range (more than z_max
- * standard deviations) or s2 <= 0, in which case the plain return mapping
- * runs instead.
+ * a state inside the yield surface keeps F; the tension and compression
+ * caps (the closed-form Cases A and B of nacc.rs) keep the plain return; a
+ * state projected onto the yield surface (Case C) goes through the network
+ * unless its inputs leave the training range (more than z_max standard
+ * deviations) or s2 <= 0, in which case the plain return mapping runs
+ * instead.
  * ------------------------------------------------------------------- */
 typedef struct emh_surrogate emh_surrogate;
 
@@ -464,7 +466,7 @@ typedef struct emh_surrogate_train_options {
 typedef struct emh_surrogate_counts {
     uint64_t yielding;   /* trial states that failed the yield check */
     uint64_t surrogate;  /* ... returned through the network */
-    uint64_t fallback;   /* ... returned by the plain mapping (out of range, s2 <= 0) */
+    uint64_t fallback;   /* ... returned by the plain mapping (caps A / B, out of range, s2 <= 0) */
 } emh_surrogate_counts;
 
 /* Glorot-uniform weights from `seed`, zero biases, identity normalisation. */
@@ -477,17 +479,26 @@ int emh_surrogate_get_params(emh_surrogate* sg, float* params, float* norm);
 int emh_surrogate_set_params(emh_surrogate* sg, const float* params, const float* norm);
 /* mode 0: plain return mapping; 1: plain, and record every NACC particle's
  * trial state (F_trial as update_particle forms it, alpha, material) into
- * sg's buffer of harvest_capacity records; 2: return mapping through sg.
+ * sg's buffer of harvest_capacity records (every harvest_stride-th trial
+ * state offered; 0 or 1: every one); 2: return mapping through sg.
  * Modes 1 and 2 need a scene whose particles are all NACC (the next step
  * fails with EMH_ERR_UNSUPPORTED otherwise).  sg must outlive its use by the
  * sim (or be detached with mode 0). */
-int emh_sim_set_surrogate_mode(emh_sim* sim, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity);
+int emh_sim_set_surrogate_mode(emh_sim* sim, emh_surrogate* sg, uint32_t mode, uint64_t harvest_capacity,
+                               uint32_t harvest_stride);
 int emh_surrogate_harvest_count(emh_surrogate* sg, uint64_t* offered, uint64_t* stored);
 /* Harvested records [first, first + n), 12 floats each: F_trial (column-major
  * a, b, c, d), alpha, mu, kappa, M, beta, xi, flags (uint32 bits), 0. */
 int emh_surrogate_get_records(emh_surrogate* sg, uint64_t first, uint64_t n, float* out);
-/* Training set from the harvested records: those failing the yield check
- * with s2 > 0, labelled by the plain return mapping; sets the normalisation. */
+/* The batch return mapping (emh_nacc_return_batch) of harvested records
+ * [first, first + n), plain or through sg; outputs copied to the host
+ * arrays when non-NULL (f_out [n][4], alpha_out [n]). */
+int emh_surrogate_eval_records(emh_surrogate* sg, const emh_material_params* p, const emh_material_ext* ext,
+                               uint64_t first, uint64_t n, int use_surrogate, uint32_t reps, float* ms_per_rep,
+                               float* f_out, float* alpha_out);
+/* Training set from the harvested records: those projected onto the yield
+ * surface (Case C) with s2 > 0, labelled by the plain return mapping; sets
+ * the normalisation. */
 int emh_surrogate_build_training_set(emh_surrogate* sg, uint64_t* n_samples);
 /* Host samples x[n][6] (raw inputs), y[n][3] (raw targets); sets the normalisation. */
 int emh_surrogate_set_training_set(emh_surrogate* sg, const float* x, const float* y, uint64_t n);

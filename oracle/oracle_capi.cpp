@@ -421,6 +421,31 @@ void orc_nacc_project(const emh_material_params* p, const emh_material_ext* e, c
     *alpha_out = ao;
 }
 
+// NaccMaterial::project (nacc.rs:129-223) over n trial states (f [n][4]
+// column-major, alpha [n]) in precision 32 or 64; results widened to double.
+void orc_nacc_project_batch(const emh_material_params* p, const emh_material_ext* e, int precision, size_t n,
+                            const float* f, const float* alpha, double* f_out, double* alpha_out) {
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        Material<T> m = Material<T>::from_params(*p, e);
+        for (size_t i = 0; i < n; ++i) {
+            const float* q = f + 4 * i;
+            M2<T> fo;
+            T ao;
+            m.nacc_project(M2<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}, (T)alpha[i], fo, ao);
+            f_out[4 * i + 0] = (double)fo.m00;
+            f_out[4 * i + 1] = (double)fo.m10;
+            f_out[4 * i + 2] = (double)fo.m01;
+            f_out[4 * i + 3] = (double)fo.m11;
+            alpha_out[i] = (double)ao;
+        }
+    };
+    if (precision == 64)
+        run(0.0);
+    else
+        run(0.0f);
+}
+
 void orc_slip_velocity(uint32_t thickness, uint32_t cell_index, uint32_t grid_res, float v[2]) {
     uint32_t hi = grid_res - (thickness + 1);
     uint32_t x = cell_index / grid_res, y = cell_index % grid_res;

@@ -89,6 +89,8 @@ def load() -> ctypes.CDLL:
         "orc_timestep_bound": (f, [P(CMaterialParams), P(CMaterialExt), f, f, f, f, f]),
         "orc_dp_alpha": (f, [P(CMaterialParams), f]),
         "orc_nacc_project": (None, [P(CMaterialParams), P(CMaterialExt), P(f), f, P(f), P(f)]),
+        "orc_nacc_project_batch": (None, [P(CMaterialParams), P(CMaterialExt), ctypes.c_int, ctypes.c_size_t,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "orc_slip_velocity": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P(f)]),
         "orc_lame_from_young": (None, [f, f, P(f), P(f)]),
         "orc_to_radians": (f, [f]),
@@ -321,6 +323,17 @@ def nacc_project(m: Material, f, alpha: float):
     ao = ctypes.c_float()
     load().orc_nacc_project(ctypes.byref(m.params), ctypes.byref(m.ext), _f4(f), alpha, fo, ctypes.byref(ao))
     return np.array(fo[:], np.float32), ao.value
+
+
+def nacc_project_batch(m: Material, f: np.ndarray, alpha: np.ndarray, precision: int = 64):
+    """NaccMaterial::project over trial states f [n, 4] (column-major), alpha [n]; float64 results."""
+    f = np.ascontiguousarray(f, dtype=np.float32).reshape(-1, 4)
+    alpha = np.ascontiguousarray(alpha, dtype=np.float32).reshape(-1)
+    fo = np.zeros((len(f), 4))
+    ao = np.zeros(len(f))
+    load().orc_nacc_project_batch(ctypes.byref(m.params), ctypes.byref(m.ext), precision, len(f), f.ctypes.data,
+                                  alpha.ctypes.data, fo.ctypes.data, ao.ctypes.data)
+    return fo, ao
 
 
 BOUNDARY_KINDS = {"slip": 0, "friction": 1, "heightmap": 2, "predictive": 3}

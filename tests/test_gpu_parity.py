@@ -37,6 +37,23 @@ CHAOTIC = {"quickstart_impact"}
 # alone.  The GPU may flip at most a few times as many particles as the f32
 # oracle does in any of its particle orders, plus 2 % of the particles.
 FLIP_FIELDS = {"mui_column": ("q", 1e-3)}
+# Scenes whose plastic branch is a threshold test (Von Mises: yield when the
+# trial deviatoric stress exceeds the current yield stress, von_mises.rs:
+# 108-141).  A particle sitting on the threshold takes either branch
+# depending on summation order; once it flips, its v, C, F and q leave the
+# f32 spread for good (one flip in about a hundred GPU runs of ductile_drop
+# moved v by 1.8e-3 against a spread of 4e-5; 60 oracle orders and 40 GPU
+# runs without a flip stay below 1e-4).  After N steps, at most
+# max(3, 2 %) particles may exceed the bound, each staying finite and within
+# 100 x the bound; every other particle must meet the bound.
+BRANCH_SCENES = {"ductile_drop"}
+
+
+def _per_particle(a, ref, field):
+    """Per-particle errors on the field's own scale (scenes.rel_errors' norm)."""
+    floors = {"x": 1.0, "v": 1e-3, "C": 1e-3, "F": 1.0, "Jp": 1.0, "q": 1e-3, "lvs": 1e-3, "density": 1e-6}
+    d = np.abs(np.asarray(a, np.float64) - ref).reshape(len(ref), -1).max(axis=1)
+    return d / max(float(np.max(np.abs(ref))) if ref.size else 0.0, floors[field])
 
 
 def _flips(a, ref, tol):
@@ -108,7 +125,18 @@ def test_scene_parity(name, oracle, capsys):
             for f in ONE_STEP_TOL:
                 if flip is not None and f == flip[0]:
                     continue
-                assert e_g64[f] <= RATIO * e_3264[f] + FLOOR, (name, s, f, e_g64[f], e_3264[f])
+                bound = RATIO * e_3264[f] + FLOOR
+                if k > 0 and name in BRANCH_SCENES:
+                    e = _per_particle(g[k][f], o64[k][f], f)
+                    allowed = max(3, int(np.ceil(0.02 * len(e))))
+                    over = np.sort(e)[::-1]
+                    with capsys.disabled():
+                        if over[0] > bound:
+                            print(f"  {f}: {int(np.sum(e > bound))} particle(s) beyond {bound:.3e}, largest {over[0]:.3e}")
+                    assert np.all(np.isfinite(e)) and over[0] <= 100.0 * bound, (name, s, f, over[0], bound)
+                    assert over[min(allowed, len(over) - 1)] <= bound, (name, s, f, over[:allowed + 1], bound)
+                    continue
+                assert e_g64[f] <= bound, (name, s, f, e_g64[f], e_3264[f])
     # invariants that hold for every scene after N steps
     p = gsim.particles()
     assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["deformation_gradient"]))
