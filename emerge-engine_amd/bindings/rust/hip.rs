@@ -193,6 +193,43 @@ extern "C" {
     fn emh_sim_step(sim: *mut EmhSim, n_steps: u32) -> c_int;
     fn emh_sim_stats(sim: *const EmhSim, out: *mut EmhStepStats) -> c_int;
     fn emh_sim_get_grid(sim: *mut EmhSim, mass: *mut f32, momentum: *mut f32, n_cells: usize) -> c_int;
+    // learned NACC return mapping
+    fn emh_surrogate_create(seed: u64, out: *mut *mut EmhSurrogate) -> c_int;
+    fn emh_surrogate_destroy(sg: *mut EmhSurrogate) -> c_int;
+    fn emh_sim_set_surrogate_mode(sim: *mut EmhSim, sg: *mut EmhSurrogate, mode: u32, harvest_capacity: u64,
+                                  harvest_stride: u32) -> c_int;
+    fn emh_surrogate_build_training_set(sg: *mut EmhSurrogate, n_samples: *mut u64) -> c_int;
+    fn emh_surrogate_train(sg: *mut EmhSurrogate, o: *const EmhTrainOptions, epoch_loss: *mut f32, n: u32) -> c_int;
+    fn emh_surrogate_get_counts(sg: *mut EmhSurrogate, out: *mut EmhSurrogateCounts, reset: c_int) -> c_int;
+}
+
+#[repr(C)]
+pub struct EmhSurrogate {
+    _opaque: [u8; 0],
+}
+
+/// `emh_surrogate_train_options`.
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct EmhTrainOptions {
+    pub epochs: u32,
+    pub batch: u32,
+    pub lr: f32,
+    pub lr_final: f32,
+    pub beta1: f32,
+    pub beta2: f32,
+    pub eps: f32,
+    pub pad: u32,
+    pub seed: u64,
+    pub max_steps: u64,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default)]
+pub struct EmhSurrogateCounts {
+    pub yielding: u64,
+    pub surrogate: u64,
+    pub fallback: u64,
 }
 
 /// A failed C-ABI call: the status code and the library's message.
@@ -320,5 +357,55 @@ impl HipSimulation {
         let mut p = vec![[0f32; 2]; n];
         check(unsafe { emh_sim_get_grid(self.raw, m.as_mut_ptr(), p.as_mut_ptr() as *mut f32, n) })?;
         Ok((m, p))
+    }
+}
+
+/// Learned NACC return mapping on the GPU (no counterpart in the crate): harvest
+/// trial states from a plain run, train, then let a NACC-only simulation use it.
+pub struct NaccSurrogate {
+    raw: *mut EmhSurrogate,
+}
+
+impl Drop for NaccSurrogate {
+    fn drop(&mut self) {
+        unsafe {
+            emh_surrogate_destroy(self.raw);
+        }
+    }
+}
+
+impl NaccSurrogate {
+    pub fn new(seed: u64) -> Result<Self, HipError> {
+        let mut raw = std::ptr::null_mut();
+        check(unsafe { emh_surrogate_create(seed, &mut raw) })?;
+        Ok(NaccSurrogate { raw })
+    }
+    /// Plain return mapping while recording every `stride`-th trial state (at most `capacity`).
+    pub fn harvest_from(&mut self, sim: &mut HipSimulation, capacity: u64, stride: u32) -> Result<(), HipError> {
+        check(unsafe { emh_sim_set_surrogate_mode(sim.raw, self.raw, 1, capacity, stride) })
+    }
+    /// Returns the number of training samples (Case C states).
+    pub fn build_training_set(&mut self) -> Result<u64, HipError> {
+        let mut n = 0u64;
+        check(unsafe { emh_surrogate_build_training_set(self.raw, &mut n) })?;
+        Ok(n)
+    }
+    pub fn train(&mut self, o: &EmhTrainOptions) -> Result<Vec<f32>, HipError> {
+        let mut loss = vec![0f32; o.epochs as usize];
+        check(unsafe { emh_surrogate_train(self.raw, o, loss.as_mut_ptr(), o.epochs) })?;
+        Ok(loss)
+    }
+    /// The simulation's NACC return mapping goes through this network from the next step.
+    /// The simulation keeps a pointer: call `detach` (or drop the simulation) before dropping `self`.
+    pub fn attach(&self, sim: &mut HipSimulation) -> Result<(), HipError> {
+        check(unsafe { emh_sim_set_surrogate_mode(sim.raw, self.raw, 2, 0, 1) })
+    }
+    pub fn detach(sim: &mut HipSimulation) -> Result<(), HipError> {
+        check(unsafe { emh_sim_set_surrogate_mode(sim.raw, std::ptr::null_mut(), 0, 0, 1) })
+    }
+    pub fn counts(&self, reset: bool) -> Result<EmhSurrogateCounts, HipError> {
+        let mut c = EmhSurrogateCounts::default();
+        check(unsafe { emh_surrogate_get_counts(self.raw, &mut c, reset as c_int) })?;
+        Ok(c)
     }
 }
