@@ -873,6 +873,13 @@ struct emh_sim {
             for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode));
             extra += more;
         }
+        return publish(c, st, profiling);
+    }
+
+    // The records of the last finished step (step.rs:51-59) from the control
+    // block the loop left; st = c.st[seq & 1].
+    // timed: fold the live-timing events of this drain into the profile.
+    int publish(const Ctrl& c, const CtrlState& st, bool timed) {
         // the last step: finished but not yet recorded if no launch followed it
         uint32_t subs, step;
         float last_dt, dropped;
@@ -887,7 +894,7 @@ struct emh_sim {
             last_dt = st.done_dt;
             dropped = st.done_dropped;
         }
-        if (profiling) {
+        if (timed) {
             float a = 0.0f, b = 0.0f;
             HIP_TRY(hipEventElapsedTime(&a, prof_ev[0], prof_ev[1]));
             HIP_TRY(hipEventElapsedTime(&b, prof_ev[1], prof_ev[2]));
@@ -2298,6 +2305,198 @@ int emh_nacc_return_batch(emh_surrogate* sg, const emh_material_params* p, const
         if (x) (void)hipEventDestroy(x);
     if (own) (void)hipStreamDestroy(st);
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched scenes
+// Many independent simulations stepped together: one launch per stage over
+// all of them (kernels.hip, "batched scenes").  Each keeps its own
+// device-side control, so each takes its own adaptive substeps.
+struct emh_batch {
+    std::vector<emh_sim*> sims;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pipeline* d_pipes = nullptr;
+    uint32_t* d_starts = nullptr;  // five work lists of n + 1 entries
+    Ctrl* d_ctrl = nullptr;        // gathered control blocks
+    std::vector<Ctrl> h_ctrl;
+    BatchLists L{};
+    uint64_t seq = 0;
+    uint32_t substeps_since_reorder = kReorderSubsteps;
+    uint32_t predicted = 1;
+    uint32_t max_substeps = 1;
+
+    ~emh_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (d_pipes) (void)hipFree(d_pipes);
+        if (d_starts) (void)hipFree(d_starts);
+        if (d_ctrl) (void)hipFree(d_ctrl);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Bring every scene's loop to the same launch sequence (each restarts
+    // from its stored particles, as an upload does) and rebuild the work lists.
+    int sync() {
+        const uint32_t n = (uint32_t)sims.size();
+        std::vector<Pipeline> pipes(n);
+        std::vector<uint32_t> st(5 * (n + 1), 0u);
+        uint32_t* one = st.data();
+        uint32_t* keys = one + (n + 1);
+        uint32_t* cells = keys + (n + 1);
+        uint32_t* fused = cells + (n + 1);
+        uint32_t* dens = fused + (n + 1);
+        max_substeps = 1;
+        predicted = 1;
+        for (uint32_t i = 0; i < n; ++i) {
+            emh_sim* s = sims[i];
+            int rc;
+            if (s->host_dirty) {
+                rc = s->upload();
+            } else {
+                CtrlState cs;
+                rc = s->read_ctrl(cs);
+                if (!rc) rc = s->restart_loop(cs.cur_buf);
+            }
+            if (rc) return rc;
+            if (s->pipe.n == 0) return fail(EMH_ERR_INVALID_ARG, "scene %u of the batch has no particles", i);
+            if (s->seq != 1) return fail(EMH_ERR_STATE, "scene %u did not restart its loop", i);
+            pipes[i] = s->pipe;
+            const uint32_t chunks = (s->pipe.n + kChunk - 1) / kChunk;
+            const bool d = s->pipe.cfg.density_pass != 0;
+            one[i + 1] = one[i] + 1;
+            keys[i + 1] = keys[i] + (s->pipe.n + kBlock - 1) / kBlock;
+            cells[i + 1] = cells[i] + s->n_tiles();
+            fused[i + 1] = fused[i] + (d ? 0u : chunks);
+            dens[i + 1] = dens[i] + (d ? chunks : 0u);
+            max_substeps = std::max<uint32_t>(max_substeps, s->cfg.max_substeps_per_step);
+            predicted = std::max<uint32_t>(predicted, s->predicted_substeps);
+        }
+        HIP_TRY(hipMemcpyAsync(d_pipes, pipes.data(), sizeof(Pipeline) * n, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_starts, st.data(), sizeof(uint32_t) * st.size(), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        auto view = [&](int k) { return BatchView{d_pipes, d_starts + (size_t)k * (n + 1), n}; };
+        L.one = view(0);
+        L.keys = view(1);
+        L.cells = view(2);
+        L.fused = view(3);
+        L.density = view(4);
+        L.keys_blocks = keys[n];
+        L.cells_blocks = cells[n];
+        L.fused_blocks = fused[n];
+        L.density_blocks = dens[n];
+        seq = 1;
+        substeps_since_reorder = kReorderSubsteps;
+        return EMH_OK;
+    }
+
+    bool in_sync() const {
+        for (emh_sim* s : sims)
+            if (s->host_dirty || s->seq != seq || s->sg_mode != 0) return false;
+        return seq != 0;
+    }
+
+    // Wait for every scene's queued steps, launching further substeps while
+    // any scene still runs one; then publish each scene's last step.
+    int drain() {
+        const uint32_t n = (uint32_t)sims.size();
+        uint32_t extra = 0;
+        const uint32_t limit = (max_substeps + 1) * (kStepRing + 1);
+        while (true) {
+            HIP_TRY(launch_batch_gather_ctrl(L, d_ctrl, stream));
+            HIP_TRY(hipMemcpyAsync(h_ctrl.data(), d_ctrl, sizeof(Ctrl) * n, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            uint32_t more = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                const CtrlState& st = h_ctrl[i].st[seq & 1u];
+                if (!sims[i]->running(st) && st.pending == 0) continue;
+                more = std::max<uint32_t>(more, (sims[i]->running(st) ? 1u : 0u) +
+                                                    st.pending * std::max<uint32_t>(sims[i]->predicted_substeps, 1u));
+            }
+            if (more == 0) break;
+            more = std::min<uint32_t>(more, 64u);
+            if (extra + more > limit) return fail(EMH_ERR_DEVICE, "batched substep loop did not terminate");
+            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_batch_substep(L, seq++, stream));
+            extra += more;
+        }
+        predicted = 1;
+        for (uint32_t i = 0; i < n; ++i) {
+            int rc = sims[i]->publish(h_ctrl[i], h_ctrl[i].st[seq & 1u], false);
+            if (rc) return rc;
+            predicted = std::max<uint32_t>(predicted, sims[i]->predicted_substeps);
+        }
+        return EMH_OK;
+    }
+
+    int step(uint32_t n_steps) {
+        if (!in_sync()) {
+            int rc = sync();
+            if (rc) return rc;
+        }
+        constexpr uint32_t kDrainSteps = kStepRing / 2;
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            const bool reorder = substeps_since_reorder >= kReorderSubsteps;
+            HIP_TRY(launch_batch_begin_step(L, seq++, reorder, stream));
+            if (reorder) substeps_since_reorder = 0;
+            const uint32_t slots = std::min<uint32_t>(std::max<uint32_t>(predicted, 1u), max_substeps);
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_batch_substep(L, seq++, stream));
+            substeps_since_reorder += slots;
+            if ((s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
+                int rc = drain();
+                if (rc) return rc;
+            }
+        }
+        for (emh_sim* sim : sims) {
+            sim->seq = seq;
+            sim->device_ahead = true;
+            sim->stats.frame_index += n_steps;
+        }
+        return EMH_OK;
+    }
+};
+
+extern "C" {
+
+int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out) {
+    if (!out || !sims || n == 0) return fail(EMH_ERR_INVALID_ARG, "null or empty argument");
+    *out = nullptr;
+    auto* b = new emh_batch();
+    auto bail = [&](int rc) {
+        delete b;
+        return rc;
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!sims[i]) return bail(fail(EMH_ERR_INVALID_ARG, "scene %u is null", i));
+        for (uint32_t k = 0; k < i; ++k)
+            if (sims[k] == sims[i]) return bail(fail(EMH_ERR_INVALID_ARG, "scene %u appears twice", i));
+        if (sims[i]->sg_mode != 0)
+            return bail(fail(EMH_ERR_UNSUPPORTED, "scene %u uses the surrogate harvest / inference mode", i));
+        if (i > 0 && sims[i]->device != sims[0]->device)
+            return bail(fail(EMH_ERR_INVALID_ARG, "scenes are on different devices"));
+        b->sims.push_back(sims[i]);
+    }
+    b->device = sims[0]->device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(EMH_ERR_DEVICE, "stream creation failed"));
+    b->d_pipes = dalloc<Pipeline>(n);
+    b->d_starts = dalloc<uint32_t>(5 * ((size_t)n + 1));
+    b->d_ctrl = dalloc<Ctrl>(n);
+    if (!b->d_pipes || !b->d_starts || !b->d_ctrl) return bail(fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed"));
+    b->h_ctrl.resize(n);
+    int rc = b->sync();
+    if (rc) return bail(rc);
+    *out = b;
+    return EMH_OK;
+}
+
+int emh_batch_destroy(emh_batch* b) {
+    delete b;
+    return EMH_OK;
+}
+
+int emh_batch_step(emh_batch* b, uint32_t n_steps) {
+    if (!b) return fail(EMH_ERR_INVALID_ARG, "null batch");
+    return b->step(n_steps);
 }
 
 }  // extern "C"

@@ -656,7 +656,7 @@ __global__ void k_zero_all(const Pipeline p) {
 // Control launch `seq` that queues a step start (remaining = config.dt, no
 // substeps yet: step.rs:24-33, applied by the substep launch that begins it)
 // and clears that step's counters; the CFL reduction carries over.
-__global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
+__device__ __forceinline__ void begin_step_body(const Pipeline& p, uint64_t seq, uint32_t reorder) {
     Ctrl* c = p.ctrl;
     CtrlState N = c->st[seq & 1u];
     const uint32_t step = N.step + N.pending + 1u;  // the step this marker starts
@@ -668,16 +668,17 @@ __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
     c->clamp_count[step % kStepRing] = 0;
     c->proj_count[step % kStepRing] = 0;
 }
+__global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) { begin_step_body(p, seq, reorder); }
 
 // Cell key of the stored (projected) position of every particle, its arrival
 // rank inside the key's bin, and the per-tile counts.  `seq` is the control
 // launch that will read the keys.
-__global__ __launch_bounds__(kBlock) void k_keys(const Pipeline p, uint64_t seq) {
+__device__ __forceinline__ void keys_body(const Pipeline& p, uint64_t seq, uint32_t blk) {
     const Ctrl* c = p.ctrl;
     const DevConfig& cfg = p.cfg;
     const PBuf a = sel_buf(p, c->st[seq & 1u].cur_buf);
     const bool project = cfg.project && c->red[seq_mod3(seq + 2u)].dirty;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = blk * blockDim.x + threadIdx.x;
     const bool valid = j < p.n;
     uint32_t key = 0;
     if (valid) {
@@ -692,9 +693,10 @@ __global__ __launch_bounds__(kBlock) void k_keys(const Pipeline p, uint64_t seq)
         p.sort.rank[j] = rank;
     }
 }
+__global__ __launch_bounds__(kBlock) void k_keys(const Pipeline p, uint64_t seq) { keys_body(p, seq, blockIdx.x); }
 
 // Exclusive scan of the tile counts (one workgroup); clears the counts.
-__global__ __launch_bounds__(1024) void k_tile_scan(const Pipeline p) {
+__device__ __forceinline__ void tile_scan_body(const Pipeline& p) {
     __shared__ uint32_t s_scan[17];
     const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
     const uint32_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
@@ -713,11 +715,11 @@ __global__ __launch_bounds__(1024) void k_tile_scan(const Pipeline p) {
         }
     }
 }
+__global__ __launch_bounds__(1024) void k_tile_scan(const Pipeline p) { tile_scan_body(p); }
 
 // Start slot of every cell-key bin inside each 16x16 tile; clears the counts.
-__global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
+__device__ __forceinline__ void cell_scan_body(const Pipeline& p, uint32_t tile) {
     __shared__ uint32_t s_scan[17];
-    const uint32_t tile = blockIdx.x;
     const uint32_t idx = tile * kTileCells + threadIdx.x;
     const uint32_t cnt = p.sort.cell_count[idx];
     uint32_t total;
@@ -725,6 +727,7 @@ __global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) {
     p.sort.cell_start[idx] = p.sort.tile_start[tile] + ex;
     p.sort.cell_count[idx] = 0;
 }
+__global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) { cell_scan_body(p, blockIdx.x); }
 
 constexpr int kModeG2P = 1;   // gather + update the stored state (a substep)
 constexpr int kModeP2G = 2;   // scatter mass, momentum and stress moment
@@ -1008,8 +1011,12 @@ __device__ __forceinline__ void surrogate_update(const SgDev& sg, const SgLds& n
     }
 }
 
-template <int MODE, int ONLY>
-__global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_substep(const Pipeline p, uint64_t seq) {
+// One substep launch for one scene's chunks: blk is this workgroup's index
+// among the nblk workgroups the scene's particles get (the whole grid of a
+// single-scene launch; the scene's share of a batch launch, whose chunks
+// are dealt without the XCD remap).
+template <int MODE, int ONLY, bool BATCH>
+__device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, uint32_t blk, uint32_t nblk) {
     constexpr bool G2P = (MODE & kModeG2P) != 0;
     constexpr bool P2G = (MODE & kModeP2G) != 0;
     constexpr bool HARVEST = (MODE & kModeHarvest) != 0;
@@ -1034,7 +1041,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
     // (a density pass scatters with the exact dt)
     const float pred = predict_dt(N, cfg);
     const float ddt = (G2P && !cfg.density_pass) ? N.dt - P.pred_dt : 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blk == 0 && threadIdx.x == 0) {
         CtrlState W = N;
         if (!G2P) W.active = 0;
         if (P2G) W.pred_dt = pred;
@@ -1060,7 +1067,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
     const uint32_t mask = full ? kFAll : cfg.fields;
     const PBuf a = sel_buf(p, P.cur_buf);
     const PBuf o = reorder ? sel_buf(p, P.cur_buf ^ 1u) : a;
-    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t chunk = BATCH ? blk : xcd_chunk(blk, nblk);
     const uint32_t base = chunk * kChunk;
 
     // 1. the chunk's box; a substep's output base cell moves at most one cell
@@ -1199,8 +1206,8 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
     // 7. clear this workgroup's slice of the set the substep after next uses
     if (G2P) {
         const uint32_t R2 = (uint32_t)R * (uint32_t)R;
-        const uint32_t per = (R2 + gridDim.x - 1) / gridDim.x;
-        const uint32_t lo = blockIdx.x * per, hi = min(R2, lo + per);
+        const uint32_t per = (R2 + nblk - 1) / nblk;
+        const uint32_t lo = blk * per, hi = min(R2, lo + per);
         for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
 #pragma unroll
             for (int ch = 0; ch < kAccChannels; ++ch) p.grid.acc[old][ch][i] = 0.0f;
@@ -1218,10 +1225,16 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
     reduce_cfl(red, speed, mdt, s_red);
 }
 
+template <int MODE, int ONLY>
+__global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_substep(const Pipeline p, uint64_t seq) {
+    substep_body<MODE, ONLY, false>(p, seq, blockIdx.x, gridDim.x);
+}
+
 // Density re-estimate (density.rs:216-247) from the mass grid the previous
 // substep scattered, then this substep's P2G.  This kernel is the first to
 // read the substep's state, so it also projects (and persists) it if flagged.
-__global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64_t seq) {
+template <bool BATCH>
+__device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq, uint32_t blk, uint32_t nblk) {
     __shared__ float s_acc[kAccChannels][kLdsCells];
     __shared__ float s_dm[kLdsCells];
     __shared__ int s_box[4 * kWaves];
@@ -1237,7 +1250,7 @@ __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64
     const bool project_in = cfg.project && Q.dirty;
     const uint32_t mask = project_in ? kFAll : (cfg.fields | kFVolDens);
     const PBuf a = sel_buf(p, P.cur_buf);
-    const uint32_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    const uint32_t chunk = BATCH ? blk : xcd_chunk(blk, nblk);
     const uint32_t base = chunk * kChunk;
     const float* dm = p.grid.dm[cur];
     float* gdst[kAccChannels];
@@ -1297,6 +1310,69 @@ __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64
     flush_box<kAccChannels>(box, s_acc, gdst, R);
     const uint32_t wp = wave_sum(n_proj);
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
+}
+__global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64_t seq) {
+    density_p2g_body<false>(p, seq, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------ batched scenes
+// One launch per stage for many independent scenes: workgroup b of a batch
+// launch belongs to the scene s with start[s] <= b < start[s + 1] (a
+// per-stage work list of each scene's workgroup count), and runs the
+// single-scene body on that scene's pipeline.  Every scene keeps its own
+// device-side control (its own substep dt from its own CFL bound); scenes
+// whose step has finished idle through the launches the others still need.
+__device__ __forceinline__ uint32_t batch_scene(const BatchView& b, uint32_t block, uint32_t& blk, uint32_t& nblk) {
+    uint32_t lo = 0, hi = b.n;  // the last scene whose start is <= block
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.start[mid] <= block)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    blk = block - b.start[lo];
+    nblk = b.start[lo + 1] - b.start[lo];
+    return lo;
+}
+__global__ void kb_begin_step(BatchView b, uint64_t seq, uint32_t reorder) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    if (threadIdx.x == 0) begin_step_body(b.pipes[s], seq, reorder);
+}
+__global__ __launch_bounds__(kBlock) void kb_keys(BatchView b, uint64_t seq) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    keys_body(b.pipes[s], seq, blk);
+}
+__global__ __launch_bounds__(1024) void kb_tile_scan(BatchView b) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    tile_scan_body(b.pipes[s]);
+}
+__global__ __launch_bounds__(kTileCells) void kb_cell_scan(BatchView b) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    cell_scan_body(b.pipes[s], blk);
+}
+template <int MODE>
+__global__ __launch_bounds__(kBlock, 2) void kb_substep(BatchView b, uint64_t seq) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    substep_body<MODE, -1, true>(b.pipes[s], seq, blk, nblk);
+}
+__global__ __launch_bounds__(kBlock) void kb_density_p2g(BatchView b, uint64_t seq) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    density_p2g_body<true>(b.pipes[s], seq, blk, nblk);
+}
+// Every scene's control block, for the host's drain.
+__global__ void kb_gather_ctrl(BatchView b, Ctrl* out) {
+    uint32_t blk, nblk;
+    const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(b.pipes[s].ctrl);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(out + s);
+    for (uint32_t i = threadIdx.x; i < sizeof(Ctrl) / 4; i += blockDim.x) dst[i] = src[i];
 }
 
 // ------------------------------------------------------ diagnostics
@@ -1532,6 +1608,30 @@ hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream) 
         k_substep<kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
     else
         k_substep<kModeP2G, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reorder, hipStream_t stream) {
+    kb_begin_step<<<L.one.n, 64, 0, stream>>>(L.one, seq, reorder ? 1u : 0u);
+    if (reorder) {
+        if (L.keys_blocks) kb_keys<<<L.keys_blocks, kBlock, 0, stream>>>(L.keys, seq + 1u);
+        kb_tile_scan<<<L.one.n, 1024, 0, stream>>>(L.one);
+        kb_cell_scan<<<L.cells_blocks, kTileCells, 0, stream>>>(L.cells);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_substep(const BatchLists& L, uint64_t seq, hipStream_t stream) {
+    if (L.density_blocks) {
+        kb_density_p2g<<<L.density_blocks, kBlock, 0, stream>>>(L.density, seq);
+        kb_substep<kModeG2P | kModeMass><<<L.density_blocks, kBlock, 0, stream>>>(L.density, seq);
+    }
+    if (L.fused_blocks) kb_substep<kModeG2P | kModeP2G><<<L.fused_blocks, kBlock, 0, stream>>>(L.fused, seq);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_gather_ctrl(const BatchLists& L, Ctrl* out, hipStream_t stream) {
+    kb_gather_ctrl<<<L.one.n, 256, 0, stream>>>(L.one, out);
     return hipGetLastError();
 }
 

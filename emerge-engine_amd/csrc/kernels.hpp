@@ -128,6 +128,27 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
 // the learned surrogate (Ctrl::sg); modes 1 and 2 need a NACC-only scene.
 hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode = 0);
 
+// Work list of one batch launch stage: workgroup b belongs to scene s with
+// start[s] <= b < start[s + 1] (device arrays of n + 1 entries).
+struct BatchView {
+    const Pipeline* __restrict__ pipes;  // device array of the scenes' pipelines (read-only in every kernel)
+    const uint32_t* __restrict__ start;
+    uint32_t n;
+};
+struct BatchLists {
+    BatchView one;      // one workgroup per scene (step start, tile scan, control gather)
+    BatchView keys;     // ceil(n_s / 256) workgroups (cell keys)
+    BatchView cells;    // the scene's tiles (cell scan)
+    BatchView fused;    // chunks of the scenes without a density pass (fused substep)
+    BatchView density;  // chunks of the scenes with one (density + P2G, then G2P + mass)
+    uint32_t keys_blocks, cells_blocks, fused_blocks, density_blocks;
+};
+// One launch per stage over every scene of the batch (the single-scene
+// kernels' bodies, ONLY = mixed).
+hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reorder, hipStream_t stream);
+hipError_t launch_batch_substep(const BatchLists& L, uint64_t seq, hipStream_t stream);
+hipError_t launch_batch_gather_ctrl(const BatchLists& L, Ctrl* out, hipStream_t stream);
+
 // Device accumulator of the diagnostics snapshot (see launch_snapshot).
 struct DevSnapshot {
     double mass, mom_x, mom_y, kinetic, jp_sum, h_sum;

@@ -273,6 +273,9 @@ def load_library() -> ctypes.CDLL:
         "emh_nacc_return_batch": (ctypes.c_int, [ctypes.c_void_p, P(CMaterialParams), P(CMaterialExt),
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_float)]),
+        "emh_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_void_p)]),
+        "emh_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+        "emh_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "emh_cell_key": (ctypes.c_uint32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32]),
         "emh_sort_cell_keys": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
                                               ctypes.c_void_p]),
@@ -1037,3 +1040,30 @@ def nacc_return_batch(material: "NaccMaterial", f_ptr: int, alpha_ptr: int, n: i
                                                 ctypes.c_void_p(alpha_ptr), n, ctypes.c_void_p(f_out_ptr),
                                                 ctypes.c_void_p(alpha_out_ptr), reps, ctypes.byref(ms)))
     return ms.value
+
+
+# -------------------------------------------------------------- batched scenes
+class SceneBatch:
+    """Many independent simulations stepped together (one launch per stage for
+    all of them); each keeps its own adaptive substeps."""
+
+    def __init__(self, sims: Sequence["Simulation"]):
+        self._sims = list(sims)  # keeps them alive while batched
+        arr = (ctypes.c_void_p * len(self._sims))(*[sim._h for sim in self._sims])
+        h = ctypes.c_void_p()
+        _check(load_library().emh_batch_create(arr, len(self._sims), ctypes.byref(h)))
+        self._h = h
+
+    def step_n(self, steps: int) -> None:
+        _check(load_library().emh_batch_step(self._h, int(steps)))
+
+    def close(self) -> None:
+        if self._h:
+            load_library().emh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -428,6 +428,26 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
                        uint32_t* keys_out, uint32_t* perm_out);
 
 /* ---------------------------------------------------------------------
+ * Batched scenes.
+ *
+ * No counterpart in the reference (it steps one Simulation at a time,
+ * step.rs:23-60): many independent simulations -- each with its own grid
+ * size, particle count, materials and boundaries -- stepped together, one
+ * kernel launch per stage for all of them.  Each scene keeps its own
+ * adaptive substep schedule: emh_batch_step(b, n) leaves every scene where
+ * n calls of emh_sim_step on it would (up to f32 roundoff of the grid sums).
+ * The batch does not own the scenes.  While batched, a scene is stepped only
+ * through the batch; reading its particles or stats, or editing it, between
+ * batch steps is fine (an edit makes the next batch step restart every
+ * scene's loop from its stored state, as an upload does).  Every scene needs
+ * particles; the surrogate modes are not batched.
+ * ------------------------------------------------------------------- */
+typedef struct emh_batch emh_batch;
+int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out);
+int emh_batch_destroy(emh_batch* b);
+int emh_batch_step(emh_batch* b, uint32_t n_steps);
+
+/* ---------------------------------------------------------------------
  * Learned NACC return-mapping surrogate.
  *
  * No counterpart in the reference: the crate's return mapping is
