@@ -12,7 +12,7 @@ A "step" is one Simulation::step (solver/step.rs:23-60): every adaptive
 substep of config.dt.  `value` is particle-substeps per second over the K
 timed steps; particle-steps per second is reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-surrogate] [--no-batch]
 """
 from __future__ import annotations
 
@@ -231,6 +231,90 @@ def surrogate_section(args) -> dict:
     }
 
 
+# ------------------------------------------------------------ batched scenes
+def build_batch_scenes(seed: int, n_scenes: int):
+    """Independent scenes: grid side uniform over {64, 128, 192, 256}; particle
+    count log-uniform from 2k to 64k (capped at 16 per cell over 35 % of the
+    interior, so the smallest grids hold at most ~17.5k); 1-4 boxes, one per
+    interior quadrant, each of Drucker-Prager sand (E = 3.5e5, nu = 0.3,
+    30 deg), NeoHookean (400, 200) or Newtonian water (1000, 1e4) at random;
+    gravity (0, -1), step dt 0.05 with each scene's own CFL substeps."""
+    rng = np.random.default_rng(seed)
+    lam, mu = eh.lame_from_young(3.5e5, 0.3)
+    sand = eh.DruckerPragerMaterial.new(lam, mu)
+    sand.params.dp_h0 = eh.to_radians(30.0)
+    solid = eh.NeoHookeanMaterial.new(400.0, 200.0)
+    water = eh.NewtonianFluidMaterial.low_viscosity(1000.0, 1e4)
+    sims = []
+    for k in range(n_scenes):
+        r = int(rng.choice([64, 128, 192, 256]))
+        cap = min(64000, int(0.35 * (r - 8) ** 2 * 16))
+        n_target = float(np.exp(rng.uniform(np.log(2000.0), np.log(cap))))
+        boxes = int(rng.integers(1, 5))
+        sim = (eh.Simulation.empty(eh.SimConfig.standard(r, 0.05, (0.0, -1.0)))
+               .with_material(0, sand).with_material(1, solid).with_material(2, water)
+               .with_boundary(eh.SlipBoundary.new(2)))
+        q = (r - 8) / 2.0
+        for b in range(boxes):
+            area = n_target / boxes / 16.0
+            aspect = rng.uniform(0.6, 1.6)
+            w = int(max(2, min(q - 2, round(np.sqrt(area * aspect)))))
+            h = int(max(2, min(q - 2, round(area / w))))
+            cx = 4.0 + q * (b % 2) + q / 2 + rng.uniform(-0.5, 0.5) * (q - w - 1)
+            cy = 4.0 + q * (b // 2) + q / 2 + rng.uniform(-0.5, 0.5) * (q - h - 1)
+            sim.add_body(eh.SpawnRegion(spacing=0.25, box_size=(w, h), box_center=(float(cx), float(cy)),
+                                        position_jitter=0.5, rng_seed=seed * 1000 + k * 4 + b,
+                                        material_id=int(rng.integers(0, 3))))
+        sims.append(sim)
+    return sims
+
+
+def batch_section(args) -> dict:
+    """The scenes stepped as one batch against the same scenes stepped one
+    after another (separate runs of the single-scene path)."""
+    t0 = time.perf_counter()
+    alone = build_batch_scenes(31, args.batch_scenes)
+    together = build_batch_scenes(31, args.batch_scenes)
+    n = np.array([s.num_particles() for s in alone], dtype=np.float64)
+    for s in alone:
+        s.step_n(1)
+    batch = eh.SceneBatch(together)
+    batch.step_n(1)
+    build_s = time.perf_counter() - t0
+    sa0 = np.array([s.stats().total_substeps for s in alone])
+    sb0 = np.array([s.stats().total_substeps for s in together])
+    t0 = time.perf_counter()
+    for s in alone:
+        s.step_n(args.batch_steps)
+    t_alone = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch.step_n(args.batch_steps)
+    t_batch = time.perf_counter() - t0
+    sa = np.array([s.stats().total_substeps for s in alone]) - sa0
+    sb = np.array([s.stats().total_substeps for s in together]) - sb0
+    dx = 0.0
+    for a, b in zip(alone, together):
+        dx = max(dx, float(np.abs(a.particles()["x"].astype(np.float64) - b.particles()["x"]).max()))
+    batch.close()
+    for s in alone + together:
+        s.close()
+    work_a, work_b = float((n * sa).sum()), float((n * sb).sum())
+    return {
+        "workload": "batched_scenes",
+        "scenes": len(n),
+        "particles": int(n.sum()),
+        "particles_min_max": [int(n.min()), int(n.max())],
+        "steps": args.batch_steps,
+        "substeps_per_scene_min_max": [int(sb.min()), int(sb.max())],
+        "substeps_equal": bool(np.array_equal(sa, sb)),
+        "max_abs_dx_cells": dx,
+        "batch": {"particle_substeps_per_s": work_b / t_batch, "seconds": t_batch},
+        "separate": {"particle_substeps_per_s": work_a / t_alone, "seconds": t_alone},
+        "speedup": (work_b / t_batch) / (work_a / t_alone),
+        "build_s": build_s,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +335,9 @@ def main() -> int:
     ap.add_argument("--sg-substep-steps", type=int, default=10)
     ap.add_argument("--sg-fp64-sample", type=int, default=50_000)
     ap.add_argument("--sg-cpu-sample", type=int, default=200_000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes section")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the bench line, run three device-to-device copies of this size (a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -331,6 +418,7 @@ def main() -> int:
         out["cpu_baseline"] = None
     sim.close()
     out["nacc_surrogate"] = None if args.no_surrogate else surrogate_section(args)
+    out["batched_scenes"] = None if args.no_batch else batch_section(args)
     print(json.dumps(out), flush=True)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)
