@@ -98,7 +98,8 @@ int validate_config(const emh_config& c) {  // SimConfig::validate  config.rs:24
 }
 
 int check_supported_config(const emh_config& c) {
-    if (c.sleep_threshold != 0.0f) return fail(EMH_ERR_UNSUPPORTED, "sleep_threshold > 0 (sleep partition) is not supported");
+    if (!(c.sleep_threshold >= 0.0f) || !std::isfinite(c.sleep_threshold))
+        return fail(EMH_ERR_INVALID_ARG, "sleep_threshold must be finite and >= 0");
     if (c.asflip_blend != 0.0f) return fail(EMH_ERR_UNSUPPORTED, "asflip_blend > 0 is not supported");
     if (c.grid_res > 16384) return fail(EMH_ERR_UNSUPPORTED, "grid_res above 16384 is not supported");
     return EMH_OK;
@@ -360,6 +361,11 @@ struct emh_sim {
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
     // learned NACC return mapping: 0 plain, 1 harvest trial states, 2 infer
     uint32_t sg_mode = 0;
+    // sleep partition: on when sleep_threshold > 0 or an uploaded particle sleeps
+    bool sleep = false;
+    float sleep_sq() const {
+        return cfg.sleep_threshold > 0.0f ? cfg.sleep_threshold * cfg.sleep_threshold : 0.0f;
+    }
     SgDev sg_host{};  // the descriptor Ctrl::sg holds
 
     // Live timing: three events per step on the library stream -- before
@@ -644,6 +650,7 @@ struct emh_sim {
         std::vector<float4> c(n), f(n);
         std::vector<float> mass(n), v0(n), vol(n), dens(n), jp(n), h(n), q(n), lvs(n);
         std::vector<uint32_t> mat(n), id(n);
+        bool any_sleeping = false;
         for (uint32_t i = 0; i < n; ++i) {
             const emh_particle& p = host[i];
             x[i] = make_float2(p.x[0], p.x[1]);
@@ -662,6 +669,9 @@ struct emh_sim {
             lvs[i] = p.log_volume_strain;
             mat[i] = (p.material_id & kMatMask) | (p.pinned ? kPinnedBit : 0u);
             if (p.material_id > kMatMask) mat[i] = kMatMask;  // unregistered: falls back to slot 0
+            if (p.sleeping) mat[i] |= kSleepBit;
+            if (p.activation != 0.0f) mat[i] |= kNoSleepBit;
+            any_sleeping = any_sleeping || p.sleeping != 0;
             id[i] = i;
         }
         const PBuf& b = pipe.buf[0];
@@ -681,6 +691,7 @@ struct emh_sim {
         HIP_TRY(put(b.q, q.data(), sizeof(float) * n));
         HIP_TRY(put(b.lvs, lvs.data(), sizeof(float) * n));
         HIP_TRY(put(b.mat, mat.data(), sizeof(uint32_t) * n));
+        sleep = cfg.sleep_threshold > 0.0f || any_sleeping;
         HIP_TRY(put(b.id, id.data(), sizeof(uint32_t) * n));
         Ctrl ctrl;
         std::memset(&ctrl, 0, sizeof(ctrl));
@@ -694,10 +705,11 @@ struct emh_sim {
             r.dirty = 0u;
         }
         ctrl.sg = sg_host;
+        ctrl.sleep_sq = sleep_sq();
         seq = 0;
         HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
         HIP_TRY(launch_zero_all(pipe, stream));
-        HIP_TRY(launch_prologue(pipe, seq++, stream));
+        HIP_TRY(launch_prologue(pipe, seq++, stream, sleep));
         HIP_TRY(hipStreamSynchronize(stream));
         substeps_since_reorder = kReorderSubsteps;  // first step puts the upload in cell-key order
         host_dirty = false;
@@ -733,10 +745,11 @@ struct emh_sim {
             r.dirty = 0u;
         }
         ctrl.sg = sg_host;
+        ctrl.sleep_sq = sleep_sq();
         seq = 0;
         HIP_TRY(hipMemcpyAsync(d_ctrl, &ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, stream));
         HIP_TRY(launch_zero_all(pipe, stream));
-        HIP_TRY(launch_prologue(pipe, seq++, stream));
+        HIP_TRY(launch_prologue(pipe, seq++, stream, sleep));
         HIP_TRY(hipStreamSynchronize(stream));
         device_ahead = true;
         return EMH_OK;
@@ -817,6 +830,11 @@ struct emh_sim {
         HIP_TRY(get(q.data(), b.q, sizeof(float) * n));
         HIP_TRY(get(lvs.data(), b.lvs, sizeof(float) * n));
         HIP_TRY(get(id.data(), b.id, sizeof(uint32_t) * n));
+        std::vector<uint32_t> mat;
+        if (sleep) {
+            mat.resize(n);
+            HIP_TRY(get(mat.data(), b.mat, sizeof(uint32_t) * n));
+        }
         HIP_TRY(hipStreamSynchronize(stream));
         for (uint32_t k = 0; k < n; ++k) {
             const uint32_t i = id[k];
@@ -842,6 +860,7 @@ struct emh_sim {
             p.hardening_scale = h[k];
             p.friction_hardening = q[k];
             p.log_volume_strain = lvs[k];
+            if (sleep) p.sleeping = (mat[k] & kSleepBit) ? 1u : 0u;
         }
         device_ahead = false;
         return EMH_OK;
@@ -870,7 +889,7 @@ struct emh_sim {
             uint32_t more = (running(st) ? 1u : 0u) + st.pending * std::max<uint32_t>(predicted_substeps, 1u);
             more = std::min<uint32_t>(std::max<uint32_t>(more, 1u), 64u);
             if (extra + more > limit) return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
-            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode));
+            for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
             extra += more;
         }
         return publish(c, st, profiling);
@@ -923,6 +942,8 @@ struct emh_sim {
         if (sg_mode != 0 && pipe.n > 0 && (pipe.cfg.only_model != kModelNacc || pipe.cfg.density_pass))
             return fail(EMH_ERR_UNSUPPORTED, "the surrogate harvest / inference modes need a scene whose particles "
                                              "are all NACC");
+        if (sg_mode != 0 && sleep)
+            return fail(EMH_ERR_UNSUPPORTED, "the surrogate harvest / inference modes have no sleep partition");
         if (pipe.n == 0) {  // no particles: the loop still advances time
             for (uint32_t s = 0; s < n_steps; ++s) {
                 float remaining = cfg.dt;
@@ -960,7 +981,7 @@ struct emh_sim {
             if (reorder) substeps_since_reorder = 0;
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode));
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
             substeps_since_reorder += slots;
             if (profiling || (s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
                 int rc = drain();
@@ -1511,6 +1532,24 @@ int emh_sim_stats(const emh_sim* s, emh_step_stats* out) {
     return EMH_OK;
 }
 
+// Simulation::sleep_tag / wake_tag (solver/particles.rs:270-312) on the
+// host copy; the next step uploads it.
+static int set_tag_sleep(emh_sim* s, uint32_t tag, uint32_t sleeping) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    int rc = s->host_current();
+    if (rc) return rc;
+    bool any = false;
+    for (emh_particle& p : s->host)
+        if (p.user_tag == tag && p.sleeping != sleeping) {
+            p.sleeping = sleeping;
+            any = true;
+        }
+    if (any) s->host_dirty = true;
+    return EMH_OK;
+}
+int emh_sim_sleep_tag(emh_sim* s, uint32_t tag) { return set_tag_sleep(s, tag, 1u); }
+int emh_sim_wake_tag(emh_sim* s, uint32_t tag) { return set_tag_sleep(s, tag, 0u); }
+
 int emh_sim_set_gravity(emh_sim* s, float gx, float gy) {  // lifecycle.rs:403-405
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     s->cfg.gravity[0] = gx;
@@ -1651,8 +1690,8 @@ int emh_sim_snapshot(emh_sim* s, emh_snapshot* out) {  // snapshot.rs:227-372, q
     o.cfl_number = o.max_particle_speed * o.effective_dt / cell;
     o.recommended_max_dt_from_velocity_cfl =
         o.max_particle_speed > kF32Eps ? cell / o.max_particle_speed : std::numeric_limits<float>::infinity();
-    o.active_count = n;
-    o.sleeping_count = 0;
+    o.active_count = n - d.sleeping;
+    o.sleeping_count = d.sleeping;
     o.vel_clamp_count = s->stats.last_vel_clamp_count;
     o.j_projection_count = s->stats.last_j_projection_count;
     o.sim_time_dropped = s->stats.last_sim_time_dropped;
@@ -2360,6 +2399,7 @@ struct emh_batch {
             }
             if (rc) return rc;
             if (s->pipe.n == 0) return fail(EMH_ERR_INVALID_ARG, "scene %u of the batch has no particles", i);
+            if (s->sleep) return fail(EMH_ERR_UNSUPPORTED, "scene %u uses the sleep partition", i);
             if (s->seq != 1) return fail(EMH_ERR_STATE, "scene %u did not restart its loop", i);
             pipes[i] = s->pipe;
             const uint32_t chunks = (s->pipe.n + kChunk - 1) / kChunk;
@@ -2471,6 +2511,8 @@ int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out) {
             if (sims[k] == sims[i]) return bail(fail(EMH_ERR_INVALID_ARG, "scene %u appears twice", i));
         if (sims[i]->sg_mode != 0)
             return bail(fail(EMH_ERR_UNSUPPORTED, "scene %u uses the surrogate harvest / inference mode", i));
+        if (sims[i]->cfg.sleep_threshold > 0.0f || sims[i]->sleep)
+            return bail(fail(EMH_ERR_UNSUPPORTED, "scene %u uses the sleep partition", i));
         if (i > 0 && sims[i]->device != sims[0]->device)
             return bail(fail(EMH_ERR_INVALID_ARG, "scenes are on different devices"));
         b->sims.push_back(sims[i]);

@@ -24,6 +24,11 @@ constexpr float kInf = __builtin_huge_valf();
 
 // Packed flags in the per-particle material word.
 constexpr uint32_t kPinnedBit = 0x80000000u;
+// sleep partition (solver/mod.rs:40-42): the particle sleeps (skipped by P2G,
+// G2P, the CFL bound and projection); it has non-zero activation and never
+// falls asleep (step.rs:299-306)
+constexpr uint32_t kSleepBit = 0x40000000u;
+constexpr uint32_t kNoSleepBit = 0x20000000u;
 constexpr uint32_t kMatMask = 0x0000ffffu;
 
 // ------------------------------------------------------------------ config

@@ -37,6 +37,8 @@ __global__ __launch_bounds__(256) void k_export_particles(const PBuf b, uint32_t
     r[3] = make_float4(b.mass[s], b.v0[s], b.vol[s], b.dens[s]);
     r[4] = make_float4(r[4].x, b.jp[s], b.h[s], b.q[s]);  // material_id stays the caller's
     r[5] = make_float4(b.lvs[s], r[5].y, r[5].z, r[5].w);
+    static_assert(offsetof(emh_particle, sleeping) == 112, "Particle layout");
+    r[7].x = __uint_as_float((b.mat[s] & kSleepBit) ? 1u : 0u);
     float4* dst = reinterpret_cast<float4*>(out + i);
 #pragma unroll
     for (int k = 0; k < 8; ++k) dst[k] = r[k];
@@ -83,6 +85,8 @@ __global__ __launch_bounds__(256) void k_impulse(const PBuf b, uint32_t n, float
         v.y = v.y * k;
     }
     b.v[s] = v;
+    const uint32_t word = b.mat[s];
+    if (word & kSleepBit) b.mat[s] = word & ~kSleepBit;  // particles in range wake (particles.rs:122-137)
 }
 
 }  // namespace

@@ -734,6 +734,34 @@ constexpr int kModeP2G = 2;   // scatter mass, momentum and stress moment
 constexpr int kModeMass = 4;  // scatter mass only (density re-estimate)
 constexpr int kModeHarvest = 8;     // record NACC trial states (training data of the surrogate)
 constexpr int kModeSurrogate = 16;  // NACC return mapping through the learned surrogate
+constexpr int kModeSleep = 32;      // sleep partition on (sleeping particles are skipped)
+
+// wake_particle's test (step.rs:107-131): a stencil cell of the sleeping
+// particle holds mass from this substep's P2G (the reference asks whether P2G
+// touched the cell; the two differ only for a stencil weight of exactly 0).
+__device__ __forceinline__ bool stencil_touched(const Pipeline& p, uint32_t set, float x, float y) {
+    const Wts w = weights(x, y);
+    const int R = (int)p.cfg.grid_res;
+    const float* m = p.grid.acc[set][kAccM];
+    bool hit = false;
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy) {
+            const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
+            if (cx >= 0 && cy >= 0 && cx < R && cy < R) hit = hit || m[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] > 0.0f;
+        }
+    return hit;
+}
+// Sleep scoring (step.rs:299-315): a particle without activation whose
+// speed^2 is below threshold^2 falls asleep at the end of the substep.
+__device__ __forceinline__ void sleep_test(float sleep_sq, PState& s, bool& asleep, bool& changed) {
+    if (sleep_sq > 0.0f && !(s.mat & kNoSleepBit) && s.vx * s.vx + s.vy * s.vy < sleep_sq) {
+        s.mat |= kSleepBit;
+        asleep = true;
+        changed = true;
+    }
+}
 
 // Bounding box of the chunk's base cells (positions projected when the state
 // is flagged), widened by `margin` cells.
@@ -1021,6 +1049,8 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     constexpr bool P2G = (MODE & kModeP2G) != 0;
     constexpr bool HARVEST = (MODE & kModeHarvest) != 0;
     constexpr bool SURR = (MODE & kModeSurrogate) != 0;
+    constexpr bool SLEEP = (MODE & kModeSleep) != 0;
+    static_assert(!(SURR && SLEEP), "the surrogate substep has no sleep partition");
     constexpr int NCH = P2G ? kAccChannels : 1;
     __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgWaveLds : 1];
     __shared__ std::conditional_t<SURR, SgLds, char> s_net;
@@ -1106,12 +1136,21 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         const uint32_t j = base + (valid ? s_perm[q] : q);
         PState sp = neutral_state();
         PState s = neutral_state();
+        bool asleep = false, mat_changed = false;
         if (valid) {
             const float2 x = a.x[j];
             s.x = x.x;
             s.y = x.y;
             load_fields(s, a, j, mask, !G2P || project_in);
-            if (G2P) {
+            if constexpr (SLEEP) {
+                asleep = (s.mat & kSleepBit) != 0u;
+                if (G2P && asleep && stencil_touched(p, cur, s.x, s.y)) {
+                    s.mat &= ~kSleepBit;
+                    asleep = false;
+                    mat_changed = true;
+                }
+            }
+            if (G2P && !asleep) {
                 if (project_in && project_state(s, cfg)) ++n_proj;
                 // 3. gather_grid_to_particles phase 1 (g2p.rs:203-325)
                 const Wts w = weights(s.x, s.y);
@@ -1160,15 +1199,19 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 // phase 2: update_particle (g2p.rs:333-340)
                 if constexpr (HARVEST) harvest_nacc(p.ctrl->sg, mat_of(p, s.mat), s, dt);
                 if constexpr (!SURR) update_particle<ONLY>(mat_of(p, s.mat), s, dt);
+                if constexpr (SLEEP) sleep_test(c->sleep_sq, s, asleep, mat_changed);
             }
             if constexpr (!SURR) {
                 // 4. the state the next substep starts from: CFL, projection check
-                cfl_of<ONLY>(p, s, speed, mdt);
-                any_dirty |= needs_projection(s, sp, cfg);
-                if (G2P) {
+                if (!asleep) {
+                    cfl_of<ONLY>(p, s, speed, mdt);
+                    any_dirty |= needs_projection(s, sp, cfg);
+                }
+                if (G2P && (!SLEEP || !asleep || mat_changed || reorder)) {
                     const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
                     store_fields(o, dest, s, mask, full);
                     if (reorder) o.id[dest] = a.id[j];
+                    if (SLEEP && mat_changed && !full) o.mat[dest] = s.mat;
                 }
             }
         }
@@ -1183,17 +1226,19 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 if (reorder) o.id[dest] = a.id[j];
             }
         }
-        // 5. P2G of the projected state into the next accumulator set
+        // 5. P2G of the projected state into the next accumulator set (the
+        //    awake particles only)
+        const bool scat = valid && !asleep;
         if (P2G) {
             const P2GTerms t = p2g_terms<ONLY>(p, sp, pred);
-            scatter_p2g(t, valid, box, s_acc, gdst, R);
+            scatter_p2g(t, scat, box, s_acc, gdst, R);
         } else {
             P2GTerms t;
             t.w = weights(sp.x, sp.y);
             t.mass = sp.mass;
-            const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
-            const bool fma_ok = __ballot(valid && !(fabsf(t.mass) < 1e18f)) == 0ull;
-            scatter_channel<kAccM>(t, valid, sm, scan_pred(sm), fma_ok, box, s_acc[0], gdst[0], R);
+            const SegMask sm = run_mask(scat ? t.w.bx : INT_MIN + (int)lane_id(), scat ? t.w.by : 0);
+            const bool fma_ok = __ballot(scat && !(fabsf(t.mass) < 1e18f)) == 0ull;
+            scatter_channel<kAccM>(t, scat, sm, scan_pred(sm), fma_ok, box, s_acc[0], gdst[0], R);
         }
         if (sliced) {
             flush_box<NCH>(box, s_acc, gdst, R);
@@ -1233,7 +1278,7 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
 // Density re-estimate (density.rs:216-247) from the mass grid the previous
 // substep scattered, then this substep's P2G.  This kernel is the first to
 // read the substep's state, so it also projects (and persists) it if flagged.
-template <bool BATCH>
+template <bool BATCH, bool SLEEP>
 __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq, uint32_t blk, uint32_t nblk) {
     __shared__ float s_acc[kAccChannels][kLdsCells];
     __shared__ float s_dm[kLdsCells];
@@ -1273,11 +1318,15 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
         const uint32_t j = base + k * kBlock + threadIdx.x;
         const bool valid = j < p.n;
         PState s = neutral_state();
+        bool asleep = false;
         if (valid) {
             const float2 x = a.x[j];
             s.x = x.x;
             s.y = x.y;
             load_fields(s, a, j, mask, true);
+            if (SLEEP) asleep = (s.mat & kSleepBit) != 0u;  // estimate_particle_volumes visits the awake ones
+        }
+        if (valid && !asleep) {
             if (project_in && project_state(s, cfg)) ++n_proj;
             const Wts w = weights(s.x, s.y);
             const bool lds = box.holds3x3(w.bx, w.by);
@@ -1305,14 +1354,15 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
             }
         }
         const P2GTerms t = p2g_terms<-1>(p, s, N.dt);
-        scatter_p2g(t, valid, box, s_acc, gdst, R);
+        scatter_p2g(t, valid && !asleep, box, s_acc, gdst, R);
     }
     flush_box<kAccChannels>(box, s_acc, gdst, R);
     const uint32_t wp = wave_sum(n_proj);
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
+template <bool SLEEP>
 __global__ __launch_bounds__(kBlock) void k_density_p2g(const Pipeline p, uint64_t seq) {
-    density_p2g_body<false>(p, seq, blockIdx.x, gridDim.x);
+    density_p2g_body<false, SLEEP>(p, seq, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------ batched scenes
@@ -1364,7 +1414,7 @@ __global__ __launch_bounds__(kBlock, 2) void kb_substep(BatchView b, uint64_t se
 __global__ __launch_bounds__(kBlock) void kb_density_p2g(BatchView b, uint64_t seq) {
     uint32_t blk, nblk;
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
-    density_p2g_body<true>(b.pipes[s], seq, blk, nblk);
+    density_p2g_body<true, false>(b.pipes[s], seq, blk, nblk);
 }
 // Every scene's control block, for the host's drain.
 __global__ void kb_gather_ctrl(BatchView b, Ctrl* out) {
@@ -1395,12 +1445,13 @@ __global__ __launch_bounds__(kBlock) void k_snap_particles(const Pipeline p, uin
     double mass = 0.0, mx = 0.0, my = 0.0, ke = 0.0, jp_sum = 0.0, h_sum = 0.0;
     float speed = 0.0f, pinned_speed = 0.0f;
     uint32_t min_j = 0xffffffffu, max_j = 0u, min_jp = 0xffffffffu;
-    uint32_t n_valid = 0, n_nonfinite = 0, n_invalid = 0, n_oob = 0, n_jp = 0;
+    uint32_t n_valid = 0, n_nonfinite = 0, n_invalid = 0, n_oob = 0, n_jp = 0, n_sleep = 0;
     if (valid) {
         const float2 x = b.x[i], v = b.v[i];
         const float4 c = b.c[i], f = b.f[i];
         const float m = b.mass[i], v0 = b.v0[i], vol = b.vol[i], dens = b.dens[i], jp = b.jp[i], h = b.h[i];
         const uint32_t word = b.mat[i];
+        n_sleep = (word & kSleepBit) ? 1u : 0u;
         const float j = f.x * f.w - f.z * f.y;  // Mat2::determinant, column-major
         mass = m;
         mx = (double)(m * v.x);
@@ -1454,6 +1505,7 @@ __global__ __launch_bounds__(kBlock) void k_snap_particles(const Pipeline p, uin
     n_invalid = wave_sum(n_invalid);
     n_oob = wave_sum(n_oob);
     n_jp = wave_sum(n_jp);
+    n_sleep = wave_sum(n_sleep);
     if (lane_id() == 0) {
         atomicAdd(&out->mass, mass);
         atomicAdd(&out->mom_x, mx);
@@ -1470,6 +1522,7 @@ __global__ __launch_bounds__(kBlock) void k_snap_particles(const Pipeline p, uin
         if (n_nonfinite) atomicAdd(&out->non_finite, n_nonfinite);
         if (n_invalid) atomicAdd(&out->invalid, n_invalid);
         if (n_oob) atomicAdd(&out->out_of_bounds, n_oob);
+        if (n_sleep) atomicAdd(&out->sleeping, n_sleep);
         if (n_jp) atomicAdd(&out->jp_finite, n_jp);
     }
 }
@@ -1602,12 +1655,19 @@ hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream) {
+hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream, bool sleep) {
     const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
-    if (p.cfg.density_pass)
-        k_substep<kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
-    else
-        k_substep<kModeP2G, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    if (p.cfg.density_pass) {
+        if (sleep)
+            k_substep<kModeMass | kModeSleep, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+        else
+            k_substep<kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    } else {
+        if (sleep)
+            k_substep<kModeP2G | kModeSleep, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+        else
+            k_substep<kModeP2G, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+    }
     return hipGetLastError();
 }
 
@@ -1647,10 +1707,19 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode) {
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode, bool sleep) {
     const uint32_t nb = p.n == 0 ? 1u : div_up(p.n, kChunk);
+    if (sleep) {  // the sleep partition: mixed-model instances only
+        if (p.cfg.density_pass) {
+            k_density_p2g<true><<<nb, kBlock, 0, stream>>>(p, seq);
+            k_substep<kModeG2P | kModeMass | kModeSleep, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+        } else {
+            k_substep<kModeG2P | kModeP2G | kModeSleep, -1><<<nb, kBlock, 0, stream>>>(p, seq);
+        }
+        return hipGetLastError();
+    }
     if (p.cfg.density_pass) {
-        k_density_p2g<<<nb, kBlock, 0, stream>>>(p, seq);
+        k_density_p2g<false><<<nb, kBlock, 0, stream>>>(p, seq);
         k_substep<kModeG2P | kModeMass, -1><<<nb, kBlock, 0, stream>>>(p, seq);
     } else {
         constexpr int kFused = kModeG2P | kModeP2G;

@@ -105,6 +105,8 @@ struct Ctrl {
     uint32_t clamp_count[kStepRing];  // G2P velocity clamps of each step
     uint32_t proj_count[kStepRing];   // state projections of each step
     SgDev sg;  // learned NACC return mapping (read by the harvest and surrogate substeps only)
+    float sleep_sq;  // sleep_threshold^2 (f32), read by the sleep-partition substeps only
+    uint32_t sleep_pad;
 };
 constexpr uint32_t kInfBits = 0x7f800000u;
 
@@ -122,11 +124,12 @@ struct Pipeline {
 // Launch wrappers (all asynchronous on `stream`); `seq` is the launch's
 // sequence number in the device-side loop (see Ctrl).
 hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream);
-hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream);
+hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream, bool sleep = false);
 hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipStream_t stream);
 // sg_mode: 0 plain, 1 harvest NACC trial states, 2 NACC return mapping through
 // the learned surrogate (Ctrl::sg); modes 1 and 2 need a NACC-only scene.
-hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode = 0);
+hipError_t launch_substep(const Pipeline& p, uint64_t seq, hipStream_t stream, uint32_t sg_mode = 0,
+                          bool sleep = false);
 
 // Work list of one batch launch stage: workgroup b belongs to scene s with
 // start[s] <= b < start[s + 1] (device arrays of n + 1 entries).
@@ -157,6 +160,7 @@ struct DevSnapshot {
     uint32_t min_j, max_j, min_jp;                        // order-preserving f32 keys
     uint32_t valid, non_finite, invalid, out_of_bounds, jp_finite;
     uint32_t active_cells, non_finite_grid, touched_cells, mixed_cells, particles_in_mixed;
+    uint32_t sleeping;
 };
 // Diagnostics over the stored particles and the last substep's grid: set
 // `grid_set` updated with `dt` / `ddt`; `cell_lo/hi/count` are R^2 scratch

@@ -273,6 +273,8 @@ def load_library() -> ctypes.CDLL:
         "emh_nacc_return_batch": (ctypes.c_int, [ctypes.c_void_p, P(CMaterialParams), P(CMaterialExt),
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_float)]),
+        "emh_sim_sleep_tag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_sim_wake_tag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "emh_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_void_p)]),
         "emh_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
@@ -887,6 +889,14 @@ class Simulation:
         _check(load_library().emh_sim_set_surrogate_mode(self._h, surrogate._h if surrogate else None, code,
                                                           int(harvest_capacity), int(harvest_stride)))
         self._surrogate = surrogate  # keeps it alive while the sim uses it
+
+    def sleep_tag(self, tag: int) -> None:
+        """Simulation::sleep_tag (solver/particles.rs:293-312)."""
+        _check(load_library().emh_sim_sleep_tag(self._h, int(tag)))
+
+    def wake_tag(self, tag: int) -> None:
+        """Simulation::wake_tag (solver/particles.rs:270-291)."""
+        _check(load_library().emh_sim_wake_tag(self._h, int(tag)))
 
     def set_gravity(self, gravity) -> None:
         """Simulation::set_gravity (lifecycle.rs:403-405)."""

@@ -210,7 +210,8 @@ typedef struct emh_step_stats {
  * (src/spacetime/solver/queries.rs:15-30), without the wall-clock timing.
  * Particle sums are accumulated in f64 (the reference sums in f32 in
  * particle order); the grid fields are those of the last substep's updated
- * grid.  Sleeping particles do not exist on this path (sleeping_count = 0). */
+ * grid.  active_count / sleeping_count split the particles by the sleep
+ * partition. */
 typedef struct emh_snapshot {
     uint64_t frame_index;
     float    configured_dt;
@@ -389,6 +390,18 @@ int emh_sim_export_particles_device(emh_sim* sim, void* dst, size_t capacity);
  * emh_sim_get_grid. */
 int emh_sim_export_grid_device(emh_sim* sim, void* dst, size_t capacity_cells);
 /* Simulation::set_gravity  lifecycle.rs:403-405 (applies from the next step) */
+/* Sleep partition (solver/mod.rs:40-42, step.rs:107-131,299-315): with
+ * config.sleep_threshold > 0, a particle without activation whose speed falls
+ * below the threshold sleeps from the end of that substep (skipped by P2G,
+ * G2P, the CFL bound and projection, its state frozen) until this substep's
+ * P2G reaches one of its stencil cells, or an impulse covers it.  Particles
+ * uploaded with Particle::sleeping = 1 start asleep.  The reference also
+ * moves sleeping particles to the end of its arrays; here particles keep the
+ * caller's order and carry the state in Particle::sleeping.
+ * sleep_tag / wake_tag = Simulation::sleep_tag / wake_tag
+ * (solver/particles.rs:270-312): every particle with that user_tag. */
+int emh_sim_sleep_tag(emh_sim* sim, uint32_t tag);
+int emh_sim_wake_tag(emh_sim* sim, uint32_t tag);
 int emh_sim_set_gravity(emh_sim* sim, float gx, float gy);
 /* Simulation::apply_impulse(center, radius, force)  solver/particles.rs:115-139:
  * v += force (1 - |d| / radius) within radius, speed clamped to

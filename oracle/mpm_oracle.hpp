@@ -267,6 +267,20 @@ template <class T> struct Particles {
         p.scalar_field = (float)scalar_field[i];
         return p;
     }
+    // Particles::swap / rotate_range (every field), the moves of the sleep partition
+    template <class F> void each_field(F&& fn) {
+        fn(x); fn(v); fn(c); fn(f); fn(mass); fn(initial_volume); fn(volume); fn(density); fn(material_id);
+        fn(plastic_volume_ratio); fn(hardening_scale); fn(friction_hardening); fn(log_volume_strain);
+        fn(temperature); fn(activation); fn(activation_dir); fn(user_tag); fn(muscle_group_id); fn(contact_group);
+        fn(sleeping); fn(pinned); fn(scalar_field);
+    }
+    void swap(size_t i, size_t j) {
+        each_field([&](auto& vec) { std::swap(vec[i], vec[j]); });
+    }
+    // [a, mid) and [mid, end) trade places: [mid, end) moves to a
+    void rotate_range(size_t a, size_t mid, size_t end) {
+        each_field([&](auto& vec) { std::rotate(vec.begin() + a, vec.begin() + mid, vec.begin() + end); });
+    }
     void set(size_t i, const emh_particle& p) {
         Particles<T> one;
         one.push(p);
@@ -1100,6 +1114,12 @@ template <class T> struct Grid {
         int64_t s = find(idx);
         return s >= 0 ? cells[(size_t)s].mass : T(0);
     }
+    // cell_is_active  grid/mod.rs:231-241: touched by this substep's P2G
+    bool cell_is_active(int32_t cx, int32_t cy) const {
+        uint32_t idx;
+        if (!flat_index(cx, cy, idx)) return false;
+        return find(idx) >= 0;
+    }
 };
 
 // ----------------------------------------------------------------- spawning
@@ -1261,6 +1281,9 @@ template <class T> struct Simulation {
     std::vector<Boundary> boundaries;
     Particles<T> P;
     Grid<T> grid;
+    // sleep partition (solver/mod.rs:40-42): P[0, active_count) active,
+    // P[active_count, N) sleeping; P2G / G2P / CFL / projection visit the active part
+    size_t active_count = 0;
     uint32_t next_tag = 1;
     T last_step_dt = 0;
     uint32_t last_substeps = 0, last_vel_clamp_count = 0, last_j_projection_count = 0, total_substeps = 0;
@@ -1268,15 +1291,59 @@ template <class T> struct Simulation {
     uint64_t frame_index = 0;
     std::vector<T> substep_dts;  // record of every substep dt taken (diagnostics)
 
-    // Simulation::apply_impulse / apply_radial_impulse  solver/particles.rs:115-165
-    // (every particle is awake: the sleep partition is off)
+    // sleep_particle / wake_particle  solver/particles.rs:227-268 (tag_index,
+    // a lookup structure, is not kept: nothing here queries by tag)
+    void sleep_particle(size_t i) {
+        if (P.sleeping[i] || active_count == 0) return;
+        P.sleeping[i] = 1;
+        const size_t last_active = active_count - 1;
+        if (i != last_active) P.swap(i, last_active);
+        active_count -= 1;
+    }
+    void wake_particle(size_t i) {
+        if (!P.sleeping[i]) return;
+        P.sleeping[i] = 0;
+        const size_t first_sleeping = active_count;
+        if (i != first_sleeping) P.swap(i, first_sleeping);
+        active_count += 1;
+    }
+    // wake_tag / sleep_tag  solver/particles.rs:270-312: ascending wakes, descending sleeps
+    void wake_tag(uint32_t tag) {
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < P.size(); ++i)
+            if (P.user_tag[i] == tag && P.sleeping[i]) idx.push_back(i);
+        for (size_t i : idx) wake_particle(i);
+    }
+    void sleep_tag(uint32_t tag) {
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < P.size(); ++i)
+            if (P.user_tag[i] == tag && !P.sleeping[i]) idx.push_back(i);
+        for (size_t k = idx.size(); k-- > 0;) sleep_particle(idx[k]);
+    }
+    // A particle set handed over whole: awake ones first (stable), then the sleeping ones.
+    void reset_partition() {
+        std::vector<size_t> order;
+        for (size_t i = 0; i < P.size(); ++i)
+            if (!P.sleeping[i]) order.push_back(i);
+        active_count = order.size();
+        for (size_t i = 0; i < P.size(); ++i)
+            if (P.sleeping[i]) order.push_back(i);
+        Particles<T> q;
+        for (size_t i : order) q.push(P.get(i));
+        P = q;
+    }
+
+    // Simulation::apply_impulse / apply_radial_impulse  solver/particles.rs:115-165:
+    // every particle in range, sleeping ones woken afterwards in index order
     void apply_impulse(V2<T> center, T radius, V2<T> force, T strength, bool radial) {
         T vel_limit = T(cfg.grid_cell_size) / T(cfg.min_dt);
         T r2 = radius * radius;
+        std::vector<size_t> to_wake;
         for (size_t i = 0; i < P.size(); ++i) {
             V2<T> d = P.x[i] - center;
             T dist2 = dot(d, d);
             if (!(dist2 <= r2 && dist2 > T(1e-8f))) continue;
+            if (P.sleeping[i]) to_wake.push_back(i);
             if (radial) {
                 T dist = o_sqrt(dist2);
                 T falloff = T(1) - dist / radius;
@@ -1288,6 +1355,7 @@ template <class T> struct Simulation {
             T spd = length(P.v[i]);
             if (spd > vel_limit) P.v[i] = P.v[i] * (vel_limit / spd);
         }
+        for (size_t i : to_wake) wake_particle(i);
     }
 
     void apply_impulse_f(float cx, float cy, float radius, float fx, float fy, float strength, bool radial) {
@@ -1379,10 +1447,12 @@ template <class T> struct Simulation {
         gather_density(new_start, active_count, write_initial);
     }
 
-    // add_body  solver/particles.rs:355-424 (no sleeping partition: all active)
+    // add_body  solver/particles.rs:355-424: the new particles join the active
+    // part, in front of any sleeping ones
     int add_body(const emh_spawn_region& sp, uint32_t* tag_out) {
         if (!spawn_fits(sp, cfg)) return EMH_ERR_INVALID_ARG;
         uint32_t tag = next_tag++;
+        const size_t old_active = active_count;
         size_t old_len = P.size();
         LcgRng rng(sp.rng_seed);
         std::vector<emh_particle> np = initialize_particles(cfg, sp, rng);
@@ -1392,7 +1462,10 @@ template <class T> struct Simulation {
             if (registered) materials[sp.material_id].init_particle(p);
             P.push(p);
         }
-        estimate_particle_volumes_local(P.size(), old_len, true);
+        const size_t new_count = P.size() - old_len;
+        if (old_len > old_active) P.rotate_range(old_active, old_len, P.size());
+        active_count = old_active + new_count;
+        estimate_particle_volumes_local(active_count, old_active, true);
         if (tag_out) *tag_out = tag;
         return EMH_OK;
     }
@@ -1403,7 +1476,8 @@ template <class T> struct Simulation {
         if (!spawn_fits(sp, cfg)) return EMH_ERR_INVALID_ARG;
         LcgRng rng(sp.rng_seed);
         for (auto& p : initialize_particles(cfg, sp, rng)) P.push(p);
-        if (sp.precompute_initial_volumes) estimate_particle_volumes(P.size(), true);
+        active_count = P.size();
+        if (sp.precompute_initial_volumes) estimate_particle_volumes(active_count, true);
         return EMH_OK;
     }
 
@@ -1420,7 +1494,7 @@ template <class T> struct Simulation {
         if (!cfg.adaptive_timestep) return rmin(max_dt, T(cfg.dt));
         T max_speed = T(0);
         T min_mat_dt = max_dt;
-        for (size_t i = 0; i < P.size(); ++i) {
+        for (size_t i = 0; i < active_count; ++i) {
             T s = length(P.v[i]);
             if (cfg.cfl_include_affine_speed) s = s + affine_cfl(P.c[i]);
             max_speed = rmax(max_speed, s);
@@ -1527,7 +1601,7 @@ template <class T> struct Simulation {
 
     // scatter_particles_to_grid  transfer/p2g.rs:24-72
     void p2g(T dt) {
-        for (size_t i = 0; i < P.size(); ++i) {
+        for (size_t i = 0; i < active_count; ++i) {
             const Material<T>& m = mat(P.material_id[i]);
             V2<T> x = P.x[i];
             T mass_i = P.mass[i];
@@ -1583,7 +1657,7 @@ template <class T> struct Simulation {
         T apic = T(cfg.apic_blend);
         uint32_t R = cfg.grid_res;
         uint32_t clamped = 0;
-        for (size_t i = 0; i < P.size(); ++i) {  // phase 1
+        for (size_t i = 0; i < active_count; ++i) {  // phase 1
             V2<T> x = P.x[i];
             Weights<T> w = quadratic_weights(x);
             V2<T> new_v{T(0), T(0)};
@@ -1614,22 +1688,45 @@ template <class T> struct Simulation {
             P.c[i] = (b * T(KERNEL_D_INVERSE)) * apic;
             P.x[i] = np;
         }
-        for (size_t i = 0; i < P.size(); ++i) mat(P.material_id[i]).update_particle(P, i, dt);  // phase 2
+        for (size_t i = 0; i < active_count; ++i) mat(P.material_id[i]).update_particle(P, i, dt);  // phase 2
         return clamped;
     }
 
     // do_substep  step.rs:62-317 (the parts that run for slip-only scenes)
     void do_substep(T sub_dt) {
         if (cfg.project_invalid_state) {
-            for (size_t i = 0; i < P.size(); ++i)
+            for (size_t i = 0; i < active_count; ++i)
                 if (project(i)) last_j_projection_count++;
         }
         if (cfg.recompute_density_each_step || any_needs_density_recompute())
-            estimate_particle_volumes(P.size(), false);
+            estimate_particle_volumes(active_count, false);
         grid.clear();
         p2g(sub_dt);
+        // wake every sleeping particle whose stencil touches a cell this P2G reached
+        // (step.rs:107-131), in index order
+        if (active_count < P.size()) {
+            std::vector<size_t> wake;
+            for (size_t i = active_count; i < P.size(); ++i) {
+                Weights<T> w = quadratic_weights(P.x[i]);
+                bool hit = false;
+                for (int gx = 0; gx < 3 && !hit; ++gx)
+                    for (int gy = 0; gy < 3 && !hit; ++gy) hit = grid.cell_is_active(w.bx + gx - 1, w.by + gy - 1);
+                if (hit) wake.push_back(i);
+            }
+            for (size_t i : wake) wake_particle(i);
+        }
         grid_update(sub_dt);
         last_vel_clamp_count += g2p(sub_dt);
+        // sleep scoring (step.rs:299-315): slow particles without activation,
+        // put to sleep in descending index order
+        const T threshold = T(cfg.sleep_threshold);
+        if (threshold > T(0)) {
+            const T threshold_sq = threshold * threshold;
+            std::vector<size_t> slow;
+            for (size_t i = 0; i < active_count; ++i)
+                if (P.activation[i] == T(0) && dot(P.v[i], P.v[i]) < threshold_sq) slow.push_back(i);
+            for (size_t k = slow.size(); k-- > 0;) sleep_particle(slow[k]);
+        }
     }
 
     // Simulation::step  step.rs:23-60

@@ -207,8 +207,27 @@ int orc_set_particles(void* hv, const emh_particle* p, size_t n) {
         using T = typename std::decay_t<decltype(s.P.mass)>::value_type;
         s.P = Particles<T>();
         for (size_t i = 0; i < n; ++i) s.P.push(p[i]);
+        s.reset_partition();
         return EMH_OK;
     });
+}
+
+int orc_sleep_tag(void* hv, uint32_t tag, uint32_t sleep) {
+    Handle* h = (Handle*)hv;
+    if (!h) return EMH_ERR_INVALID_ARG;
+    return with_sim(h, [&](auto& s) {
+        if (sleep)
+            s.sleep_tag(tag);
+        else
+            s.wake_tag(tag);
+        return EMH_OK;
+    });
+}
+
+size_t orc_active_count(void* hv) {
+    Handle* h = (Handle*)hv;
+    if (!h) return 0;
+    return with_sim(h, [&](auto& s) { return s.active_count; });
 }
 
 size_t orc_num_particles(void* hv) {

@@ -57,6 +57,8 @@ def load() -> ctypes.CDLL:
         "orc_add_slip_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "orc_clear_boundaries": (ctypes.c_int, [ctypes.c_void_p]),
         "orc_apply_impulse": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 6 + [ctypes.c_uint32]),
+        "orc_sleep_tag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+        "orc_active_count": (ctypes.c_size_t, [ctypes.c_void_p]),
         "orc_set_gravity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float]),
         "orc_add_friction_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float]),
         "orc_add_heightmap_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
@@ -191,6 +193,15 @@ class OracleSim:
 
     def set_gravity(self, g):
         self._ok(load().orc_set_gravity(self._h, float(g[0]), float(g[1])))
+
+    def sleep_tag(self, tag):
+        self._ok(load().orc_sleep_tag(self._h, int(tag), 1))
+
+    def wake_tag(self, tag):
+        self._ok(load().orc_sleep_tag(self._h, int(tag), 0))
+
+    def active_count(self) -> int:
+        return int(load().orc_active_count(self._h))
 
     def apply_impulse(self, center, radius, force):
         self._ok(load().orc_apply_impulse(self._h, float(center[0]), float(center[1]), float(radius),

@@ -207,6 +207,30 @@ def nacc_block(backend: str, advance_f=True):
     return sim
 
 
+def sleep_collide(backend: str):
+    """Sleep partition: a NeoHookean block at rest in zero gravity (asleep after
+    its first substep, sleep_threshold 0.05) and a second block thrown at it
+    at 6 cells/s; muscle_group_id holds each particle's index, a key that
+    survives the reference's partition swaps."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, 0.0)).replace(sleep_threshold=0.05)
+    m = Mats(backend)
+    sim = new_sim(backend, cfg).with_default_material(m.neohookean(400.0, 200.0)).with_boundary(eh.SlipBoundary.new(2))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(10, 10), box_center=(31.0, 24.0), rng_seed=3,
+                                position_jitter=0.2))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(8, 8), box_center=(18.0, 24.0), rng_seed=4,
+                                position_jitter=0.2))
+    p = sim.particles()
+    p["muscle_group_id"] = np.arange(len(p), dtype=np.uint32)
+    p["v"][p["user_tag"] == 2] = np.array([6.0, 0.0], np.float32)
+    sim.set_particles(p)
+    return sim
+
+
+def by_key(p):
+    """Particles in muscle_group_id order (undoes the sleep partition's swaps)."""
+    return p[np.argsort(p["muscle_group_id"], kind="stable")]
+
+
 def _add_velocity(sim, v):
     """Adds a uniform velocity to every particle (a thrown body)."""
     p = sim.particles()
