@@ -438,9 +438,9 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
 // broadcasts 15 and 31; VALU work, no LDS traffic), leaving each run's total
 // in its last lane, which alone issues the atomics.  A step adds the shifted
 // partial sum only if the source lane is in the same run: as fma(t, p, v)
-// with p = 1 or 0, which is exactly v + t or v when t is finite (2
-// instructions), or as a select (3 instructions) for waves whose terms are
-// not all small enough for every partial sum to stay finite.
+// with p = 1 or 0, which is exactly v + t or v when t is finite (one
+// v_fmac_f32_dpp per step), or as a select (3 instructions) for waves whose
+// terms are not all small enough for every partial sum to stay finite.
 struct SegMask {
     uint64_t heads;  // bit L: lane L starts a run
     uint32_t lane;
@@ -478,8 +478,40 @@ __device__ __forceinline__ float scan_step(float v, bool p, float f) {
     const float t = dpp_take<CTRL, ROWS>(v);
     return FMA ? __builtin_fmaf(t, f, v) : v + (p ? t : 0.0f);
 }
+// The fma form of the nine-value scan with the DPP read folded into the fma:
+// v_fmac_f32_dpp v, v, f computes fma(dpp(v), f, v), the same rounding as the
+// two-instruction form, in one VALU instruction per step, and a row_bcast
+// step leaves the rows it does not write unchanged instead of adding a zero
+// (which only differs in the sign of a zero partial sum, and the sums only
+// ever reach memory through additions to +0-initialised cells).  Step-major
+// order puts eight instructions between a value's write and its next DPP
+// read, more than the two wait states the read needs; the leading s_nop 1
+// covers whatever VALU write precedes the block.
+#define EMH_SCAN9(F, DPP)                                                                                   \
+    "v_fmac_f32_dpp %0, %0, " F " " DPP "\n\tv_fmac_f32_dpp %1, %1, " F " " DPP "\n\t"                     \
+    "v_fmac_f32_dpp %2, %2, " F " " DPP "\n\tv_fmac_f32_dpp %3, %3, " F " " DPP "\n\t"                     \
+    "v_fmac_f32_dpp %4, %4, " F " " DPP "\n\tv_fmac_f32_dpp %5, %5, " F " " DPP "\n\t"                     \
+    "v_fmac_f32_dpp %6, %6, " F " " DPP "\n\tv_fmac_f32_dpp %7, %7, " F " " DPP "\n\t"                     \
+    "v_fmac_f32_dpp %8, %8, " F " " DPP "\n\t"
+__device__ __forceinline__ void seg_scan_fma9(float (&v)[9], const ScanPred& q) {
+    asm("s_nop 1\n\t"
+        EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+        EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+        EMH_SCAN9("%11", "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+        EMH_SCAN9("%12", "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+        EMH_SCAN9("%13", "row_bcast:15 row_mask:0xa bank_mask:0xf")
+        EMH_SCAN9("%14", "row_bcast:31 row_mask:0xc bank_mask:0xf")
+        : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
+          "+&v"(v[8])
+        : "v"(q.f[0]), "v"(q.f[1]), "v"(q.f[2]), "v"(q.f[3]), "v"(q.f[4]), "v"(q.f[5]));
+}
+#undef EMH_SCAN9
 template <bool FMA, int N>
 __device__ __forceinline__ void seg_scan(float (&v)[N], const ScanPred& q) {
+    if constexpr (FMA && N == 9) {
+        seg_scan_fma9(v, q);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         float x = v[i];
