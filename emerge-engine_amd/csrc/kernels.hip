@@ -55,8 +55,18 @@ __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 __device__ __forceinline__ PBuf sel_buf(const Pipeline& p, uint32_t i) { return i ? p.buf[1] : p.buf[0]; }
 
+// Global float add (every caller passes grid memory): the address-space cast
+// keeps it a global atomic where the pointer was loaded from memory (the
+// batch kernels' per-scene pipelines) and would otherwise be a flat one.
 __device__ __forceinline__ void gatomic_add(float* addr, float v) {
-    __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) float*)addr, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+// LDS float add, workgroup scope.  Keeping it distinct from gatomic_add also
+// stops the compiler from merging an LDS add and a global add on two sides of
+// a branch into one flat atomic.
+__device__ __forceinline__ void latomic_add(float* addr, float v) {
+    __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -545,11 +555,18 @@ __device__ __forceinline__ bool run_tail(const SegMask& m) {
 struct Box {
     int ox, oy, w, h;
     bool lds;
-    __device__ __forceinline__ bool holds3x3(int bx, int by) const {  // cells bx-1..bx+1, by-1..by+1
-        const long long lx = (long long)bx - 1 - ox, ly = (long long)by - 1 - oy;
-        return lds && lx >= 0 && ly >= 0 && lx + 3 <= w && ly + 3 <= h;
-    }
     __device__ __forceinline__ int index(int cx, int cy) const { return (cx - ox) * h + (cy - oy); }
+    // LDS index of cell (bx - 1, by - 1) when the 3x3 stencil at base cell
+    // (bx, by) (cells bx-1..bx+1, by-1..by+1) lies in the box, else -1;
+    // stencil cell (gx, gy) is then at + gx * h + gy.  The test 0 <= bx - 1 -
+    // ox <= w - 3 runs on the wrapped 32-bit difference: the true difference
+    // of any int bx spans less than 2^32 values, so wrapping maps it one to
+    // one, with the negative ones above every w - 3 (an LDS box has
+    // w, h >= 3: margin >= 1).
+    __device__ __forceinline__ int stencil_at(int bx, int by) const {
+        const uint32_t lx = (uint32_t)bx - 1u - (uint32_t)ox, ly = (uint32_t)by - 1u - (uint32_t)oy;
+        return (lds && lx <= (uint32_t)(w - 3) && ly <= (uint32_t)(h - 3)) ? (int)(lx * (uint32_t)h + ly) : -1;
+    }
 };
 __device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_box) {
     mnx = wave_imin(mnx);
@@ -618,9 +635,10 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
 }
 // Segmented wave sum of one channel, then the run tails add it to the LDS box
 // (or to the global grid when the stencil leaves the box).
+// at: the stencil's LDS index (Box::stencil_at), -1 for the global grid.
 template <int CH>
 __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, const SegMask& sm, const ScanPred& sq,
-                                                bool fma_ok, const Box& box, float* s_ch, float* g_ch, int R) {
+                                                bool fma_ok, const Box& box, int at, float* s_ch, float* g_ch, int R) {
     float v[9];
     if (valid) {
         p2g_channel<CH>(t, v);
@@ -634,11 +652,12 @@ __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, c
         seg_scan<false>(v, sq);
     if (valid && run_tail(sm)) {
         const int bx = t.w.bx, by = t.w.by;
-        if (box.holds3x3(bx, by)) {
+        if (at >= 0) {
+            float* c = s_ch + at;
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-                for (int gy = 0; gy < 3; ++gy) atomicAdd(&s_ch[box.index(bx + gx - 1, by + gy - 1)], v[gx * 3 + gy]);
+                for (int gy = 0; gy < 3; ++gy) latomic_add(&c[gx * box.h + gy], v[gx * 3 + gy]);
         } else {
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
@@ -926,11 +945,12 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
     big = fmaxf(big, fmaxf(fmaxf(fabsf(t.tau.a), fabsf(t.tau.b)), fmaxf(fabsf(t.tau.c), fabsf(t.tau.d))));
     const bool small = !valid || (big < 1e18f && !__builtin_isnan(t.mass + t.coeff + t.vx + t.vy));
     const bool fma_ok = __ballot(!small) == 0ull;
-    scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, s_acc[kAccM], g[kAccM], R);
-    scatter_channel<kAccPx>(t, valid, sm, sq, fma_ok, box, s_acc[kAccPx], g[kAccPx], R);
-    scatter_channel<kAccPy>(t, valid, sm, sq, fma_ok, box, s_acc[kAccPy], g[kAccPy], R);
-    scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, s_acc[kAccSx], g[kAccSx], R);
-    scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, s_acc[kAccSy], g[kAccSy], R);
+    const int at = box.stencil_at(t.w.bx, t.w.by);
+    scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccM], g[kAccM], R);
+    scatter_channel<kAccPx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPx], g[kAccPx], R);
+    scatter_channel<kAccPy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPy], g[kAccPy], R);
+    scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSx], g[kAccSx], R);
+    scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSy], g[kAccSy], R);
 }
 
 // Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
@@ -1186,7 +1206,8 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 if (project_in && project_state(s, cfg)) ++n_proj;
                 // 3. gather_grid_to_particles phase 1 (g2p.rs:203-325)
                 const Wts w = weights(s.x, s.y);
-                const bool lds = box.holds3x3(w.bx, w.by);
+                const int at = box.stencil_at(w.bx, w.by);
+                const bool lds = at >= 0;
                 float nvx = 0.0f, nvy = 0.0f;
                 M2 b{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -1197,7 +1218,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                         const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
                         const float dx = ((float)cx - s.x) + 0.5f;
                         const float dy = ((float)cy - s.y) + 0.5f;
-                        const float2 nv = lds ? s_vel[box.index(cx, cy)] : cell_velocity(p, cur, dt, ddt, cx, cy);
+                        const float2 nv = lds ? s_vel[at + gx * box.h + gy] : cell_velocity(p, cur, dt, ddt, cx, cy);
                         const float wvx = nv.x * weight, wvy = nv.y * weight;
                         b.a = b.a + wvx * dx;
                         b.b = b.b + wvy * dx;
@@ -1270,7 +1291,8 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
             t.mass = sp.mass;
             const SegMask sm = run_mask(scat ? t.w.bx : INT_MIN + (int)lane_id(), scat ? t.w.by : 0);
             const bool fma_ok = __ballot(scat && !(fabsf(t.mass) < 1e18f)) == 0ull;
-            scatter_channel<kAccM>(t, scat, sm, scan_pred(sm), fma_ok, box, s_acc[0], gdst[0], R);
+            scatter_channel<kAccM>(t, scat, sm, scan_pred(sm), fma_ok, box, box.stencil_at(t.w.bx, t.w.by), s_acc[0],
+                                   gdst[0], R);
         }
         if (sliced) {
             flush_box<NCH>(box, s_acc, gdst, R);
@@ -1361,15 +1383,15 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
         if (valid && !asleep) {
             if (project_in && project_state(s, cfg)) ++n_proj;
             const Wts w = weights(s.x, s.y);
-            const bool lds = box.holds3x3(w.bx, w.by);
+            const int at = box.stencil_at(w.bx, w.by);
             float density = 0.0f;
             for (int gx = 0; gx < 3; ++gx)
                 for (int gy = 0; gy < 3; ++gy) {
                     const float weight = w.wx[gx] * w.wy[gy];
                     const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
                     float m = 0.0f;
-                    if (lds)
-                        m = s_dm[box.index(cx, cy)];
+                    if (at >= 0)
+                        m = s_dm[at + gx * box.h + gy];
                     else if (cx >= 0 && cy >= 0 && cx < R && cy < R)
                         m = dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy];
                     density = density + m * weight;

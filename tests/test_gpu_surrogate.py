@@ -195,12 +195,14 @@ def test_harvest_records_trial_states(oracle):
     assert np.all(rec[:, 8] == np.float32(mat.params.compression_limit))
     assert np.all(rec[:, 9] == np.float32(mat.params.hardening_exponent))
     assert np.all(np.isfinite(rec[:, :5]))
-    # harvesting leaves the simulation as it is (two GPU runs differ only by
-    # the order of their float atomics)
+    # harvesting leaves the simulation as it is.  Two GPU runs differ only by
+    # the order of their float atomics, which over the few hundred substeps
+    # of three steps has moved a particle by up to ~1.2e-4 cell; a harvest
+    # that wrote into the state would move them by far more
     plain = S.nacc_block("gpu")
     plain.step_n(3)
     dx = np.abs(plain.particles()["x"].astype(np.float64) - sim.particles()["x"]).max()
-    assert dx < 1e-4, dx
+    assert dx < 1e-3, dx
     # the training set holds exactly the yielding records, labelled by the plain mapping
     n_set = sg.build_training_set()
     om = _soil_oracle(oracle)
