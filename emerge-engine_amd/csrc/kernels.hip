@@ -1210,22 +1210,34 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 const bool lds = at >= 0;
                 float nvx = 0.0f, nvy = 0.0f;
                 M2 b{0.0f, 0.0f, 0.0f, 0.0f};
+                auto gather = [&](int gx, int gy, float2 nv) {
+                    const float weight = w.wx[gx] * w.wy[gy];
+                    const float dx = ((float)(w.bx + gx - 1) - s.x) + 0.5f;
+                    const float dy = ((float)(w.by + gy - 1) - s.y) + 0.5f;
+                    const float wvx = nv.x * weight, wvy = nv.y * weight;
+                    b.a = b.a + wvx * dx;
+                    b.b = b.b + wvy * dx;
+                    b.c = b.c + wvx * dy;
+                    b.d = b.d + wvy * dy;
+                    nvx = nvx + wvx;
+                    nvy = nvy + wvy;
+                };
+                // the nine LDS reads go out together; a lane whose stencil
+                // leaves the box reads from base 0 and replaces what it read
+                // (in bounds: 2 h + 2 < w h <= kLdsCells for an LDS box,
+                // h = 0 otherwise)
+                const int lb = lds ? at : 0;
+                float2 nv[9];
+#pragma unroll
+                for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+                    for (int gy = 0; gy < 3; ++gy) nv[gx * 3 + gy] = s_vel[lb + gx * box.h + gy];
 #pragma unroll
                 for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
                     for (int gy = 0; gy < 3; ++gy) {
-                        const float weight = w.wx[gx] * w.wy[gy];
-                        const int cx = w.bx + gx - 1, cy = w.by + gy - 1;
-                        const float dx = ((float)cx - s.x) + 0.5f;
-                        const float dy = ((float)cy - s.y) + 0.5f;
-                        const float2 nv = lds ? s_vel[at + gx * box.h + gy] : cell_velocity(p, cur, dt, ddt, cx, cy);
-                        const float wvx = nv.x * weight, wvy = nv.y * weight;
-                        b.a = b.a + wvx * dx;
-                        b.b = b.b + wvy * dx;
-                        b.c = b.c + wvx * dy;
-                        b.d = b.d + wvy * dy;
-                        nvx = nvx + wvx;
-                        nvy = nvy + wvy;
+                        if (!lds) nv[gx * 3 + gy] = cell_velocity(p, cur, dt, ddt, w.bx + gx - 1, w.by + gy - 1);
+                        gather(gx, gy, nv[gx * 3 + gy]);
                     }
                 if (s.mat & kPinnedBit) {
                     s.vx = 0.0f;
