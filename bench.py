@@ -320,7 +320,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="extra steps after the timed ones, added to the live kernel timing only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the NACC surrogate section")
@@ -343,8 +344,8 @@ def main() -> int:
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the substep has no multi-GPU "
-              "path in this build; see the sharding plan in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the substep path runs on a "
+              "single GPU", file=sys.stderr)
         return 2
 
     t_build = time.perf_counter()
@@ -355,6 +356,9 @@ def main() -> int:
     first_dt = sim.effective_dt()
     build_s = time.perf_counter() - t_build
 
+    # live per-kernel timing over the timed steps: HIP events on the
+    # library's stream, two per step, recorded without stopping the queue
+    sim.set_profiling(True)
     s0 = sim.stats().total_substeps
     t0 = time.perf_counter()
     sim.step_n(args.steps)
@@ -364,11 +368,9 @@ def main() -> int:
     elapsed = t1 - t0
     substeps = s1 - s0
     rate = n * substeps / elapsed
-
-    # live per-kernel timing (HIP events on the library's stream)
-    sim.set_profiling(True)
-    sim.step_n(args.profile_steps)
-    sim.synchronize()
+    if args.profile_steps > 0:
+        sim.step_n(args.profile_steps)
+        sim.synchronize()
     times = sim.kernel_times()
     sim.set_profiling(False)
     active = {k: v for k, v in times.items() if v[0] > 0}

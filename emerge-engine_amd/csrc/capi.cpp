@@ -368,15 +368,23 @@ struct emh_sim {
     }
     SgDev sg_host{};  // the descriptor Ctrl::sg holds
 
-    // Live timing: three events per step on the library stream -- before
-    // the step's set-up launches (k_begin_step and, on reorder steps, the
-    // key / scan kernels), before its first substep launch and after its
-    // last.  The substep launches run back to back, so their span divided by
-    // the substeps taken is the substep kernel's average duration.
+    // Live timing: event marks on the library stream, recorded without
+    // stopping the queue -- before each step's set-up launches (k_begin_step
+    // and, on reorder steps, the key / scan kernels), before its substep
+    // launches, and where a drain stops to read the control block (and around
+    // the extra substep launches a drain makes).  A mark's kind says what the
+    // span up to the next mark holds; a drain folds the spans once its stream
+    // synchronisation has completed every mark.  The substep launches run back
+    // to back, so their spans divided by the launches is the substep kernel's
+    // average duration (inter-kernel gaps included, no host wait).
+    enum : uint8_t { kMarkSetup = 0, kMarkSubsteps = 1, kMarkIdle = 2 };
     bool profiling = false;
-    hipEvent_t prof_ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<uint8_t> prof_kind;
+    size_t prof_used = 0;
     double prof_ms[2] = {0.0, 0.0};       // set-up, substep kernels
-    uint64_t prof_count[2] = {0, 0};      // steps, substeps
+    uint64_t prof_count[2] = {0, 0};      // steps, substeps taken
+    uint32_t prof_sub_base = 0;           // total_substeps at the last fold
 
     // device
     int device = 0;
@@ -402,14 +410,36 @@ struct emh_sim {
         cap = 0;
     }
     void release_events() {
-        for (auto& e : prof_ev) {
-            if (e) (void)hipEventDestroy(e);
-            e = nullptr;
-        }
+        for (auto& e : prof_pool) (void)hipEventDestroy(e);
+        prof_pool.clear();
+        prof_kind.clear();
+        prof_used = 0;
     }
-    int ensure_events() {
-        for (auto& e : prof_ev)
-            if (!e) HIP_TRY(hipEventCreate(&e));
+    // Record a mark of `kind` on the stream (no-op unless profiling).
+    int prof_mark(uint8_t kind) {
+        if (!profiling) return EMH_OK;
+        if (prof_used == prof_pool.size()) {
+            // timing only (read after a stream synchronisation): no system-scope
+            // fence, whose cache writeback would slow the kernels that follow
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            prof_pool.push_back(e);
+            prof_kind.push_back(kMarkIdle);
+        }
+        HIP_TRY(hipEventRecord(prof_pool[prof_used], stream));
+        prof_kind[prof_used++] = kind;
+        return EMH_OK;
+    }
+    // Fold the recorded spans into the totals; every mark must have completed
+    // (the caller has synchronised the stream after the last one).
+    int prof_fold() {
+        for (size_t i = 0; i + 1 < prof_used; ++i) {
+            if (prof_kind[i] == kMarkIdle) continue;
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, prof_pool[i], prof_pool[i + 1]));
+            prof_ms[prof_kind[i] == kMarkSetup ? 0 : 1] += ms;
+        }
+        prof_used = 0;
         return EMH_OK;
     }
 
@@ -881,7 +911,10 @@ struct emh_sim {
         uint32_t extra = 0;
         const uint32_t limit = (cfg.max_substeps_per_step + 1) * (kStepRing + 1);
         while (true) {
-            if (profiling) HIP_TRY(hipEventRecord(prof_ev[2], stream));
+            {
+                int rc = prof_mark(kMarkIdle);
+                if (rc) return rc;
+            }
             HIP_TRY(hipMemcpyAsync(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
             st = c.st[seq & 1u];  // the state the next launch would decide from
@@ -889,16 +922,25 @@ struct emh_sim {
             uint32_t more = (running(st) ? 1u : 0u) + st.pending * std::max<uint32_t>(predicted_substeps, 1u);
             more = std::min<uint32_t>(std::max<uint32_t>(more, 1u), 64u);
             if (extra + more > limit) return fail(EMH_ERR_DEVICE, "substep loop did not terminate");
+            {
+                int rc = prof_mark(kMarkSubsteps);
+                if (rc) return rc;
+            }
             for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
             extra += more;
         }
-        return publish(c, st, profiling);
+        if (profiling) {
+            int rc = prof_fold();
+            if (rc) return rc;
+            prof_count[1] += (uint32_t)(st.total_substeps - prof_sub_base);
+            prof_sub_base = st.total_substeps;
+        }
+        return publish(c, st);
     }
 
     // The records of the last finished step (step.rs:51-59) from the control
     // block the loop left; st = c.st[seq & 1].
-    // timed: fold the live-timing events of this drain into the profile.
-    int publish(const Ctrl& c, const CtrlState& st, bool timed) {
+    int publish(const Ctrl& c, const CtrlState& st) {
         // the last step: finished but not yet recorded if no launch followed it
         uint32_t subs, step;
         float last_dt, dropped;
@@ -912,15 +954,6 @@ struct emh_sim {
             step = st.done_step;
             last_dt = st.done_dt;
             dropped = st.done_dropped;
-        }
-        if (timed) {
-            float a = 0.0f, b = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&a, prof_ev[0], prof_ev[1]));
-            HIP_TRY(hipEventElapsedTime(&b, prof_ev[1], prof_ev[2]));
-            prof_ms[0] += a;
-            prof_count[0] += 1;
-            prof_ms[1] += b;
-            prof_count[1] += subs;
         }
         predicted_substeps = subs;
         stats.last_substeps = subs;
@@ -964,26 +997,29 @@ struct emh_sim {
             }
             return EMH_OK;
         }
-        if (profiling) {
-            int rc = ensure_events();
-            if (rc) return rc;
-        }
         // Queue every step without waiting: the device loop starts each queued
         // step once the previous one has finished.  Drain (wait and, if some
         // step needed more substep launches than queued, launch more) every
-        // kDrainSteps steps, at the end, and after every step when profiling.
+        // kDrainSteps steps and at the end.
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
             const bool reorder = substeps_since_reorder >= kReorderSubsteps;
-            if (profiling) HIP_TRY(hipEventRecord(prof_ev[0], stream));
+            {
+                int rc = prof_mark(kMarkSetup);
+                if (rc) return rc;
+            }
             HIP_TRY(launch_begin_step(pipe, seq++, reorder, stream));
-            if (profiling) HIP_TRY(hipEventRecord(prof_ev[1], stream));
+            {
+                int rc = prof_mark(kMarkSubsteps);
+                if (rc) return rc;
+            }
             if (reorder) substeps_since_reorder = 0;
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
             for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
+            if (profiling) prof_count[0] += 1;
             substeps_since_reorder += slots;
-            if (profiling || (s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
+            if ((s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
                 int rc = drain();
                 if (rc) return rc;
             }
@@ -1709,6 +1745,8 @@ int emh_sim_synchronize(emh_sim* s) {
 int emh_sim_set_profiling(emh_sim* s, int enable) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     s->profiling = enable != 0;
+    s->prof_used = 0;
+    s->prof_sub_base = s->stats.total_substeps;
     for (int k = 0; k < 2; ++k) {
         s->prof_ms[k] = 0.0;
         s->prof_count[k] = 0;
@@ -2461,7 +2499,7 @@ struct emh_batch {
         }
         predicted = 1;
         for (uint32_t i = 0; i < n; ++i) {
-            int rc = sims[i]->publish(h_ctrl[i], h_ctrl[i].st[seq & 1u], false);
+            int rc = sims[i]->publish(h_ctrl[i], h_ctrl[i].st[seq & 1u]);
             if (rc) return rc;
             predicted = std::max<uint32_t>(predicted, sims[i]->predicted_substeps);
         }

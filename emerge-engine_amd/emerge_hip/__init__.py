@@ -859,7 +859,7 @@ class Simulation:
         return self.stats().last_substeps
 
     def set_profiling(self, enable: bool) -> None:
-        """Step-level HIP-event timing on the library stream (resets the totals)."""
+        """Step-level HIP-event timing on the library stream, without stopping its queue (resets the totals)."""
         _check(load_library().emh_sim_set_profiling(self._h, int(bool(enable))))
 
     def kernel_times(self) -> dict:

@@ -411,12 +411,14 @@ int emh_sim_apply_impulse(emh_sim* sim, float cx, float cy, float radius, float 
 int emh_sim_apply_radial_impulse(emh_sim* sim, float cx, float cy, float radius, float strength);
 
 /* ---- per-kernel timing ---------------------------------------------------
- * HIP events on the library's stream, three per step: two entries, summed
- * over the steps taken since enabling -- "step_setup" (launches = steps): the
- * step's start kernel and, on reorder steps, the cell-key and scan kernels;
- * "k_substep" ("k_density_p2g+k_substep" when a density pass runs; launches =
- * substeps taken): the back-to-back substep launches of each step.  The
- * counterpart of GpuSimulation::enable_profiling / last_pass_timings_ns
+ * HIP events on the library's stream, two per step and one where a step loop
+ * stops to read its control block, recorded without stopping the queue (the
+ * spans are read at those stops).  Two entries, summed over the steps taken
+ * since enabling -- "step_setup" (launches = steps): the step's start kernel
+ * and, on reorder steps, the cell-key and scan kernels; "k_substep"
+ * ("k_density_p2g+k_substep" when a density pass runs; launches = substeps
+ * taken): the back-to-back substep launches of each step.  The counterpart of
+ * GpuSimulation::enable_profiling / last_pass_timings_ns
  * (src/systems/gpu/solver/profiling.rs:9-75).  Enabling resets the totals. */
 typedef struct emh_kernel_time {
     char     name[32];
