@@ -185,6 +185,32 @@ def test_full_size_invariants():
 
 
 @pytest.mark.gpu
+def test_multi_material_full_size_invariants():
+    """Config 3 at full size (1024^2, ~4M particles of four models, density
+    pass for the water): every particle returned once with its mass, id and
+    material, state finite and inside the domain, grid mass = particle mass,
+    every model's particles moved."""
+    sim = S.multi_material_full("gpu")
+    p0 = sim.particles()
+    n = len(p0)
+    assert 3_500_000 < n < 4_600_000, n
+    sim.step_n(2)
+    assert sim.stats().last_substeps >= 1
+    p = sim.particles()
+    assert len(p) == n
+    for f in ("mass", "user_tag", "material_id"):
+        assert np.array_equal(p[f], p0[f]), f
+    assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["v"])) and np.all(np.isfinite(p["deformation_gradient"]))
+    assert p["x"].min() >= 1.0 and p["x"].max() <= 1022.0
+    gm, _ = sim.grid()
+    total = p["mass"].astype(np.float64).sum()
+    assert abs(gm.astype(np.float64).sum() - total) <= 1e-5 * total
+    for mid in range(4):
+        sel = p["material_id"] == mid
+        assert sel.sum() > 0 and np.any(p["x"][sel] != p0["x"][sel]), mid
+
+
+@pytest.mark.gpu
 def test_error_paths():
     cfg = eh.SimConfig.standard(64, 0.05, (0.0, -1.0))
     sim = eh.Simulation.empty(cfg)
