@@ -305,7 +305,7 @@ template <class T> struct Particles {
 };
 
 // ------------------------------------------------------------------ materials
-// One struct covers the six north-star models plus Fallback; `model` picks the
+// One struct covers the six core constitutive models plus Fallback; `model` picks the
 // branch.  Fields are the model structs' own (not the 96 B upload), rebuilt
 // from MaterialParams + the extension exactly as each `params()` encodes them.
 template <class T> struct Material {

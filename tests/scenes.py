@@ -128,7 +128,7 @@ class Scene:
 
 
 def quickstart(backend: str):
-    """BASELINE config 1 == the crate's README quick start (README.md:20-51)."""
+    """The crate's README quick start (README.md:20-51)."""
     cfg = eh.SimConfig.standard(64, 0.05, (0.0, -1.0))
     m = Mats(backend)
     sim = (new_sim(backend, cfg)
@@ -160,7 +160,7 @@ def quickstart_impact(backend: str):
 
 
 def sand_column(backend: str, res=64, width=16, height=24, spacing=0.5, dt=0.02, seed=7, g=-9.81):
-    """BASELINE config 2 in miniature: Drucker-Prager sand (E = 3.5e5, nu = 0.3,
+    """The sand-column benchmark in miniature: Drucker-Prager sand (E = 3.5e5, nu = 0.3,
     friction 30 deg) column on the floor, jittered by +/-0.25 cell."""
     cfg = eh.SimConfig.standard(res, dt, (0.0, g))
     m = Mats(backend)
@@ -343,7 +343,7 @@ def predictive_bounce(backend: str):
 
 
 def multi_material(backend: str):
-    """BASELINE config 3 in miniature: NeoHookean, water, snow and corotated
+    """The multi-material benchmark scene in miniature: NeoHookean, water, snow and corotated
     boxes with N(0, 0.5)-like random velocities, one grid."""
     cfg = eh.SimConfig.standard(96, 0.02, (0.0, -9.81))
     m = Mats(backend)
@@ -364,7 +364,7 @@ def multi_material(backend: str):
 
 
 def multi_material_full(backend: str = "gpu"):
-    """BASELINE config 3 at its stated size: 1024^2, ~4M particles.  The stated
+    """The multi-material benchmark scene at its stated size: 1024^2, ~4M particles.  The stated
     40 boxes with sides U[8, 96] at 4 per cell hold ~0.54M particles, so the
     box count is scaled to 370 (E[side^2] = 3,349 cells; with this seed 3.98M
     particles);

@@ -186,7 +186,7 @@ def test_full_size_invariants():
 
 @pytest.mark.gpu
 def test_multi_material_full_size_invariants():
-    """Config 3 at full size (1024^2, ~4M particles of four models, density
+    """The multi-material scene at full size (1024^2, ~4M particles of four models, density
     pass for the water): every particle returned once with its mass, id and
     material, state finite and inside the domain, grid mass = particle mass,
     every model's particles moved."""

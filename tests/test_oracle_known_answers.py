@@ -1,7 +1,7 @@
 """Pins the CPU oracle before it is trusted as the checker.
 
-The reference ships no golden vectors (SURVEY: no fixtures exist anywhere in
-the crate); what it does ship are known-answer and invariant unit tests next
+The reference ships no golden vectors (no fixtures exist anywhere in the
+crate); what it does ship are known-answer and invariant unit tests next
 to the math.  Each test below restates one of them against the oracle (file:
 line of the reference test in the docstring), plus checks the mathematics
 fixes on its own (closed forms, textbook SVD, conservation), chosen so that a
