@@ -191,6 +191,7 @@ extern "C" {
     fn emh_sim_num_particles(sim: *const EmhSim, n: *mut usize) -> c_int;
     fn emh_sim_get_particles(sim: *mut EmhSim, out: *mut Particle, n: usize) -> c_int;
     fn emh_sim_step(sim: *mut EmhSim, n_steps: u32) -> c_int;
+    fn emh_sim_set_reorder_interval(sim: *mut EmhSim, substeps: u32) -> c_int;
     fn emh_sim_stats(sim: *const EmhSim, out: *mut EmhStepStats) -> c_int;
     fn emh_sim_get_grid(sim: *mut EmhSim, mass: *mut f32, momentum: *mut f32, n_cells: usize) -> c_int;
     // learned NACC return mapping
@@ -325,6 +326,12 @@ impl HipSimulation {
     /// `step_n` (step.rs:327-331), queued on the device and synchronised once.
     pub fn step_n(&mut self, steps: usize) -> Result<(), HipError> {
         check(unsafe { emh_sim_step(self.raw, steps as u32) })
+    }
+
+    /// Physical cell-key reorder interval in substeps (no counterpart in the
+    /// reference; the upload is always reordered at the first step).
+    pub fn set_reorder_interval(&mut self, substeps: u32) -> Result<(), HipError> {
+        check(unsafe { emh_sim_set_reorder_interval(self.raw, substeps) })
     }
     pub fn set_particles(&mut self, particles: &[Particle]) -> Result<(), HipError> {
         check(unsafe { emh_sim_set_particles(self.raw, particles.as_ptr(), particles.len()) })

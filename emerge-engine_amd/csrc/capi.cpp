@@ -358,6 +358,8 @@ struct emh_sim {
     emh_step_stats stats{};
     uint32_t predicted_substeps = 1;
     uint32_t substeps_since_reorder = 0;
+    uint32_t reorder_substeps = kReorderSubsteps;  // emh_sim_set_reorder_interval
+    bool reorder_pending = false;                  // an upload: the next step reorders
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
     // learned NACC return mapping: 0 plain, 1 harvest trial states, 2 infer
     uint32_t sg_mode = 0;
@@ -741,7 +743,7 @@ struct emh_sim {
         HIP_TRY(launch_zero_all(pipe, stream));
         HIP_TRY(launch_prologue(pipe, seq++, stream, sleep));
         HIP_TRY(hipStreamSynchronize(stream));
-        substeps_since_reorder = kReorderSubsteps;  // first step puts the upload in cell-key order
+        reorder_pending = true;  // the first step puts the upload in cell-key order
         host_dirty = false;
         device_ahead = false;
         cold_valid = false;
@@ -1003,7 +1005,7 @@ struct emh_sim {
         // kDrainSteps steps and at the end.
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
-            const bool reorder = substeps_since_reorder >= kReorderSubsteps;
+            const bool reorder = reorder_pending || substeps_since_reorder >= reorder_substeps;
             {
                 int rc = prof_mark(kMarkSetup);
                 if (rc) return rc;
@@ -1013,7 +1015,10 @@ struct emh_sim {
                 int rc = prof_mark(kMarkSubsteps);
                 if (rc) return rc;
             }
-            if (reorder) substeps_since_reorder = 0;
+            if (reorder) {
+                substeps_since_reorder = 0;
+                reorder_pending = false;
+            }
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
             for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
@@ -1739,6 +1744,13 @@ int emh_sim_snapshot(emh_sim* s, emh_snapshot* out) {  // snapshot.rs:227-372, q
 int emh_sim_synchronize(emh_sim* s) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return EMH_OK;
+}
+
+int emh_sim_set_reorder_interval(emh_sim* s, uint32_t substeps) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    if (substeps == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
+    s->reorder_substeps = substeps;
     return EMH_OK;
 }
 

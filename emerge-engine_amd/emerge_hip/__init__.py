@@ -246,6 +246,7 @@ def load_library() -> ctypes.CDLL:
         "emh_sim_snapshot": (ctypes.c_int, [ctypes.c_void_p, P(CSnapshot)]),
         "emh_sim_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "emh_sim_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "emh_sim_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
                                                 P(ctypes.c_uint32)]),
         "emh_surrogate_create": (ctypes.c_int, [ctypes.c_uint64, P(ctypes.c_void_p)]),
@@ -861,6 +862,10 @@ class Simulation:
     def set_profiling(self, enable: bool) -> None:
         """Step-level HIP-event timing on the library stream, without stopping its queue (resets the totals)."""
         _check(load_library().emh_sim_set_profiling(self._h, int(bool(enable))))
+
+    def set_reorder_interval(self, substeps: int) -> None:
+        """Physical cell-key reorder every `substeps` substeps (and at the first step after an upload)."""
+        _check(load_library().emh_sim_set_reorder_interval(self._h, int(substeps)))
 
     def kernel_times(self) -> dict:
         """{"step_setup": (steps, ms), "k_substep": (substeps, ms)} since profiling was enabled."""

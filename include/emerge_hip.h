@@ -426,6 +426,15 @@ typedef struct emh_kernel_time {
     double   total_ms;
 } emh_kernel_time;
 int emh_sim_set_profiling(emh_sim* sim, int enable);
+
+/* ---- particle order -------------------------------------------------------
+ * No counterpart in the reference (its CPU path has no particle sort; the WGSL
+ * path re-sorts every frame, gpu/solver/step.rs:339-385).  The particles are
+ * physically reordered into cell-key order at the first step after an upload
+ * and then at a step start once `substeps` substeps have run since the last
+ * reorder (default 4096; >= 1).  Results do not depend on it beyond f32
+ * summation order; it trades the reorder's cost against chunk locality. */
+int emh_sim_set_reorder_interval(emh_sim* sim, uint32_t substeps);
 int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
 
 /* ---- the cell-key sort on its own ---------------------------------------

@@ -25,7 +25,8 @@ def test_export_particles_matches_download():
     p0["user_tag"] = np.arange(len(p0), dtype=np.uint32) * 7 + 3
     p0["temperature"] = np.linspace(-1.0, 1.0, len(p0), dtype=np.float32)
     sim.set_particles(p0)
-    sim.step_n(12)  # crosses the 8-substep reorder
+    sim.set_reorder_interval(8)
+    sim.step_n(12)  # the upload's reorder, then reorders every 8 substeps
     n = len(p0)
     buf = torch.zeros(n * 128 + 256, dtype=torch.uint8, device="cuda")
     sim.export_particles_device(buf.data_ptr(), n)
