@@ -43,8 +43,8 @@ ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 # HBM bytes per k_substep launch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (one run each, mean over launches) and scaled by
 # the same counters' reading for a 256 MiB device-to-device copy
-# (profiles/k_substep_sand_1m_512.md): 27042 KiB x 2.000 + 78950 KiB x 1.000.
-MEASURED_TRAFFIC = {"k_substep": (27042.1 * 2.000 + 78949.8 * 1.000) * 1024.0}
+# (profiles/k_substep_sand_1m_512.md): 27728 KiB x 2.000 + 80394 KiB x 1.000.
+MEASURED_TRAFFIC = {"k_substep": (27728.2 * 2.000 + 80394.4 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu"):
