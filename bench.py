@@ -277,6 +277,7 @@ def batch_section(args) -> dict:
     together = build_batch_scenes(31, args.batch_scenes)
     n = np.array([s.num_particles() for s in alone], dtype=np.float64)
     for s in alone:
+        s.set_reorder_interval(8)  # the batch path's reorder interval
         s.step_n(1)
     batch = eh.SceneBatch(together)
     batch.step_n(1)

@@ -2412,7 +2412,7 @@ struct emh_batch {
     std::vector<Ctrl> h_ctrl;
     BatchLists L{};
     uint64_t seq = 0;
-    uint32_t substeps_since_reorder = kReorderSubsteps;
+    uint32_t substeps_since_reorder = kBatchReorderSubsteps;
     uint32_t predicted = 1;
     uint32_t max_substeps = 1;
 
@@ -2476,7 +2476,7 @@ struct emh_batch {
         L.fused_blocks = fused[n];
         L.density_blocks = dens[n];
         seq = 1;
-        substeps_since_reorder = kReorderSubsteps;
+        substeps_since_reorder = kBatchReorderSubsteps;
         return EMH_OK;
     }
 
@@ -2525,7 +2525,7 @@ struct emh_batch {
         }
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
-            const bool reorder = substeps_since_reorder >= kReorderSubsteps;
+            const bool reorder = substeps_since_reorder >= kBatchReorderSubsteps;
             HIP_TRY(launch_batch_begin_step(L, seq++, reorder, stream));
             if (reorder) substeps_since_reorder = 0;
             const uint32_t slots = std::min<uint32_t>(std::max<uint32_t>(predicted, 1u), max_substeps);
