@@ -54,6 +54,22 @@ template <class T> inline bool finite(T x) { return std::isfinite(x); }
 
 constexpr float F32_EPSILON = 1.1920929e-07f;  // f32::EPSILON
 
+// Planted mistakes for the oracle's own tests: a test switches one on to show
+// that its closed-form check fails on a plausible error, then switches it off.
+// 0 (the default) is the faithful restatement.
+enum Mutation : int {
+    kMutNone = 0,
+    kMutCflNoAffine = 1,       // choose_substep_dt without the affine speed term
+    kMutCflNoMinClamp = 2,     // cfl_bound without the lower clamp to min(min_dt, max_dt)
+    kMutProjJScaleLinear = 3,  // project: F scaled by j_max/J instead of sqrt(j_max/J)
+    kMutProjNoDensityFloor = 4,  // project: a valid density is not floored
+    kMutNaccOtherRoot = 5,     // NACC hardening: the intersection on the far side of p_c
+    kMutNaccCaseANoAlpha = 6,  // NACC Case A: alpha not updated
+    kMutNaccHardenSign = 7,    // NACC hardening: alpha -= ln(J_tr / J_x)
+};
+inline int g_mutation = kMutNone;
+inline bool mutated(int m) { return g_mutation == m; }
+
 // ---------------------------------------------------------------- glam mirror
 template <class T> struct V2 {
     T x, y;
@@ -751,7 +767,7 @@ template <class T> struct Material {
         if (p_tr > p0) {  // case A
             T j_n1 = o_sqrt(rmax(T(-2) * p0 / kappa + T(1), T(1.0e-8f)));
             T sv_new = o_pow(j_n1, T(0.5));
-            if (hardening_enabled) alpha = alpha + o_ln(j_e_tr / j_n1);
+            if (hardening_enabled && !mutated(kMutNaccCaseANoAlpha)) alpha = alpha + o_ln(j_e_tr / j_n1);
             f_out = reconstruct({sv_new, sv_new});
             alpha_out = alpha;
             return;
@@ -786,8 +802,10 @@ template <class T> struct Material {
             T p1 = p_c + l1 * dir.x;
             T p2 = p_c + l2 * dir.x;
             T p_x = ((p_tr - p_c) * (p1 - p_c) > T(0)) ? p1 : p2;
+            if (mutated(kMutNaccOtherRoot)) p_x = (p_x == p1) ? p2 : p1;
             T j_e_x = o_sqrt(rmax(std::fabs(T(-2) * p_x / kappa + T(1)), T(1.0e-8f)));
-            if (j_e_x > T(1.0e-4f)) alpha = alpha + o_ln(j_e_tr / j_e_x);
+            if (j_e_x > T(1.0e-4f))
+                alpha = mutated(kMutNaccHardenSign) ? alpha - o_ln(j_e_tr / j_e_x) : alpha + o_ln(j_e_tr / j_e_x);
         }
         T scale = o_sqrt(rmax(-y1 / rmax(y0, T(1.0e-10f)), T(0))) * (j_e_tr / mu);
         V2<T> b_n1 = scale * normalize_or_zero(s_tr) + V2<T>{sv_sq_trace / T(2), sv_sq_trace / T(2)};
@@ -1496,7 +1514,7 @@ template <class T> struct Simulation {
         T min_mat_dt = max_dt;
         for (size_t i = 0; i < active_count; ++i) {
             T s = length(P.v[i]);
-            if (cfg.cfl_include_affine_speed) s = s + affine_cfl(P.c[i]);
+            if (cfg.cfl_include_affine_speed && !mutated(kMutCflNoAffine)) s = s + affine_cfl(P.c[i]);
             max_speed = rmax(max_speed, s);
             T mdt = mat(P.material_id[i])
                         .timestep_bound(P.density[i], P.hardening_scale[i], T(cfg.grid_cell_size),
@@ -1506,6 +1524,7 @@ template <class T> struct Simulation {
         T dt = max_dt;
         if (max_speed > T(F32_EPSILON)) dt = rmin(dt, T(cfg.cfl_coefficient) * T(cfg.grid_cell_size) / max_speed);
         dt = rmin(dt, min_mat_dt);
+        if (mutated(kMutCflNoMinClamp)) return rmin(dt, max_dt);
         return rclamp(dt, rmin(T(cfg.min_dt), max_dt), max_dt);
     }
 
@@ -1537,7 +1556,7 @@ template <class T> struct Simulation {
         } else {
             T j = det(f);
             if (j > T(cfg.j_max)) {
-                P.f[i] = P.f[i] * o_sqrt(T(cfg.j_max) / j);
+                P.f[i] = P.f[i] * (mutated(kMutProjJScaleLinear) ? T(cfg.j_max) / j : o_sqrt(T(cfg.j_max) / j));
                 projected = true;
             }
         }
@@ -1572,7 +1591,7 @@ template <class T> struct Simulation {
         if (!finite(P.density[i]) || P.density[i] <= T(0)) {
             P.density[i] = rmax(P.mass[i] / P.volume[i], T(cfg.projection_min_density));
             projected = true;
-        } else {
+        } else if (!mutated(kMutProjNoDensityFloor)) {
             P.density[i] = rmax(P.density[i], T(cfg.projection_min_density));
         }
         return projected;

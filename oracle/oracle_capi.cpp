@@ -340,6 +340,30 @@ int orc_p2g_only(void* hv, float dt, double* mass, double* momentum, size_t n_ce
     }) == 0 ? orc_get_grid(hv, mass, momentum, n_cells) : EMH_ERR_STATE;
 }
 
+// choose_substep_dt (step.rs:434-479) over the current state, for max_dt.
+double orc_choose_substep_dt(void* hv, float max_dt) {
+    Handle* h = (Handle*)hv;
+    if (!h) return -1.0;
+    return with_sim(h, [&](auto& s) {
+        using T = typename std::decay_t<decltype(s.P.mass)>::value_type;
+        return (double)s.choose_substep_dt(T(max_dt));
+    });
+}
+
+// project_particle_state_to_admissible (step.rs:347-429) of particle i, in
+// place; returns 1 if a field was corrected, 0 if not, -1 on a bad index.
+int orc_project_particle(void* hv, size_t i) {
+    Handle* h = (Handle*)hv;
+    if (!h) return -1;
+    return with_sim(h, [&](auto& s) {
+        if (i >= s.P.mass.size()) return -1;
+        return s.project(i) ? 1 : 0;
+    });
+}
+
+// Planted mistakes (Mutation in mpm_oracle.hpp) for the oracle's own tests.
+void orc_set_mutation(int m) { g_mutation = m; }
+
 // Seconds of wall time for `n` full steps (the CPU baseline timer).
 double orc_time_steps(void* hv, uint32_t n) {
     auto t0 = std::chrono::steady_clock::now();

@@ -80,6 +80,9 @@ def load() -> ctypes.CDLL:
         "orc_substep_dts": (ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
         "orc_get_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
         "orc_p2g_only": (ctypes.c_int, [ctypes.c_void_p, f, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "orc_choose_substep_dt": (ctypes.c_double, [ctypes.c_void_p, f]),
+        "orc_project_particle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
+        "orc_set_mutation": (None, [ctypes.c_int]),
         "orc_time_steps": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_uint32]),
         "orc_time_substeps": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_uint32, f]),
         "orc_quadratic_weights": (None, [f, f, P(ctypes.c_int32), P(f), P(f)]),
@@ -248,6 +251,17 @@ class OracleSim:
     def substeps(self, n: int, dt: float):
         self._ok(load().orc_substeps(self._h, n, dt))
 
+    def choose_substep_dt(self, max_dt: float) -> float:
+        """choose_substep_dt (step.rs:434-479) over the current state."""
+        return float(load().orc_choose_substep_dt(self._h, max_dt))
+
+    def project_particle(self, i: int) -> bool:
+        """project_particle_state_to_admissible (step.rs:347-429) of particle i, in place."""
+        r = int(load().orc_project_particle(self._h, i))
+        if r < 0:
+            raise IndexError(i)
+        return r == 1
+
     def time_steps(self, n: int) -> float:
         return float(load().orc_time_steps(self._h, n))
 
@@ -388,6 +402,26 @@ def material(kind: str, *args) -> Material:
     p, e = CMaterialParams(), CMaterialExt()
     getattr(load(), f"orc_material_{kind}")(*args, ctypes.byref(p), ctypes.byref(e))
     return Material(p, e)
+
+
+# Planted mistakes of the oracle (mpm_oracle.hpp Mutation), for its own tests.
+MUT_NONE, MUT_CFL_NO_AFFINE, MUT_CFL_NO_MIN_CLAMP, MUT_PROJ_JSCALE_LINEAR = 0, 1, 2, 3
+MUT_PROJ_NO_DENSITY_FLOOR, MUT_NACC_OTHER_ROOT, MUT_NACC_CASE_A_NO_ALPHA, MUT_NACC_HARDEN_SIGN = 4, 5, 6, 7
+
+
+class mutation:
+    """Context manager: the oracle runs with planted mistake `m` inside the block."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        load().orc_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        load().orc_set_mutation(MUT_NONE)
+        return False
 
 
 def cell_key(cx: int, cy: int, grid_res: int) -> int:
