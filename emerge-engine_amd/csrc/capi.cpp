@@ -395,10 +395,6 @@ struct emh_sim {
     uint32_t cap = 0;  // particle capacity of the device buffers
     DevMat* d_mats = nullptr;
     void* grid_block = nullptr;
-    uint32_t* d_cell_count = nullptr;
-    uint32_t* d_cell_start = nullptr;
-    uint32_t* d_tile_count = nullptr;
-    uint32_t* d_tile_start = nullptr;
     Ctrl* d_ctrl = nullptr;
     DevSnapshot* d_snap = nullptr;     // diagnostics scratch, allocated on first use
     uint32_t* d_snap_cells = nullptr;  // 3 R^2 per-cell material ranges and counts
@@ -453,10 +449,6 @@ struct emh_sim {
         if (d_heights) (void)hipFree(d_heights);
         if (d_cold) (void)hipFree(d_cold);
         if (grid_block) (void)hipFree(grid_block);
-        if (d_cell_count) (void)hipFree(d_cell_count);
-        if (d_cell_start) (void)hipFree(d_cell_start);
-        if (d_tile_count) (void)hipFree(d_tile_count);
-        if (d_tile_start) (void)hipFree(d_tile_start);
         if (d_ctrl) (void)hipFree(d_ctrl);
         if (d_snap) (void)hipFree(d_snap);
         if (d_snap_cells) (void)hipFree(d_snap_cells);
@@ -470,7 +462,6 @@ struct emh_sim {
         cold_cap = 0;
         cold_valid = false;
         grid_block = nullptr;
-        d_cell_count = d_cell_start = d_tile_count = d_tile_start = nullptr;
         d_ctrl = nullptr;
         stream = nullptr;
     }
@@ -492,18 +483,9 @@ struct emh_sim {
             for (int ch = 0; ch < kAccChannels; ++ch) pipe.grid.acc[set][ch] = g + (size_t)(set * kAccChannels + ch) * R2;
             pipe.grid.dm[set] = g + (size_t)(3 * kAccChannels + set) * R2;
         }
-        d_cell_count = dalloc<uint32_t>(n_bins());
-        d_cell_start = dalloc<uint32_t>(n_bins() + 1);
-        d_tile_count = dalloc<uint32_t>(n_tiles());
-        d_tile_start = dalloc<uint32_t>(n_tiles() + 1);
         d_ctrl = dalloc<Ctrl>(1);
         d_mats = dalloc<DevMat>(EMH_MAX_MATERIALS);
-        if (!d_cell_count || !d_cell_start || !d_tile_count || !d_tile_start || !d_ctrl || !d_mats)
-            return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
-        pipe.sort.cell_count = d_cell_count;
-        pipe.sort.cell_start = d_cell_start;
-        pipe.sort.tile_count = d_tile_count;
-        pipe.sort.tile_start = d_tile_start;
+        if (!d_ctrl || !d_mats) return fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
         pipe.ctrl = d_ctrl;
         pipe.mats = d_mats;
         return EMH_OK;
@@ -537,8 +519,16 @@ struct emh_sim {
             pb.mat = (uint32_t*)take(sizeof(uint32_t) * c);
             pb.id = (uint32_t*)take(sizeof(uint32_t) * c);
         }
+        // the reorder's radix sort: pass 1 writes its keys over the input-order
+        // keys of pass 0, which no later pass reads
         pipe.sort.key = (uint32_t*)take(sizeof(uint32_t) * c);
-        pipe.sort.rank = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.kbuf[0] = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.kbuf[1] = pipe.sort.key;
+        pipe.sort.vbuf[0] = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.vbuf[1] = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.out = (uint32_t*)take(sizeof(uint32_t) * c);
+        pipe.sort.hist = (uint32_t*)take(sizeof(uint32_t) * sort_hist_len(c, n_bins()));
+        pipe.sort.perm = 0u;
         for (void* p : particle_allocs)
             if (!p) {
                 release_particles();
@@ -801,6 +791,19 @@ struct emh_sim {
         HIP_TRY(launch_impulse(pipe.buf[st.cur_buf & 1u], pipe.n, cx, cy, radius, fx, fy, strength, radial, vel_limit,
                                stream));
         return restart_loop(st.cur_buf);
+    }
+
+    // The caller index of the particle in each device slot (the stored order).
+    int slot_order(uint32_t* out) {
+        if (host_dirty) return fail(EMH_ERR_STATE, "the particles are not on the device yet: step first");
+        if (pipe.n == 0) return EMH_OK;
+        CtrlState st;
+        int rc = read_ctrl(st);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(out, pipe.buf[st.cur_buf & 1u].id, sizeof(uint32_t) * pipe.n, hipMemcpyDeviceToHost,
+                               stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return EMH_OK;
     }
 
     // Particle records in caller order into a device buffer of `cap` records.
@@ -1610,6 +1613,12 @@ int emh_sim_apply_radial_impulse(emh_sim* s, float cx, float cy, float radius, f
     return s->impulse(cx, cy, radius, 0.0f, 0.0f, strength, 1u);
 }
 
+int emh_sim_get_slot_order(emh_sim* s, uint32_t* ids_out, size_t n) {
+    if (!s || (!ids_out && n)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (n != s->host.size()) return fail(EMH_ERR_INVALID_ARG, "expected %zu particles, got %zu", s->host.size(), n);
+    return s->slot_order(ids_out);
+}
+
 int emh_sim_export_particles_device(emh_sim* s, void* dst, size_t capacity) {
     if (!s || !dst) return fail(EMH_ERR_INVALID_ARG, "null argument");
     return s->export_particles(dst, capacity);
@@ -1795,41 +1804,38 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out
     if (R < 1 || R > 16384) return fail(EMH_ERR_INVALID_ARG, "grid_res out of range");
     if (n > 0x7fffffffu) return fail(EMH_ERR_INVALID_ARG, "too many keys");
     if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
-    const uint32_t tpd = (R + 15u) / 16u, n_tiles = tpd * tpd, n_bins = n_tiles * 256u;
+    const uint32_t tpd = (R + 15u) / 16u, n_bins = tpd * tpd * 256u;
     const uint32_t nn = (uint32_t)n;
+    if (nn == 0) return EMH_OK;
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     float2* d_xy = dalloc<float2>(nn);
     uint32_t* d_keys = dalloc<uint32_t>(nn);
     uint32_t* d_perm = dalloc<uint32_t>(nn);
-    uint32_t* d_rank = dalloc<uint32_t>(nn);
-    uint32_t* d_pu = dalloc<uint32_t>(nn);
-    uint32_t* d_cc = dalloc<uint32_t>(n_bins);
-    uint32_t* d_cs = dalloc<uint32_t>(n_bins + 1);
-    uint32_t* d_tc = dalloc<uint32_t>(n_tiles);
-    uint32_t* d_ts = dalloc<uint32_t>(n_tiles + 1);
+    uint32_t* d_tmp = dalloc<uint32_t>(4 * (size_t)nn);
+    uint32_t* d_hist = dalloc<uint32_t>(sort_hist_len(nn, n_bins));
     int rc = EMH_OK;
-    if (!d_xy || !d_keys || !d_perm || !d_rank || !d_pu || !d_cc || !d_cs || !d_tc || !d_ts) {
+    if (!d_xy || !d_keys || !d_perm || !d_tmp || !d_hist) {
         rc = fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed");
     } else {
-        hipError_t e = hipSuccess;
-        if (nn) e = hipMemcpyAsync(d_xy, xy, sizeof(float2) * nn, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = sort_cell_keys_device(d_xy, nn, R, d_keys, d_perm, d_rank, d_cc, d_cs, d_tc, d_ts, d_pu, st);
-        if (e == hipSuccess && nn) e = hipMemcpyAsync(keys_out, d_keys, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess && nn) e = hipMemcpyAsync(perm_out, d_perm, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
+        SortBuf sb{};
+        for (int k = 0; k < 2; ++k) {
+            sb.kbuf[k] = d_tmp + (size_t)(2 * k) * nn;
+            sb.vbuf[k] = d_tmp + (size_t)(2 * k + 1) * nn;
+        }
+        sb.hist = d_hist;
+        hipError_t e = hipMemcpyAsync(d_xy, xy, sizeof(float2) * nn, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = sort_cell_keys_device(d_xy, nn, R, d_keys, d_perm, sb, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(keys_out, d_keys, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(perm_out, d_perm, sizeof(uint32_t) * nn, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = fail(EMH_ERR_DEVICE, "sort: %s", hipGetErrorString(e));
     }
     (void)hipFree(d_xy);
     (void)hipFree(d_keys);
     (void)hipFree(d_perm);
-    (void)hipFree(d_rank);
-    (void)hipFree(d_pu);
-    (void)hipFree(d_cc);
-    (void)hipFree(d_cs);
-    (void)hipFree(d_tc);
-    (void)hipFree(d_ts);
+    (void)hipFree(d_tmp);
+    (void)hipFree(d_hist);
     (void)hipStreamDestroy(st);
     return rc;
 }
@@ -2407,7 +2413,7 @@ struct emh_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     Pipeline* d_pipes = nullptr;
-    uint32_t* d_starts = nullptr;  // five work lists of n + 1 entries
+    uint32_t* d_starts = nullptr;  // four work lists of n + 1 entries
     Ctrl* d_ctrl = nullptr;        // gathered control blocks
     std::vector<Ctrl> h_ctrl;
     BatchLists L{};
@@ -2429,14 +2435,14 @@ struct emh_batch {
     int sync() {
         const uint32_t n = (uint32_t)sims.size();
         std::vector<Pipeline> pipes(n);
-        std::vector<uint32_t> st(5 * (n + 1), 0u);
+        std::vector<uint32_t> st(4 * (n + 1), 0u);
         uint32_t* one = st.data();
-        uint32_t* keys = one + (n + 1);
-        uint32_t* cells = keys + (n + 1);
-        uint32_t* fused = cells + (n + 1);
+        uint32_t* sorts = one + (n + 1);
+        uint32_t* fused = sorts + (n + 1);
         uint32_t* dens = fused + (n + 1);
         max_substeps = 1;
         predicted = 1;
+        uint32_t passes = 0;
         for (uint32_t i = 0; i < n; ++i) {
             emh_sim* s = sims[i];
             int rc;
@@ -2455,8 +2461,8 @@ struct emh_batch {
             const uint32_t chunks = (s->pipe.n + kChunk - 1) / kChunk;
             const bool d = s->pipe.cfg.density_pass != 0;
             one[i + 1] = one[i] + 1;
-            keys[i + 1] = keys[i] + (s->pipe.n + kBlock - 1) / kBlock;
-            cells[i + 1] = cells[i] + s->n_tiles();
+            sorts[i + 1] = sorts[i] + sort_tiles(s->pipe.n);
+            passes = std::max<uint32_t>(passes, sort_plan(s->n_bins()).passes);
             fused[i + 1] = fused[i] + (d ? 0u : chunks);
             dens[i + 1] = dens[i] + (d ? chunks : 0u);
             max_substeps = std::max<uint32_t>(max_substeps, s->cfg.max_substeps_per_step);
@@ -2467,12 +2473,11 @@ struct emh_batch {
         HIP_TRY(hipStreamSynchronize(stream));
         auto view = [&](int k) { return BatchView{d_pipes, d_starts + (size_t)k * (n + 1), n}; };
         L.one = view(0);
-        L.keys = view(1);
-        L.cells = view(2);
-        L.fused = view(3);
-        L.density = view(4);
-        L.keys_blocks = keys[n];
-        L.cells_blocks = cells[n];
+        L.sort = view(1);
+        L.fused = view(2);
+        L.density = view(3);
+        L.sort_blocks = sorts[n];
+        L.sort_passes = passes;
         L.fused_blocks = fused[n];
         L.density_blocks = dens[n];
         seq = 1;
@@ -2571,7 +2576,7 @@ int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out) {
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(EMH_ERR_DEVICE, "stream creation failed"));
     b->d_pipes = dalloc<Pipeline>(n);
-    b->d_starts = dalloc<uint32_t>(5 * ((size_t)n + 1));
+    b->d_starts = dalloc<uint32_t>(4 * ((size_t)n + 1));
     b->d_ctrl = dalloc<Ctrl>(n);
     if (!b->d_pipes || !b->d_starts || !b->d_ctrl) return bail(fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed"));
     b->h_ctrl.resize(n);

@@ -2,11 +2,9 @@
 //
 // Simulation::step (solver/step.rs:23-60) runs on the device as:
 //   k_begin_step     1 lane: queues the start of a step (step.rs:24-33).
-//  [k_keys]          on reorder steps: cell key and arrival rank of every
-//                    particle, per-cell and per-tile counts.
-//  [k_tile_scan]     on reorder steps: start slot of every 16x16 tile.
-//  [k_cell_scan]     on reorder steps: per tile, the start slot of each
-//                    cell-key bin.
+//  [k_sort_*]        on reorder steps: a stable radix sort of the particles'
+//                    cell keys (count, scan and scatter per pass) giving
+//                    each slot's position in cell-key order.
 // then, per substep:
 //  [k_density_p2g]   only when a material re-estimates density
 //                    (solver/density.rs:216-247): gathers the density of the
@@ -140,45 +138,6 @@ __device__ void block_maxmin(float& a, float& d, float* s_red) {
         d = fminf(d, s_red[2 * w + 1]);
     }
     __syncthreads();
-}
-
-// Arrival rank of `key` among the lanes and waves that add to counts[key];
-// same-key lanes of a wave share one atomic.  Every lane of the wave must call.
-__device__ __forceinline__ uint32_t agg_rank(uint32_t* counts, uint32_t key, bool valid) {
-    const uint32_t lane = lane_id();
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    uint32_t rank = 0;
-    bool pending = valid;
-    while (true) {
-        uint64_t pm = __ballot(pending);
-        if (pm == 0) break;
-        int leader = __ffsll((unsigned long long)pm) - 1;
-        uint32_t lk = __shfl(key, leader, 64);
-        bool mine = pending && key == lk;
-        uint64_t same = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == (uint32_t)leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if (mine) {
-            rank = base + (uint32_t)__popcll(same & lt);
-            pending = false;
-        }
-    }
-    return rank;
-}
-__device__ void agg_count(uint32_t* counts, uint32_t key, bool valid) {
-    const uint32_t lane = lane_id();
-    bool pending = valid;
-    while (true) {
-        uint64_t pm = __ballot(pending);
-        if (pm == 0) break;
-        int leader = __ffsll((unsigned long long)pm) - 1;
-        uint32_t lk = __shfl(key, leader, 64);
-        bool mine = pending && key == lk;
-        uint64_t same = __ballot(mine);
-        if (lane == (uint32_t)leader) atomicAdd(&counts[lk], (uint32_t)__popcll(same));
-        if (mine) pending = false;
-    }
 }
 
 // ------------------------------------------------------------ particle I/O
@@ -694,8 +653,6 @@ __device__ void reduce_cfl(CtrlRed* red, float speed, float mdt, float* s_red) {
 __global__ void k_zero_all(const Pipeline p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
-    if (i < p.cfg.n_bins) p.sort.cell_count[i] = 0;
-    if (i < p.cfg.tiles_per_dim * p.cfg.tiles_per_dim) p.sort.tile_count[i] = 0;
     if (i < R2) {
         for (int s = 0; s < 3; ++s) {
             for (int ch = 0; ch < kAccChannels; ++ch) p.grid.acc[s][ch][i] = 0.0f;
@@ -721,64 +678,206 @@ __device__ __forceinline__ void begin_step_body(const Pipeline& p, uint64_t seq,
 }
 __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) { begin_step_body(p, seq, reorder); }
 
-// Cell key of the stored (projected) position of every particle, its arrival
-// rank inside the key's bin, and the per-tile counts.  `seq` is the control
-// launch that will read the keys.
-__device__ __forceinline__ void keys_body(const Pipeline& p, uint64_t seq, uint32_t blk) {
-    const Ctrl* c = p.ctrl;
-    const DevConfig& cfg = p.cfg;
-    const PBuf a = sel_buf(p, c->st[seq & 1u].cur_buf);
-    const bool project = cfg.project && c->red[seq_mod3(seq + 2u)].dirty;
-    const uint32_t j = blk * blockDim.x + threadIdx.x;
-    const bool valid = j < p.n;
-    uint32_t key = 0;
-    if (valid) {
-        float2 x = a.x[j];
-        if (project) project_x(x.x, x.y, cfg);
-        key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), cfg.grid_res, cfg.tiles_per_dim);
+// ------------------------------------------------- stable cell-key sort
+// LSD radix sort of the cell keys (SortPlan): every pass is stable, so the
+// result is the order std::stable_sort gives -- by key, then by slot.
+//   count   per (digit, workgroup) counts of one pass (pass 0 also computes
+//           the keys, from the stored and, when flagged, projected positions);
+//   scan    exclusive scan of the counts in (digit, workgroup) order: the
+//           first output position of each digit of each workgroup;
+//   scatter each key's position = that start + its rank among the keys of
+//           the workgroup with the same digit and a lower slot.
+// A workgroup's kSortTile keys are 64 x kSortItems rows; wave w takes the
+// kSortItems consecutive rows from w * kSortItems, so wave order, row order
+// and lane order together are slot order.  A key's rank among the equal
+// digits of its row comes from `bits` ballots (the lanes agreeing on every
+// bit), its rank among the wave's earlier rows from a wave-private LDS count
+// that the row's lowest such lane advances: no atomics, no arrival order.
+struct SortJob {
+    const float2* x;        // positions whose cell keys pass 0 computes
+    const DevConfig* proj;  // project those positions first (nullptr: no projection)
+    uint32_t n, grid_res, tpd, n_bins;
+    SortBuf s;
+};
+
+constexpr uint32_t kSortBuckets = 1u << kSortMaxDigit;
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t lane = lane_id();
+    return lane == 0u ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Digit width and shift of pass `pass` of a key of plan `pl`.
+__device__ __forceinline__ void pass_digit(const SortPlan& pl, uint32_t pass, uint32_t& shift, uint32_t& bits) {
+    shift = pass * pl.digit;
+    bits = min(pl.digit, pl.key_bits - shift);
+}
+
+// Lanes of the wave with this lane's digit (`bits` low bits), among the valid ones.
+__device__ __forceinline__ uint64_t digit_peers(uint32_t digit, uint32_t bits, bool valid) {
+    uint64_t peers = __ballot(valid);
+    for (uint32_t b = 0; b < bits; ++b) {
+        const bool bit = ((digit >> b) & 1u) != 0u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
     }
-    const uint32_t rank = agg_rank(p.sort.cell_count, key, valid);
-    agg_count(p.sort.tile_count, key >> 8, valid);
-    if (valid) {
-        p.sort.key[j] = key;
-        p.sort.rank[j] = rank;
+    return peers;
+}
+
+// Key of slot i as pass `pass` reads it.
+__device__ __forceinline__ uint32_t sort_key_in(const SortJob& J, uint32_t pass, uint32_t i) {
+    if (pass > 0) return J.s.kbuf[(pass - 1u) & 1u][i];
+    float2 x = J.x[i];
+    if (J.proj) project_x(x.x, x.y, *J.proj);
+    return cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.tpd);
+}
+
+// Ranks of workgroup blk's keys on the digit of `pass`.  On return s_cnt[w][d]
+// holds the number of digit-d keys in waves before w plus `add(d)` (the
+// scatter's start slot, 0 for the count), loc[k] the key's rank among the
+// equal digits before it in its wave, key[k] the key (0 beyond n); the
+// return value of each thread's buckets d = tid, tid + 256, ... is their
+// total in the workgroup, written by `total(d, t)`.
+template <class Add, class Total>
+__device__ __forceinline__ void sort_rank_tile(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t shift,
+                                               uint32_t bits, uint32_t (*s_cnt)[kSortBuckets],
+                                               uint32_t (&key)[kSortItems], uint32_t (&loc)[kSortItems], Add add,
+                                               Total total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t buckets = 1u << bits, mask = buckets - 1u;
+    for (uint32_t d = lane; d < buckets; d += 64u) s_cnt[w][d] = 0u;
+    __syncthreads();
+    const uint64_t below = lanes_below();
+    const uint32_t row0 = blk * kSortTile + w * (uint32_t)kSortItems * 64u;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = row0 + (uint32_t)k * 64u + lane;
+        const bool valid = i < J.n;
+        key[k] = valid ? sort_key_in(J, pass, i) : 0u;
+        if (pass == 0 && valid) J.s.key[i] = key[k];
+        const uint32_t d = (key[k] >> shift) & mask;
+        const uint64_t peers = digit_peers(d, bits, valid);
+        const uint32_t before = (uint32_t)__popcll(peers & below);
+        uint32_t b = 0u;
+        if (valid) b = s_cnt[w][d];
+        // the read above is issued before the write below; a wave's LDS
+        // operations complete in order, so every lane reads the old count
+        if (valid && before == 0u) s_cnt[w][d] = b + (uint32_t)__popcll(peers);
+        loc[k] = b + before;
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < buckets; d += blockDim.x) {
+        const uint32_t a = add(d);
+        uint32_t run = a;
+#pragma unroll
+        for (int v = 0; v < kWaves; ++v) {
+            const uint32_t c = s_cnt[v][d];
+            s_cnt[v][d] = run;
+            run += c;
+        }
+        total(d, run - a);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void sort_count_body(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t nblk) {
+    __shared__ uint32_t s_cnt[kWaves][kSortBuckets];
+    const SortPlan pl = sort_plan(J.n_bins);
+    if (pass >= pl.passes) return;
+    uint32_t shift, bits;
+    pass_digit(pl, pass, shift, bits);
+    uint32_t key[kSortItems], loc[kSortItems];
+    sort_rank_tile(J, pass, blk, shift, bits, s_cnt, key, loc, [](uint32_t) { return 0u; },
+                   [&](uint32_t d, uint32_t t) { J.s.hist[(size_t)d * nblk + blk] = t; });
+}
+
+// Exclusive scan of one pass's counts in place (one workgroup of 1024).
+__device__ __forceinline__ void sort_scan_body(const SortJob& J, uint32_t pass) {
+    __shared__ uint32_t s_scan[17];
+    const SortPlan pl = sort_plan(J.n_bins);
+    if (pass >= pl.passes) return;
+    uint32_t shift, bits;
+    pass_digit(pl, pass, shift, bits);
+    const uint32_t len = (1u << bits) * sort_tiles(J.n);
+    constexpr uint32_t kPerThread = 16;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < len; base += blockDim.x * kPerThread) {
+        const uint32_t i0 = base + threadIdx.x * kPerThread;
+        uint32_t v[kPerThread], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            v[j] = i0 + j < len ? J.s.hist[i0 + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_excl_scan(sum, s_scan, total);
+#pragma unroll
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            if (i0 + j < len) J.s.hist[i0 + j] = run;
+            run += v[j];
+        }
+        carry += total;
     }
 }
-__global__ __launch_bounds__(kBlock) void k_keys(const Pipeline p, uint64_t seq) { keys_body(p, seq, blockIdx.x); }
 
-// Exclusive scan of the tile counts (one workgroup); clears the counts.
-__device__ __forceinline__ void tile_scan_body(const Pipeline& p) {
-    __shared__ uint32_t s_scan[17];
-    const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
-    const uint32_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
-    const uint32_t beg = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (beg + k < n_tiles) sum += p.sort.tile_count[beg + k];
-    uint32_t total;
-    uint32_t run = block_excl_scan(sum, s_scan, total);
-    for (uint32_t k = 0; k < per; ++k) {
-        if (beg + k < n_tiles) {
-            const uint32_t cnt = p.sort.tile_count[beg + k];
-            p.sort.tile_start[beg + k] = run;
-            p.sort.tile_count[beg + k] = 0;
-            run += cnt;
+__device__ __forceinline__ void sort_scatter_body(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t nblk) {
+    __shared__ uint32_t s_cnt[kWaves][kSortBuckets];
+    const SortPlan pl = sort_plan(J.n_bins);
+    if (pass >= pl.passes) return;
+    uint32_t shift, bits;
+    pass_digit(pl, pass, shift, bits);
+    const bool last = pass + 1u == pl.passes;
+    uint32_t key[kSortItems], loc[kSortItems];
+    const uint32_t* hist = J.s.hist;
+    sort_rank_tile(J, pass, blk, shift, bits, s_cnt, key, loc,
+                   [&](uint32_t d) { return hist[(size_t)d * nblk + blk]; }, [](uint32_t, uint32_t) {});
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6, mask = (1u << bits) - 1u;
+    const uint32_t row0 = blk * kSortTile + w * (uint32_t)kSortItems * 64u;
+    const uint32_t* vin = pass > 0 ? J.s.vbuf[(pass - 1u) & 1u] : nullptr;
+    uint32_t* kout = J.s.kbuf[pass & 1u];
+    uint32_t* vout = J.s.vbuf[pass & 1u];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const uint32_t i = row0 + (uint32_t)k * 64u + lane;
+        if (i >= J.n) continue;
+        const uint32_t pos = s_cnt[w][(key[k] >> shift) & mask] + loc[k];
+        const uint32_t val = vin ? vin[i] : i;
+        if (!last) {
+            kout[pos] = key[k];
+            vout[pos] = val;
+        } else if (J.s.perm) {
+            J.s.out[pos] = val;
+        } else {
+            J.s.out[val] = pos;
         }
     }
 }
-__global__ __launch_bounds__(1024) void k_tile_scan(const Pipeline p) { tile_scan_body(p); }
 
-// Start slot of every cell-key bin inside each 16x16 tile; clears the counts.
-__device__ __forceinline__ void cell_scan_body(const Pipeline& p, uint32_t tile) {
-    __shared__ uint32_t s_scan[17];
-    const uint32_t idx = tile * kTileCells + threadIdx.x;
-    const uint32_t cnt = p.sort.cell_count[idx];
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(cnt, s_scan, total);
-    p.sort.cell_start[idx] = p.sort.tile_start[tile] + ex;
-    p.sort.cell_count[idx] = 0;
+// The pipeline's reorder: keys of the stored state of control launch `seq`
+// (the launch that will move the particles), projected when flagged;
+// s.out = each slot's position in cell-key order.
+__device__ __forceinline__ SortJob pipeline_sort_job(const Pipeline& p, uint64_t seq) {
+    const Ctrl* c = p.ctrl;
+    const bool project = p.cfg.project && c->red[seq_mod3(seq + 2u)].dirty;
+    SortJob J;
+    J.x = sel_buf(p, c->st[seq & 1u].cur_buf).x;
+    J.proj = project ? &p.cfg : nullptr;
+    J.n = p.n;
+    J.grid_res = p.cfg.grid_res;
+    J.tpd = p.cfg.tiles_per_dim;
+    J.n_bins = p.cfg.n_bins;
+    J.s = p.sort;
+    return J;
 }
-__global__ __launch_bounds__(kTileCells) void k_cell_scan(const Pipeline p) { cell_scan_body(p, blockIdx.x); }
+__global__ __launch_bounds__(kBlock) void k_sort_count(const Pipeline p, uint64_t seq, uint32_t pass) {
+    sort_count_body(pipeline_sort_job(p, seq), pass, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(1024) void k_sort_scan(const Pipeline p, uint64_t seq, uint32_t pass) {
+    sort_scan_body(pipeline_sort_job(p, seq), pass);
+}
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const Pipeline p, uint64_t seq, uint32_t pass) {
+    sort_scatter_body(pipeline_sort_job(p, seq), pass, blockIdx.x, gridDim.x);
+}
 
 constexpr int kModeG2P = 1;   // gather + update the stored state (a substep)
 constexpr int kModeP2G = 2;   // scatter mass, momentum and stress moment
@@ -1273,7 +1372,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                     any_dirty |= needs_projection(s, sp, cfg);
                 }
                 if (G2P && (!SLEEP || !asleep || mat_changed || reorder)) {
-                    const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
+                    const uint32_t dest = reorder ? p.sort.out[j] : j;
                     store_fields(o, dest, s, mask, full);
                     if (reorder) o.id[dest] = a.id[j];
                     if (SLEEP && mat_changed && !full) o.mat[dest] = s.mat;
@@ -1286,7 +1385,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
             if (valid) {
                 cfl_of<ONLY>(p, s, speed, mdt);
                 any_dirty |= needs_projection(s, sp, cfg);
-                const uint32_t dest = reorder ? p.sort.cell_start[p.sort.key[j]] + p.sort.rank[j] : j;
+                const uint32_t dest = reorder ? p.sort.out[j] : j;
                 store_fields(o, dest, s, mask, full);
                 if (reorder) o.id[dest] = a.id[j];
             }
@@ -1456,20 +1555,20 @@ __global__ void kb_begin_step(BatchView b, uint64_t seq, uint32_t reorder) {
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
     if (threadIdx.x == 0) begin_step_body(b.pipes[s], seq, reorder);
 }
-__global__ __launch_bounds__(kBlock) void kb_keys(BatchView b, uint64_t seq) {
+__global__ __launch_bounds__(kBlock) void kb_sort_count(BatchView b, uint64_t seq, uint32_t pass) {
     uint32_t blk, nblk;
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
-    keys_body(b.pipes[s], seq, blk);
+    sort_count_body(pipeline_sort_job(b.pipes[s], seq), pass, blk, nblk);
 }
-__global__ __launch_bounds__(1024) void kb_tile_scan(BatchView b) {
+__global__ __launch_bounds__(1024) void kb_sort_scan(BatchView b, uint64_t seq, uint32_t pass) {
     uint32_t blk, nblk;
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
-    tile_scan_body(b.pipes[s]);
+    sort_scan_body(pipeline_sort_job(b.pipes[s], seq), pass);
 }
-__global__ __launch_bounds__(kTileCells) void kb_cell_scan(BatchView b) {
+__global__ __launch_bounds__(kBlock) void kb_sort_scatter(BatchView b, uint64_t seq, uint32_t pass) {
     uint32_t blk, nblk;
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
-    cell_scan_body(b.pipes[s], blk);
+    sort_scatter_body(pipeline_sort_job(b.pipes[s], seq), pass, blk, nblk);
 }
 template <int MODE>
 __global__ __launch_bounds__(kBlock, 2) void kb_substep(BatchView b, uint64_t seq) {
@@ -1647,65 +1746,12 @@ __global__ __launch_bounds__(kBlock) void k_snap_grid(const Pipeline p, uint32_t
 }
 
 // ------------------------------------------------- stand-alone cell sort
-__global__ void k_sort_keys(const float2* xy, uint32_t n, uint32_t R, uint32_t tpd, uint32_t* keys, uint32_t* rank,
-                            uint32_t* cell_count, uint32_t* tile_count) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < n;
-    uint32_t key = 0;
-    if (valid) {
-        float2 x = xy[i];
-        key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), R, tpd);
-    }
-    uint32_t r = agg_rank(cell_count, key, valid);
-    agg_count(tile_count, key >> 8, valid);
-    if (valid) {
-        keys[i] = key;
-        rank[i] = r;
-    }
+__global__ __launch_bounds__(kBlock) void k_sortx_count(const SortJob J, uint32_t pass) {
+    sort_count_body(J, pass, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(1024) void k_sort_tile_scan(uint32_t n_tiles, const uint32_t* tile_count,
-                                                         uint32_t* tile_start) {
-    __shared__ uint32_t s_scan[17];
-    const uint32_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
-    const uint32_t beg = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (beg + k < n_tiles) sum += tile_count[beg + k];
-    uint32_t total;
-    uint32_t run = block_excl_scan(sum, s_scan, total);
-    for (uint32_t k = 0; k < per; ++k)
-        if (beg + k < n_tiles) {
-            tile_start[beg + k] = run;
-            run += tile_count[beg + k];
-        }
-    if (threadIdx.x == 0) tile_start[n_tiles] = total;
-}
-__global__ __launch_bounds__(kTileCells) void k_sort_cell_scan(uint32_t n_bins, const uint32_t* cell_count,
-                                                               const uint32_t* tile_start, uint32_t* cell_start) {
-    __shared__ uint32_t s_scan[17];
-    const uint32_t tile = blockIdx.x;
-    const uint32_t idx = tile * kTileCells + threadIdx.x;
-    uint32_t total;
-    uint32_t ex = block_excl_scan(cell_count[idx], s_scan, total);
-    cell_start[idx] = tile_start[tile] + ex;
-    if (tile == gridDim.x - 1 && threadIdx.x == kTileCells - 1) cell_start[n_bins] = tile_start[gridDim.x];
-}
-__global__ void k_sort_scatter(uint32_t n, const uint32_t* keys, const uint32_t* rank, const uint32_t* cell_start,
-                               uint32_t* perm_unordered) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) perm_unordered[cell_start[keys[i]] + rank[i]] = i;
-}
-// Stable order inside a bin: rank of each element among its bin by input index.
-__global__ void k_sort_finalize(uint32_t n, const uint32_t* keys, const uint32_t* cell_start,
-                                const uint32_t* perm_unordered, uint32_t* perm_sorted) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    uint32_t e = perm_unordered[t];
-    uint32_t key = keys[e];
-    uint32_t start = cell_start[key], cnt = cell_start[key + 1] - start;
-    uint32_t r = 0;
-    for (uint32_t u = 0; u < cnt; ++u) r += (perm_unordered[start + u] < e) ? 1u : 0u;
-    perm_sorted[start + r] = e;
+__global__ __launch_bounds__(1024) void k_sortx_scan(const SortJob J, uint32_t pass) { sort_scan_body(J, pass); }
+__global__ __launch_bounds__(kBlock) void k_sortx_scatter(const SortJob J, uint32_t pass) {
+    sort_scatter_body(J, pass, blockIdx.x, gridDim.x);
 }
 
 inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -1716,8 +1762,7 @@ inline uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 hipError_t launch_zero_all(const Pipeline& p, hipStream_t stream) {
     const uint32_t R2 = p.cfg.grid_res * p.cfg.grid_res;
-    const uint32_t n = R2 > p.cfg.n_bins ? R2 : p.cfg.n_bins;
-    k_zero_all<<<div_up(n, kBlock), kBlock, 0, stream>>>(p);
+    k_zero_all<<<div_up(R2, kBlock), kBlock, 0, stream>>>(p);
     return hipGetLastError();
 }
 
@@ -1739,10 +1784,13 @@ hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream, 
 
 hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reorder, hipStream_t stream) {
     kb_begin_step<<<L.one.n, 64, 0, stream>>>(L.one, seq, reorder ? 1u : 0u);
-    if (reorder) {
-        if (L.keys_blocks) kb_keys<<<L.keys_blocks, kBlock, 0, stream>>>(L.keys, seq + 1u);
-        kb_tile_scan<<<L.one.n, 1024, 0, stream>>>(L.one);
-        kb_cell_scan<<<L.cells_blocks, kTileCells, 0, stream>>>(L.cells);
+    if (reorder && L.sort_blocks) {
+        // the keys are those of the state launch seq + 1 will move
+        for (uint32_t pass = 0; pass < L.sort_passes; ++pass) {
+            kb_sort_count<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
+            kb_sort_scan<<<L.one.n, 1024, 0, stream>>>(L.one, seq + 1u, pass);
+            kb_sort_scatter<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
+        }
     }
     return hipGetLastError();
 }
@@ -1765,10 +1813,13 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
     reorder = reorder && p.n > 0;
     k_begin_step<<<1, 1, 0, stream>>>(p, seq, reorder ? 1u : 0u);
     if (reorder) {
-        const uint32_t n_tiles = p.cfg.tiles_per_dim * p.cfg.tiles_per_dim;
-        k_keys<<<div_up(p.n, kBlock), kBlock, 0, stream>>>(p, seq + 1u);
-        k_tile_scan<<<1, 1024, 0, stream>>>(p);
-        k_cell_scan<<<n_tiles, kTileCells, 0, stream>>>(p);
+        // the keys are those of the state launch seq + 1 will move
+        const uint32_t nb = sort_tiles(p.n), passes = sort_plan(p.cfg.n_bins).passes;
+        for (uint32_t pass = 0; pass < passes; ++pass) {
+            k_sort_count<<<nb, kBlock, 0, stream>>>(p, seq + 1u, pass);
+            k_sort_scan<<<1, 1024, 0, stream>>>(p, seq + 1u, pass);
+            k_sort_scatter<<<nb, kBlock, 0, stream>>>(p, seq + 1u, pass);
+        }
     }
     return hipGetLastError();
 }
@@ -1830,23 +1881,26 @@ hipError_t launch_snapshot(const Pipeline& p, uint32_t cur_buf, uint32_t grid_se
     return hipGetLastError();
 }
 
-hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys,
-                                 uint32_t* perm_sorted, uint32_t* scratch_rank, uint32_t* cell_count,
-                                 uint32_t* cell_start, uint32_t* tile_count, uint32_t* tile_start,
-                                 uint32_t* perm_unordered, hipStream_t stream) {
-    const uint32_t tpd = (grid_res + 15u) / 16u;
-    const uint32_t n_tiles = tpd * tpd, n_bins = n_tiles * kTileCells;
-    hipError_t e = hipMemsetAsync(cell_count, 0, sizeof(uint32_t) * n_bins, stream);
-    if (e != hipSuccess) return e;
-    e = hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream);
-    if (e != hipSuccess) return e;
+hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys_out,
+                                 uint32_t* perm_out, const SortBuf& s, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    k_sort_keys<<<div_up(n, kBlock), kBlock, 0, stream>>>(xy, n, grid_res, tpd, keys, scratch_rank, cell_count,
-                                                          tile_count);
-    k_sort_tile_scan<<<1, 1024, 0, stream>>>(n_tiles, tile_count, tile_start);
-    k_sort_cell_scan<<<n_tiles, kTileCells, 0, stream>>>(n_bins, cell_count, tile_start, cell_start);
-    k_sort_scatter<<<div_up(n, kBlock), kBlock, 0, stream>>>(n, keys, scratch_rank, cell_start, perm_unordered);
-    k_sort_finalize<<<div_up(n, kBlock), kBlock, 0, stream>>>(n, keys, cell_start, perm_unordered, perm_sorted);
+    SortJob J;
+    J.x = xy;
+    J.proj = nullptr;
+    J.n = n;
+    J.grid_res = grid_res;
+    J.tpd = (grid_res + 15u) / 16u;
+    J.n_bins = J.tpd * J.tpd * (uint32_t)kTileCells;
+    J.s = s;
+    J.s.key = keys_out;
+    J.s.out = perm_out;
+    J.s.perm = 1u;
+    const uint32_t nb = sort_tiles(n), passes = sort_plan(J.n_bins).passes;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        k_sortx_count<<<nb, kBlock, 0, stream>>>(J, pass);
+        k_sortx_scan<<<1, 1024, 0, stream>>>(J, pass);
+        k_sortx_scatter<<<nb, kBlock, 0, stream>>>(J, pass);
+    }
     return hipGetLastError();
 }
 

@@ -59,13 +59,40 @@ struct GridBuf {
     float* dm[3];  // mass grid of the density re-estimate pass
 };
 
+// Stable LSD radix sort by cell key (the pipeline's reorder and the
+// stand-alone sort share it).  A pass ranks kSortTile keys per workgroup on
+// one digit of at most kSortMaxDigit bits: a per-(digit, workgroup) count, an
+// exclusive scan of those counts in (digit, workgroup) order, and a scatter
+// that ranks each key among the equal digits before it in index order.
+// Keys of k bits take ceil(k / kSortMaxDigit) passes of equal width.
+constexpr int kSortItems = 16;                                 // keys per lane
+constexpr uint32_t kSortTile = (uint32_t)kBlock * kSortItems;  // keys per workgroup
+constexpr uint32_t kSortMaxDigit = 10;                         // at most 1024 buckets
+struct SortPlan {
+    uint32_t key_bits, passes, digit;
+};
+__host__ __device__ inline SortPlan sort_plan(uint32_t n_bins) {
+    uint32_t b = 1;
+    while (b < 32u && (1u << b) < n_bins) ++b;
+    SortPlan s;
+    s.key_bits = b;
+    s.passes = (b + kSortMaxDigit - 1) / kSortMaxDigit;
+    s.digit = (b + s.passes - 1) / s.passes;
+    return s;
+}
+__host__ __device__ inline uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+// Counts of one pass: (1 << digit) buckets x sort_tiles(n) workgroups.
+__host__ __device__ inline size_t sort_hist_len(uint32_t n, uint32_t n_bins) {
+    return ((size_t)1 << sort_plan(n_bins).digit) * (size_t)sort_tiles(n);
+}
+
 struct SortBuf {
-    uint32_t* key;         // N: cell key of the particle in each slot
-    uint32_t* rank;        // N: arrival rank inside its key bin
-    uint32_t* cell_count;  // n_bins
-    uint32_t* cell_start;  // n_bins
-    uint32_t* tile_count;  // n_tiles
-    uint32_t* tile_start;  // n_tiles
+    uint32_t* key;      // N: cell key of the particle in each slot (written by pass 0)
+    uint32_t* kbuf[2];  // N each: keys in the order after pass 0, 1 (kbuf[1] may alias key)
+    uint32_t* vbuf[2];  // N each: the slot each of those keys came from
+    uint32_t* out;      // N: the result (see perm)
+    uint32_t* hist;     // sort_hist_len(N, n_bins): per-pass counts, then start slots
+    uint32_t perm;      // 1: out[sorted position] = slot; 0: out[slot] = sorted position
 };
 
 // Device-resident control: the adaptive substep loop runs on the device so
@@ -149,12 +176,12 @@ struct BatchView {
     uint32_t n;
 };
 struct BatchLists {
-    BatchView one;      // one workgroup per scene (step start, tile scan, control gather)
-    BatchView keys;     // ceil(n_s / 256) workgroups (cell keys)
-    BatchView cells;    // the scene's tiles (cell scan)
+    BatchView one;      // one workgroup per scene (step start, sort scans, control gather)
+    BatchView sort;     // sort_tiles(n_s) workgroups (cell-key sort counts and scatters)
     BatchView fused;    // chunks of the scenes without a density pass (fused substep)
     BatchView density;  // chunks of the scenes with one (density + P2G, then G2P + mass)
-    uint32_t keys_blocks, cells_blocks, fused_blocks, density_blocks;
+    uint32_t sort_blocks, fused_blocks, density_blocks;
+    uint32_t sort_passes;  // the most radix passes any scene's key needs
 };
 // One launch per stage over every scene of the batch (the single-scene
 // kernels' bodies, ONLY = mixed).
@@ -181,10 +208,12 @@ hipError_t launch_snapshot(const Pipeline& p, uint32_t cur_buf, uint32_t grid_se
 uint32_t order_key(float f);       // the key launch_snapshot uses for signed min / max
 float order_key_inverse(uint32_t k);
 
-// Stand-alone cell-key sort (bit-exact stable counting sort).
-hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys,
-                                 uint32_t* perm_sorted, uint32_t* scratch_rank, uint32_t* cell_count,
-                                 uint32_t* cell_start, uint32_t* tile_count, uint32_t* tile_start,
-                                 uint32_t* perm_unordered, hipStream_t stream);
+// Stand-alone cell-key sort (stable LSD radix sort): keys_out[i] is the key
+// of xy[i], perm_out[j] the index at sorted position j.  `s` supplies the
+// scratch: kbuf and vbuf of n entries each (none may alias keys_out or
+// perm_out) and hist of sort_hist_len(n, n_bins) entries; s.key and s.out
+// are ignored.  Clears nothing with hipMemset (capturable).
+hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res, uint32_t* keys_out,
+                                 uint32_t* perm_out, const SortBuf& s, hipStream_t stream);
 
 }  // namespace emh

@@ -247,6 +247,7 @@ def load_library() -> ctypes.CDLL:
         "emh_sim_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_sim_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "emh_sim_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_sim_get_slot_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
         "emh_sim_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
                                                 P(ctypes.c_uint32)]),
         "emh_surrogate_create": (ctypes.c_int, [ctypes.c_uint64, P(ctypes.c_void_p)]),
@@ -866,6 +867,13 @@ class Simulation:
     def set_reorder_interval(self, substeps: int) -> None:
         """Physical cell-key reorder every `substeps` substeps (and at the first step after an upload)."""
         _check(load_library().emh_sim_set_reorder_interval(self._h, int(substeps)))
+
+    def slot_order(self) -> np.ndarray:
+        """Caller index of the particle in each device slot (the stored order)."""
+        n = self.num_particles()
+        out = np.zeros(n, np.uint32)
+        _check(load_library().emh_sim_get_slot_order(self._h, out.ctypes.data, n))
+        return out
 
     def kernel_times(self) -> dict:
         """{"step_setup": (steps, ms), "k_substep": (substeps, ms)} since profiling was enabled."""

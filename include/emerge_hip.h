@@ -435,6 +435,12 @@ int emh_sim_set_profiling(emh_sim* sim, int enable);
  * reorder (default 4096; >= 1).  Results do not depend on it beyond f32
  * summation order; it trades the reorder's cost against chunk locality. */
 int emh_sim_set_reorder_interval(emh_sim* sim, uint32_t substeps);
+/* The stored order itself: ids_out[j] is the caller index of the particle in
+ * device slot j (n = the particle count).  Right after a reorder this is the
+ * stable sort of the cell keys of the positions the reorder saw -- the same
+ * permutation emh_sort_cell_keys returns for them.  EMH_ERR_STATE before the
+ * first step after an upload (the particles are not on the device yet). */
+int emh_sim_get_slot_order(emh_sim* sim, uint32_t* ids_out, size_t n);
 int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
 
 /* ---- the cell-key sort on its own ---------------------------------------

@@ -54,3 +54,62 @@ def test_sort_full_size_properties(oracle):
     okeys, operm = oracle.stable_sort_cell_keys(xy, 512)
     assert np.array_equal(keys, okeys)
     assert np.array_equal(perm, operm)
+
+
+@pytest.mark.gpu
+def test_sort_clustered_full_size(oracle):
+    """Every one of 1,048,576 keys in one cell (particles clamped into a corner), and a
+    half-and-half split: a single bin of n entries, stable by index."""
+    xy = np.full((1 << 20, 2), 2.5, np.float32)
+    keys, perm = eh.sort_cell_keys(xy, 512)
+    okeys, operm = oracle.stable_sort_cell_keys(xy, 512)
+    assert np.array_equal(keys, okeys) and np.array_equal(perm, operm)
+    assert np.array_equal(perm, np.arange(1 << 20, dtype=np.uint32))
+    xy[::2] = 509.5
+    keys, perm = eh.sort_cell_keys(xy, 512)
+    okeys, operm = oracle.stable_sort_cell_keys(xy, 512)
+    assert np.array_equal(keys, okeys) and np.array_equal(perm, operm)
+
+
+def _stable_order(keys_in_slot_order):
+    return np.argsort(keys_in_slot_order, kind="stable").astype(np.uint32)
+
+
+@pytest.mark.gpu
+def test_pipeline_reorder_permutation_is_the_stable_sort(oracle):
+    """The permutation the substep actually stores the particles in, on the 1M-particle
+    sand column: after the upload's reorder (first step) the device slot order is
+    std::stable_sort of the uploaded positions' cell keys, bit for bit, and two runs
+    give the same permutation."""
+    import bench
+    orders = []
+    for _ in range(2):
+        sim = bench.build_workload("gpu")
+        xy = sim.particles()["x"].copy()
+        sim.step_n(1)
+        orders.append(sim.slot_order())
+        sim.close()
+    _, want = oracle.stable_sort_cell_keys(xy, 512)
+    assert np.array_equal(orders[0], want)
+    assert np.array_equal(orders[1], orders[0])
+
+
+@pytest.mark.gpu
+def test_pipeline_mid_run_reorder_is_stable_over_the_stored_order(oracle):
+    """A reorder at a later step start sorts the stored slots by the cell key of their
+    current positions, ties in slot order: new order = old order composed with the
+    stable argsort of the keys read in old slot order (multi-material scene with a
+    density pass, reordering at every step)."""
+    import scenes as S
+    sim = S.multi_material("gpu")
+    sim.set_reorder_interval(1)
+    res = sim.config.grid_res
+    sim.step_n(3)
+    for _ in range(3):
+        old = sim.slot_order()
+        xy = sim.particles()["x"]
+        keys = oracle.stable_sort_cell_keys(xy, res)[0]  # keys in caller order
+        sim.step_n(1)
+        new = sim.slot_order()
+        assert np.array_equal(new, old[_stable_order(keys[old])])
+    sim.close()
