@@ -19,6 +19,7 @@
 #include "../../include/emerge_hip.h"
 #include "interop_kernels.hpp"
 #include "kernels.hpp"
+#include "sort_static.hpp"
 #include "spawn_kernels.hpp"
 #include "surrogate_kernels.hpp"
 
@@ -1838,6 +1839,21 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out
     (void)hipFree(d_hist);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+int emh_sort_cell_keys_noalloc(const float* xy, size_t n, uint32_t R, uint32_t* keys_out, uint32_t* perm_out,
+                               void* stream) {
+    if ((!xy || !keys_out || !perm_out) && n) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (R < 1 || R > 16384) return fail(EMH_ERR_INVALID_ARG, "grid_res out of range");
+    if (n > EMH_SORT_NOALLOC_MAX_KEYS)
+        return fail(EMH_ERR_INVALID_ARG, "%zu keys exceed the static scratch maximum of %u", n,
+                    EMH_SORT_NOALLOC_MAX_KEYS);
+    if (n == 0) return EMH_OK;
+    SortBuf sb;
+    HIP_TRY(noalloc_sort_scratch(sb));
+    HIP_TRY(sort_cell_keys_device(reinterpret_cast<const float2*>(xy), (uint32_t)n, R, keys_out, perm_out, sb,
+                                  static_cast<hipStream_t>(stream)));
+    return EMH_OK;
 }
 
 }  // extern "C"

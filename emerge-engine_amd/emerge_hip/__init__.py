@@ -248,6 +248,8 @@ def load_library() -> ctypes.CDLL:
         "emh_sim_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "emh_sim_set_reorder_interval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
         "emh_sim_get_slot_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+        "emh_sort_cell_keys_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
+                                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "emh_sim_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
                                                 P(ctypes.c_uint32)]),
         "emh_surrogate_create": (ctypes.c_int, [ctypes.c_uint64, P(ctypes.c_void_p)]),
@@ -327,6 +329,16 @@ def sort_cell_keys(xy: np.ndarray, grid_res: int) -> Tuple[np.ndarray, np.ndarra
     perm = np.zeros(n, dtype=np.uint32)
     _check(load_library().emh_sort_cell_keys(xy.ctypes.data, n, grid_res, keys.ctypes.data, perm.ctypes.data))
     return keys, perm
+
+
+SORT_NOALLOC_MAX_KEYS = 1048576  # EMH_SORT_NOALLOC_MAX_KEYS
+
+
+def sort_cell_keys_noalloc(xy_dev: int, n: int, grid_res: int, keys_dev: int, perm_dev: int, stream: int = 0) -> None:
+    """The allocation-free sort on device pointers (xy n x 2 f32, keys and perm n u32),
+    enqueued on `stream` (a hipStream_t handle, 0 = null stream) and not waited for."""
+    _check(load_library().emh_sort_cell_keys_noalloc(ctypes.c_void_p(xy_dev), n, grid_res, ctypes.c_void_p(keys_dev),
+                                                     ctypes.c_void_p(perm_dev), ctypes.c_void_p(stream)))
 
 
 # ----------------------------------------------------------------- config

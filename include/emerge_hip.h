@@ -457,6 +457,24 @@ uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
                        uint32_t* keys_out, uint32_t* perm_out);
 
+/* ---- the same sort without allocating ------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: emh_sort_cell_keys on caller-owned device buffers (xy:
+ * n x 2 floats, 8-byte aligned; keys_out, perm_out: n each, not aliasing xy or
+ * each other), with the same results, calling no allocator.  Its scratch is
+ * a static device array of EMH_SORT_NOALLOC_SCRATCH_BYTES on each GPU it runs
+ * on, held until the process ends; n above EMH_SORT_NOALLOC_MAX_KEYS fails
+ * with EMH_ERR_INVALID_ARG before any work.  The work is enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) and not waited for.  All calls on
+ * one GPU share its scratch: calls whose work can overlap (different streams
+ * or threads) race, so the caller serialises them.  Nothing in the scratch
+ * needs clearing between calls. */
+#define EMH_SORT_NOALLOC_MAX_KEYS 1048576u
+#define EMH_SORT_NOALLOC_SCRATCH_BYTES 17825792u /* (4 x 1048576 keys + 1024 x 256 counts) x 4 B */
+int emh_sort_cell_keys_noalloc(const float* xy_dev, size_t n, uint32_t grid_res, uint32_t* keys_out_dev,
+                               uint32_t* perm_out_dev, void* stream);
+
 /* ---------------------------------------------------------------------
  * Batched scenes.
  *
