@@ -232,29 +232,23 @@ def surrogate_section(args) -> dict:
 
 
 # ------------------------------------------------------------ batched scenes
-def build_batch_scenes(seed: int, n_scenes: int):
+def batch_scene_specs(seed: int, n_scenes: int) -> list:
     """Independent scenes: grid side uniform over {64, 128, 192, 256}; particle
     count log-uniform from 2k to 64k (capped at 16 per cell over 35 % of the
     interior, so the smallest grids hold at most ~17.5k); 1-4 boxes, one per
     interior quadrant, each of Drucker-Prager sand (E = 3.5e5, nu = 0.3,
     30 deg), NeoHookean (400, 200) or Newtonian water (1000, 1e4) at random;
-    gravity (0, -1), step dt 0.05 with each scene's own CFL substeps."""
+    gravity (0, -1), step dt 0.05 with each scene's own CFL substeps.  Plain
+    data: (grid side, [(w, h, cx, cy, seed, material slot), ...]) per scene."""
     rng = np.random.default_rng(seed)
-    lam, mu = eh.lame_from_young(3.5e5, 0.3)
-    sand = eh.DruckerPragerMaterial.new(lam, mu)
-    sand.params.dp_h0 = eh.to_radians(30.0)
-    solid = eh.NeoHookeanMaterial.new(400.0, 200.0)
-    water = eh.NewtonianFluidMaterial.low_viscosity(1000.0, 1e4)
-    sims = []
+    specs = []
     for k in range(n_scenes):
         r = int(rng.choice([64, 128, 192, 256]))
         cap = min(64000, int(0.35 * (r - 8) ** 2 * 16))
         n_target = float(np.exp(rng.uniform(np.log(2000.0), np.log(cap))))
         boxes = int(rng.integers(1, 5))
-        sim = (eh.Simulation.empty(eh.SimConfig.standard(r, 0.05, (0.0, -1.0)))
-               .with_material(0, sand).with_material(1, solid).with_material(2, water)
-               .with_boundary(eh.SlipBoundary.new(2)))
         q = (r - 8) / 2.0
+        bl = []
         for b in range(boxes):
             area = n_target / boxes / 16.0
             aspect = rng.uniform(0.6, 1.6)
@@ -262,11 +256,41 @@ def build_batch_scenes(seed: int, n_scenes: int):
             h = int(max(2, min(q - 2, round(area / w))))
             cx = 4.0 + q * (b % 2) + q / 2 + rng.uniform(-0.5, 0.5) * (q - w - 1)
             cy = 4.0 + q * (b // 2) + q / 2 + rng.uniform(-0.5, 0.5) * (q - h - 1)
-            sim.add_body(eh.SpawnRegion(spacing=0.25, box_size=(w, h), box_center=(float(cx), float(cy)),
-                                        position_jitter=0.5, rng_seed=seed * 1000 + k * 4 + b,
-                                        material_id=int(rng.integers(0, 3))))
-        sims.append(sim)
-    return sims
+            bl.append((w, h, float(cx), float(cy), seed * 1000 + k * 4 + b, int(rng.integers(0, 3))))
+        specs.append((r, bl))
+    return specs
+
+
+def build_batch_scene(spec, backend: str = "gpu"):
+    """One scene of batch_scene_specs on the GPU ("gpu") or the CPU oracle ("oracle32" / "oracle64")."""
+    r, boxes = spec
+    cfg = eh.SimConfig.standard(r, 0.05, (0.0, -1.0))
+    if backend == "gpu":
+        lam, mu = eh.lame_from_young(3.5e5, 0.3)
+        sand = eh.DruckerPragerMaterial.new(lam, mu)
+        sand.params.dp_h0 = eh.to_radians(30.0)
+        solid = eh.NeoHookeanMaterial.new(400.0, 200.0)
+        water = eh.NewtonianFluidMaterial.low_viscosity(1000.0, 1e4)
+        sim = eh.Simulation.empty(cfg)
+    else:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle_py
+        lam, mu = oracle_py.lame_from_young(3.5e5, 0.3)
+        sand = oracle_py.material("drucker_prager", lam, mu)
+        sand.params.dp_h0 = oracle_py.to_radians(30.0)
+        solid = oracle_py.material("neohookean", 400.0, 200.0)
+        water = oracle_py.material("fluid", 1000.0, 1.0e-3, 1e4, 7.0)  # low_viscosity  fluid.rs:73-75
+        sim = oracle_py.OracleSim.empty(cfg, 64 if backend == "oracle64" else 32)
+    sim = sim.with_material(0, sand).with_material(1, solid).with_material(2, water).with_boundary(
+        eh.SlipBoundary.new(2))
+    for w, h, cx, cy, rs, m in boxes:
+        sim.add_body(eh.SpawnRegion(spacing=0.25, box_size=(w, h), box_center=(cx, cy), position_jitter=0.5,
+                                    rng_seed=rs, material_id=m))
+    return sim
+
+
+def build_batch_scenes(seed: int, n_scenes: int):
+    return [build_batch_scene(sp) for sp in batch_scene_specs(seed, n_scenes)]
 
 
 def batch_section(args) -> dict:

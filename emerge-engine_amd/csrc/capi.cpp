@@ -364,6 +364,8 @@ struct emh_sim {
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
     // learned NACC return mapping: 0 plain, 1 harvest trial states, 2 infer
     uint32_t sg_mode = 0;
+    emh_surrogate* sg_attached = nullptr;  // the surrogate of modes 1 and 2
+    uint64_t cfg_epoch = 0;                // bumped by in-place edits of pipe.cfg (set_gravity)
     // sleep partition: on when sleep_threshold > 0 or an uploaded particle sleeps
     bool sleep = false;
     float sleep_sq() const {
@@ -401,7 +403,7 @@ struct emh_sim {
     uint32_t* d_snap_cells = nullptr;  // 3 R^2 per-cell material ranges and counts
     std::vector<void*> particle_allocs;
 
-    ~emh_sim() { release(); }
+    ~emh_sim();
 
     void release_particles() {
         for (void* p : particle_allocs) (void)hipFree(p);
@@ -1601,6 +1603,7 @@ int emh_sim_set_gravity(emh_sim* s, float gx, float gy) {  // lifecycle.rs:403-4
     s->cfg.gravity[1] = gy;
     s->pipe.cfg.gx = gx;  // read by the next grid update; nothing queued holds the old value
     s->pipe.cfg.gy = gy;
+    ++s->cfg_epoch;  // a batch holding this scene refreshes its device copy of the pipeline
     return EMH_OK;
 }
 
@@ -1887,8 +1890,22 @@ struct emh_surrogate {
     float* d_loss = nullptr;
     uint64_t adam_t = 0;
     float norm[kSgNorm] = {};
+    std::vector<emh_sim*> users;  // sims in harvest or surrogate mode with this network
 
-    ~emh_surrogate() { release(); }
+    ~emh_surrogate() {
+        for (emh_sim* u : users) {  // (the caller was to detach them first)
+            u->sg_attached = nullptr;
+            u->sg_mode = 0;
+            u->sg_host = SgDev{};
+        }
+        release();
+    }
+    void detach(emh_sim* s) { users.erase(std::remove(users.begin(), users.end(), s), users.end()); }
+    bool other_harvester(const emh_sim* s) const {
+        for (const emh_sim* u : users)
+            if (u != s && u->sg_mode == 1) return true;
+        return false;
+    }
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* q : {(void*)d_w, (void*)d_m, (void*)d_v, (void*)d_grad, (void*)d_norm, (void*)d_pk, (void*)d_bias,
@@ -1985,6 +2002,11 @@ struct emh_surrogate {
         return b;
     }
 };
+
+emh_sim::~emh_sim() {
+    if (sg_attached) sg_attached->detach(this);
+    release();
+}
 
 namespace {
 uint64_t gcd_u64(uint64_t a, uint64_t b) {
@@ -2108,6 +2130,12 @@ int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uin
     if (mode != 0 && sg->device != s->device) return fail(EMH_ERR_INVALID_ARG, "surrogate and sim are on different devices");
     if (mode == 1) {
         if (harvest_capacity > sg->rec_cap) {
+            // another scene's device descriptor points at the current buffer
+            if (sg->other_harvester(s))
+                return fail(EMH_ERR_STATE,
+                            "another simulation harvests into this surrogate's buffer: its capacity (%llu records) "
+                            "cannot grow until that simulation leaves harvest mode",
+                            (unsigned long long)sg->rec_cap);
             if (sg->d_rec) (void)hipFree(sg->d_rec);
             sg->d_rec = dalloc<float>(harvest_capacity * 12);
             sg->rec_cap = sg->d_rec ? harvest_capacity : 0;
@@ -2126,6 +2154,9 @@ int emh_sim_set_surrogate_mode(emh_sim* s, emh_surrogate* sg, uint32_t mode, uin
         sg->rec_used = harvest_capacity;
     }
     s->sg_mode = mode;
+    if (s->sg_attached) s->sg_attached->detach(s);
+    s->sg_attached = mode == 0 ? nullptr : sg;
+    if (s->sg_attached) s->sg_attached->users.push_back(s);
     if (s->d_ctrl) {
         HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(s->d_ctrl) + offsetof(Ctrl, sg), &s->sg_host, sizeof(SgDev),
                                hipMemcpyHostToDevice, s->stream));
@@ -2437,6 +2468,7 @@ struct emh_batch {
     uint32_t substeps_since_reorder = kBatchReorderSubsteps;
     uint32_t predicted = 1;
     uint32_t max_substeps = 1;
+    std::vector<uint64_t> epochs;  // each scene's cfg_epoch in d_pipes
 
     ~emh_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -2451,6 +2483,7 @@ struct emh_batch {
     int sync() {
         const uint32_t n = (uint32_t)sims.size();
         std::vector<Pipeline> pipes(n);
+        epochs.assign(n, 0);
         std::vector<uint32_t> st(4 * (n + 1), 0u);
         uint32_t* one = st.data();
         uint32_t* sorts = one + (n + 1);
@@ -2471,9 +2504,12 @@ struct emh_batch {
             }
             if (rc) return rc;
             if (s->pipe.n == 0) return fail(EMH_ERR_INVALID_ARG, "scene %u of the batch has no particles", i);
+            if (s->sg_mode != 0)
+                return fail(EMH_ERR_UNSUPPORTED, "scene %u uses the surrogate harvest / inference mode", i);
             if (s->sleep) return fail(EMH_ERR_UNSUPPORTED, "scene %u uses the sleep partition", i);
             if (s->seq != 1) return fail(EMH_ERR_STATE, "scene %u did not restart its loop", i);
             pipes[i] = s->pipe;
+            epochs[i] = s->cfg_epoch;
             const uint32_t chunks = (s->pipe.n + kChunk - 1) / kChunk;
             const bool d = s->pipe.cfg.density_pass != 0;
             one[i + 1] = one[i] + 1;
@@ -2539,11 +2575,25 @@ struct emh_batch {
         return EMH_OK;
     }
 
+    // Scenes whose configuration was edited in place since the batch copied
+    // their pipelines (set_gravity): refresh those copies, stream-ordered
+    // after the launches already queued.
+    int refresh_configs() {
+        for (size_t i = 0; i < sims.size(); ++i) {
+            if (sims[i]->cfg_epoch == epochs[i]) continue;
+            HIP_TRY(hipMemcpyAsync(d_pipes + i, &sims[i]->pipe, sizeof(Pipeline), hipMemcpyHostToDevice, stream));
+            epochs[i] = sims[i]->cfg_epoch;
+        }
+        return EMH_OK;
+    }
+
     int step(uint32_t n_steps) {
         if (!in_sync()) {
             int rc = sync();
             if (rc) return rc;
         }
+        int rc = refresh_configs();
+        if (rc) return rc;
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
             const bool reorder = substeps_since_reorder >= kBatchReorderSubsteps;

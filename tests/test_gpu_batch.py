@@ -73,6 +73,34 @@ def test_batch_resyncs_after_edits():
 
 
 @pytest.mark.gpu
+def test_batch_follows_gravity_change():
+    """set_gravity on a batched scene between batch steps applies from the next batch
+    step, as it does on the scene stepped alone."""
+    names = ["corotated_drop", "sand_column"]
+    alone = [S.SCENES[n].build("gpu") for n in names]
+    batched = [S.SCENES[n].build("gpu") for n in names]
+    batch = eh.SceneBatch(batched)
+    batch.step_n(2)
+    for s in alone:
+        s.step_n(2)
+    for group in (alone, batched):
+        group[0].set_gravity((3.0, -30.0))
+    batch.step_n(4)
+    for s in alone:
+        s.step_n(4)
+    for name, a, b in zip(names, alone, batched):
+        assert b.stats().total_substeps == a.stats().total_substeps, name
+        e = S.rel_errors(S.state_matrix(b.particles()), S.state_matrix(a.particles()))
+        assert e["x"] < 2e-5 and e["F"] < 1e-3, (name, e)
+    # the changed scene really moved differently from the unchanged one
+    ref = S.SCENES[names[0]].build("gpu")
+    ref.step_n(6)
+    assert float(np.abs(ref.particles()["x"] - batched[0].particles()["x"]).max()) > 1e-2
+    batch.close()
+    _close(alone + batched + [ref])
+
+
+@pytest.mark.gpu
 def test_batch_errors():
     sims = [S.corotated_drop("gpu"), S.snow_drop("gpu")]
     with pytest.raises(eh.EmergeError) as ei:
@@ -89,5 +117,15 @@ def test_batch_errors():
         eh.SceneBatch([sims[1], nacc])
     assert ei.value.code == -2
     nacc.set_surrogate_mode(None, "plain")
+    # a scene switched to a surrogate mode after batching fails the next batch step
+    batch = eh.SceneBatch([sims[1], nacc])
+    batch.step_n(1)
+    nacc.set_surrogate_mode(sg, "surrogate")
+    with pytest.raises(eh.EmergeError) as ei:
+        batch.step_n(1)
+    assert ei.value.code == -2
+    nacc.set_surrogate_mode(None, "plain")
+    batch.step_n(1)
+    batch.close()
     _close(sims + [empty, nacc])
     sg.close()

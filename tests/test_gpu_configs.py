@@ -1,0 +1,160 @@
+"""The two workloads that are not the headline scene, at their full sizes:
+
+- 256 batched scenes (grid sides 64-256, 2k-61k particles each, 3.77M in all;
+  Drucker-Prager sand, NeoHookean and Newtonian water boxes) stepped as one
+  batch: every scene takes the substep schedule it takes alone; a sample of
+  scenes matches the CPU oracle after one step to the parity tests' bound;
+  masses, ids and finiteness hold for all of them.
+- NACC soil on a heightmap slope (512^2, 1.97M particles, F advanced before the
+  return mapping): the plain return mappings of a whole substep match the
+  oracle's NaccMaterial::project; a surrogate trained on the scene's own
+  harvested trial states stays within its stress-error bound and below the
+  trial state's error by a wide margin.
+"""
+import numpy as np
+import pytest
+
+import emerge_hip as eh
+import scenes as S
+
+ONE_STEP_TOL = {"x": 1e-6, "v": 2e-3, "C": 5e-3, "F": 1e-4, "Jp": 1e-5, "q": 1e-4, "lvs": 2e-2, "density": 2e-5}
+RATIO, FLOOR = 8.0, 2e-6
+
+
+def _permuted_step(spec, backend, seed):
+    import bench
+    sim = bench.build_batch_scene(spec, backend)
+    p = sim.particles()
+    perm = np.random.default_rng(seed).permutation(len(p))
+    sim.set_particles(p[perm])
+    sim.step_n(1)
+    return S.state_matrix(sim.particles()), np.argsort(perm)
+
+
+@pytest.mark.gpu
+def test_batched_256_scenes_full_size(oracle):
+    import bench
+    specs = bench.batch_scene_specs(31, 256)
+    together = [bench.build_batch_scene(sp) for sp in specs]
+    alone = [bench.build_batch_scene(sp) for sp in specs]
+    n = np.array([s.num_particles() for s in together])
+    print(f"256 scenes, {n.sum()} particles ({n.min()}..{n.max()})", flush=True)
+    assert n.sum() > 3.5e6
+    p0 = [s.particles() for s in together]
+    batch = eh.SceneBatch(together)
+    batch.step_n(1)
+    for s in alone:
+        s.set_reorder_interval(8)  # the batch path's interval
+        s.step_n(1)
+    # every scene: its own schedule, masses and ids kept, finite, inside the domain
+    for k, (a, b) in enumerate(zip(alone, together)):
+        assert b.stats().total_substeps == a.stats().total_substeps, k
+        p = b.particles()
+        assert np.array_equal(p["mass"], p0[k]["mass"]) and np.array_equal(p["user_tag"], p0[k]["user_tag"])
+        assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(p["deformation_gradient"]))
+        r = specs[k][0]
+        assert p["x"].min() >= 1.0 and p["x"].max() <= r - 2.0
+    # a sample of scenes against the f32 and FP64 oracles (the parity tests' one-step bound)
+    sample = list(range(0, 256, 37))
+    for k in sample:
+        o32 = bench.build_batch_scene(specs[k], "oracle32")
+        o64 = bench.build_batch_scene(specs[k], "oracle64")
+        o32.step_n(1)
+        o64.step_n(1)
+        assert o32.stats().last_substeps == together[k].stats().last_substeps, k
+        g = S.state_matrix(together[k].particles())
+        r32 = S.state_matrix(o32.particles())
+        r64 = S.f64_state(o64.particles_f64())
+        e_g32 = S.rel_errors(g, r32)
+        spread = S.rel_errors(r32, r64)
+        for seed in (1, 2):
+            st, inv = _permuted_step(specs[k], "oracle32", seed)
+            e = S.rel_errors({f: v[inv] for f, v in st.items()}, r64)
+            spread = {f: max(spread[f], e[f]) for f in spread}
+        print(f"scene {k}: R={specs[k][0]} n={n[k]} substeps {o32.stats().last_substeps}: "
+              + " ".join(f"{f} {e_g32[f]:.1e}/{spread[f]:.1e}" for f in ("x", "v", "C", "F", "density")), flush=True)
+        for f, tol in ONE_STEP_TOL.items():
+            bound = max(tol, (RATIO + 1.0) * spread[f] + FLOOR)
+            assert e_g32[f] <= bound, (k, f, e_g32[f], bound)
+        o32.close()
+        o64.close()
+    # two more batch steps: schedules still equal, positions within f32 roundoff of the grid sums
+    batch.step_n(2)
+    for s in alone:
+        s.step_n(2)
+    dx = 0.0
+    for k, (a, b) in enumerate(zip(alone, together)):
+        assert b.stats().total_substeps == a.stats().total_substeps, k
+        dx = max(dx, float(np.abs(a.particles()["x"].astype(np.float64) - b.particles()["x"]).max()))
+    print(f"max |dx| batched vs alone after 3 steps: {dx:.2e} cells")
+    assert dx < 1e-3
+    batch.close()
+    for s in alone + together:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_nacc_slope_full_size(oracle):
+    import bench
+    import surrogate_ref
+    sim, mat = bench.build_nacc_slope(202)
+    n = sim.num_particles()
+    print(f"nacc slope: {n} particles", flush=True)
+    assert n > 1.9e6
+    p0 = sim.particles()
+    sim.step_n(400)  # the blocks land on the slope and yield (the bench evaluates here too)
+    p = sim.particles()
+    assert np.array_equal(p["mass"], p0["mass"]) and np.array_equal(p["user_tag"], p0["user_tag"])
+    F = p["deformation_gradient"].astype(np.float64)
+    J = F[:, 0] * F[:, 3] - F[:, 2] * F[:, 1]
+    assert np.all(np.isfinite(p["x"])) and np.all(np.isfinite(F)) and np.all(J > 0)
+    assert p["x"][:, 1].min() >= 20.0  # above the terrain's lowest point (terrain + 1 push)
+
+    # every trial state of one substep, returned by the device's plain mapping and the oracle's
+    sg = eh.NaccSurrogate(5)
+    sim.set_surrogate_mode(sg, "harvest", 2 * n, 1)
+    sim.step_n(1)
+    sim.synchronize()
+    sim.set_surrogate_mode(None, "plain")
+    rec = sg.records(0, n)
+    _, fp, ap = sg.eval_records(mat, 0, n, False, 1)
+    mu, kappa = float(rec[0, 5]), float(rec[0, 6])
+    om = oracle.material("nacc", mu, kappa, mat.params.stretch_limit, mat.params.compression_limit,
+                         mat.params.hardening_exponent)
+    om.ext.nacc_advance_f = 1
+    idx = np.random.default_rng(0).permutation(n)[:40000]
+    f32, a32 = oracle.nacc_project_batch(om, rec[idx, :4], rec[idx, 4], 32)
+    yl = ~np.all(fp[idx] == rec[idx, :4], axis=1)
+    print(f"yielding in the sample: {int(yl.sum())} of {len(idx)}", flush=True)
+    assert yl.sum() > 1000
+    ef = np.abs(fp[idx] - f32).max() / np.abs(f32).max()
+    ea = np.abs(ap[idx] - a32).max() / max(np.abs(a32).max(), 1e-3)
+    assert ef < 1e-5 and ea < 1e-4, (ef, ea)
+    assert np.array_equal(fp[idx][~yl], f32[~yl].astype(np.float32))  # elastic states untouched, bit for bit
+
+    # a surrogate trained on this scene's own Case C states, evaluated on the same substep
+    n_set = sg.build_training_set()
+    assert n_set > 10000
+    loss = sg.train(epochs=150, batch=16384, lr=1e-2, lr_final=1e-4, seed=3)
+    assert loss[-1] < 0.2 * loss[0]
+    sg.counts(reset=True)
+    _, fs, _ = sg.eval_records(mat, 0, n, True, 1)
+    c = sg.counts(reset=True)
+    assert c["yielding"] == c["surrogate"] + c["fallback"] and c["surrogate"] > 0
+    f_trial = rec[:, :4]
+    yall = ~np.all(fp == f_trial, axis=1)
+    tp = surrogate_ref.nacc_kirchhoff(fp[yall], mu, kappa)
+    ts = surrogate_ref.nacc_kirchhoff(fs[yall], mu, kappa)
+    tt = surrogate_ref.nacc_kirchhoff(f_trial[yall], mu, kappa)
+    err = float(np.linalg.norm(ts - tp) / np.linalg.norm(tp))
+    err_trial = float(np.linalg.norm(tt - tp) / np.linalg.norm(tp))
+    print(f"surrogate stress error {err:.2e} (trial state {err_trial:.2e}), counts {c}", flush=True)
+    assert err < 2e-3 and err_trial > 10 * err
+    # the scene keeps running with the surrogate in the substep
+    sim.set_surrogate_mode(sg, "surrogate")
+    sim.step_n(3)
+    sim.set_surrogate_mode(None, "plain")
+    q = sim.particles()
+    assert np.all(np.isfinite(q["x"])) and np.all(np.isfinite(q["deformation_gradient"]))
+    sim.close()
+    sg.close()

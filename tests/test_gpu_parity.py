@@ -60,8 +60,10 @@ def _flips(a, ref, tol):
     return int(np.sum(np.abs(np.asarray(a, np.float64) - ref) > tol))
 
 
-def _run(scene, backend, steps, perm_seed=None):
+def _run(scene, backend, steps, perm_seed=None, reorder=None):
     sim = scene.build(backend)
+    if reorder is not None and backend == "gpu":
+        sim.set_reorder_interval(reorder)  # physical reorders mid-run, not just at the upload
     inv = None
     if perm_seed is not None:  # same particles, another order
         p = sim.particles()
@@ -83,12 +85,18 @@ def _run(scene, backend, steps, perm_seed=None):
     return out, sim
 
 
+# every scene as configured (the physical reorder at the upload, then every
+# 4096 substeps), plus two scenes reordering every 8 substeps
+PARITY_CASES = [(n, None) for n in S.SCENES] + [("multi_material", 8), ("sand_column", 8)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", list(S.SCENES))
-def test_scene_parity(name, oracle, capsys):
+@pytest.mark.parametrize("name,reorder", PARITY_CASES,
+                         ids=[n if r is None else f"{n}-reorder{r}" for n, r in PARITY_CASES])
+def test_scene_parity(name, reorder, oracle, capsys):
     scene = S.SCENES[name]
     steps = [1, scene.steps]
-    g, gsim = _run(scene, "gpu", steps)
+    g, gsim = _run(scene, "gpu", steps, reorder=reorder)
     o32, osim = _run(scene, "oracle32", steps)
     o32p = [_run(scene, "oracle32", steps, perm_seed=seed)[0] for seed in PERM_SEEDS]
     o64, _ = _run(scene, "oracle64", steps)

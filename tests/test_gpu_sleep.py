@@ -16,8 +16,11 @@ def _flags(p):
 
 
 @pytest.mark.gpu
-def test_sleep_matches_oracle(oracle):
+@pytest.mark.parametrize("reorder", [None, 8], ids=["upload_reorder", "reorder8"])
+def test_sleep_matches_oracle(reorder, oracle):
     g = S.sleep_collide("gpu")
+    if reorder is not None:
+        g.set_reorder_interval(reorder)  # sleeping particles move slots with the rest
     o = S.sleep_collide("oracle32")
     n = g.num_particles()
     done = 0

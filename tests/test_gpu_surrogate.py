@@ -279,3 +279,28 @@ def test_substep_surrogate_mode():
     for s in (sim, plain, learned, other):
         s.close()
     sg.close()
+
+
+@pytest.mark.gpu
+def test_shared_harvest_buffer_cannot_grow_under_another_harvester():
+    """Two scenes harvesting into one surrogate: growing the buffer while the other
+    scene's device descriptor still points at it is refused (EMH_ERR_STATE); once
+    that scene leaves harvest mode the buffer may grow, and both scenes harvest."""
+    sg = eh.NaccSurrogate(2)
+    a, b = S.nacc_block("gpu"), S.nacc_block("gpu")
+    a.set_surrogate_mode(sg, "harvest", harvest_capacity=4096)
+    b.set_surrogate_mode(sg, "harvest", harvest_capacity=4096)  # same size: no reallocation
+    with pytest.raises(eh.EmergeError) as ei:
+        b.set_surrogate_mode(sg, "harvest", harvest_capacity=1 << 16)
+    assert ei.value.code == -5
+    a.step_n(1)  # a still harvests into the buffer it was given
+    offered, stored = sg.harvest_count()
+    assert offered > 0 and stored > 0
+    a.set_surrogate_mode(None, "plain")
+    b.set_surrogate_mode(sg, "harvest", harvest_capacity=1 << 16)
+    b.step_n(1)
+    offered, stored = sg.harvest_count()
+    assert offered == b.num_particles() * b.stats().last_substeps and stored == offered
+    for s in (a, b):
+        s.close()
+    sg.close()
