@@ -68,18 +68,74 @@ def build_workload(backend: str = "gpu"):
     return sim
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(target_s: float, first_dt: float) -> dict:
-    """The oracle (f32 restatement of the reference's CPU Simulation::step,
-    single thread) on a bounded sample: the same 1M-particle scene, a few
-    substeps at the GPU run's first CFL dt."""
-    sim = build_workload("oracle")
+    """The oracle (f32 restatement of the reference's CPU Simulation::step) on a
+    bounded sample: the same 1M-particle scene, a few substeps at the GPU run's
+    first CFL dt, on one thread and then on every core this process may use
+    (G2P phase 1 in parallel, as the reference's rayon loop runs it; P2G and
+    the plastic update stay sequential, as in the reference)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+    cores = len(os.sched_getaffinity(0))
+    threads = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    rates = {}
+    for t_count, share in ((1, 0.6), (threads, 0.4)):
+        oracle_py.set_threads(t_count)
+        try:
+            sim = build_workload("oracle")
+            n = sim.num_particles()
+            t1 = sim.time_substeps(1, first_dt)
+            k = max(1, min(400, int(target_s * share / max(t1, 1e-6))))
+            t = sim.time_substeps(k, first_dt)
+            sim.close()
+        finally:
+            oracle_py.set_threads(1)
+        rates[t_count] = (n * k / t, k, t)
+    v1, k1, t1 = rates[1]
+    vn, kn, tn = rates[threads]
+    return {"value": vn, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles x {kn} substeps (dt {first_dt:.3e}) of the same workload, after 1 untimed "
+                      f"substep; oracle f32, G2P phase 1 on {threads} threads, {tn:.1f} s",
+            "single_thread": {"value": v1, "cores": 1, "sample": f"{n} particles x {k1} substeps, {t1:.1f} s"},
+            "nproc": os.cpu_count(), "cores_available": cores, "cpu_model": _cpu_model()}
+
+
+def full_run_section(args, tuning) -> dict:
+    """The headline scene's whole run as its config states it (500 steps from
+    rest, the column collapsing), timed as one run after the same autotuned
+    launch settings, with the substep kernel's live average over all of it."""
+    sim = build_workload("gpu")
     n = sim.num_particles()
-    t1 = sim.time_substeps(1, first_dt)
-    k = max(1, min(400, int(target_s / max(t1, 1e-6))))
-    t = sim.time_substeps(k, first_dt)
-    return {"value": n * k / t, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} particles x {k} substeps (dt {first_dt:.3e}) of the same workload, after 1 untimed "
-                      f"substep; oracle f32, 1 thread, {t:.1f} s"}
+    if tuning is not None:
+        sim.set_tuning(tuning["chosen"]["reorder_interval"], bool(tuning["chosen"]["graphs"]))
+    sim.set_profiling(True)
+    t0 = time.perf_counter()
+    sim.step_n(args.full_run_steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    substeps = sim.stats().total_substeps
+    times = {k: v for k, v in sim.kernel_times().items() if v[0] > 0}
+    sim.set_profiling(False)
+    sim.close()
+    launches, total_ms = times["k_substep"]
+    avg_s = total_ms / launches / 1e3
+    achieved = SUBSTEP_BYTES * n / avg_s / 1e9
+    return {"steps": args.full_run_steps, "substeps": substeps,
+            "particle_substeps_per_s": n * substeps / elapsed, "particle_steps_per_s": n * args.full_run_steps / elapsed,
+            "seconds": elapsed, "k_substep_avg_us": avg_s * 1e6,
+            "k_substep_roofline_frac": achieved / HBM_PEAK_GBS,
+            "note": "includes the first step's upload, the physical reorders and the whole collapse"}
 
 
 # ------------------------------------------------------------ NACC surrogate
@@ -305,6 +361,14 @@ def batch_section(args) -> dict:
         s.step_n(1)
     batch = eh.SceneBatch(together)
     batch.step_n(1)
+    tuning = None
+    if not args.no_autotune:
+        chosen, timed = batch.autotune(3)
+        tuning = {"chosen": chosen, "candidates": timed, "trial_steps": 3}
+        batch.step_n(1)  # the re-upload after the trials, outside the timed region
+        for s in alone:
+            s.set_tuning(chosen["reorder_interval"], bool(chosen["graphs"]))
+            s.step_n(1)
     build_s = time.perf_counter() - t0
     sa0 = np.array([s.stats().total_substeps for s in alone])
     sb0 = np.array([s.stats().total_substeps for s in together])
@@ -337,6 +401,7 @@ def batch_section(args) -> dict:
         "separate": {"particle_substeps_per_s": work_a / t_alone, "seconds": t_alone},
         "speedup": (work_b / t_batch) / (work_a / t_alone),
         "build_s": build_s,
+        "tuning": tuning,
     }
 
 
@@ -361,6 +426,10 @@ def main() -> int:
     ap.add_argument("--sg-substep-steps", type=int, default=10)
     ap.add_argument("--sg-fp64-sample", type=int, default=50_000)
     ap.add_argument("--sg-cpu-sample", type=int, default=200_000)
+    ap.add_argument("--no-autotune", action="store_true", help="time the default launch settings")
+    ap.add_argument("--tune-steps", type=int, default=10, help="timed steps per autotuner candidate")
+    ap.add_argument("--full-run-steps", type=int, default=500,
+                    help="the whole collapse from rest, timed as one run (0: skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes section")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -379,6 +448,15 @@ def main() -> int:
     sim.step_n(max(args.warmup, 1))
     sim.synchronize()
     first_dt = sim.effective_dt()
+    # launch settings (reorder interval, hipGraph replay) timed on this scene
+    # from this state, which the autotuner then puts back; one untimed step
+    # takes the re-upload before the timed region
+    tuning = None
+    if not args.no_autotune:
+        chosen, timed = sim.autotune(args.tune_steps)
+        tuning = {"chosen": chosen, "candidates": timed, "trial_steps": args.tune_steps}
+        sim.step_n(1)
+        sim.synchronize()
     build_s = time.perf_counter() - t_build
 
     # live per-kernel timing over the timed steps: HIP events on the
@@ -438,12 +516,14 @@ def main() -> int:
         "kernel_avg_us": {k: v[1] / v[0] * 1e3 for k, v in active.items()},
         "state_finite": finite,
         "setup_s": build_s,
+        "tuning": tuning,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, first_dt)
     else:
         out["cpu_baseline"] = None
     sim.close()
+    out["full_run"] = None if args.full_run_steps <= 0 else full_run_section(args, tuning)
     out["nacc_surrogate"] = None if args.no_surrogate else surrogate_section(args)
     out["batched_scenes"] = None if args.no_batch else batch_section(args)
     print(json.dumps(out), flush=True)

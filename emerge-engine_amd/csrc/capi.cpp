@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -337,6 +338,80 @@ template <class T> T* dalloc(size_t n) {
 }  // namespace
 
 // ------------------------------------------------------------------ emh_sim
+// The settings the autotuners time: reorder intervals (substeps) x graph replay.
+static std::vector<emh_tuning> tuning_candidates(const uint32_t* intervals, size_t n) {
+    std::vector<emh_tuning> v;
+    for (uint32_t g = 0; g < 2; ++g)
+        for (size_t i = 0; i < n; ++i) {
+            emh_tuning t{};
+            t.reorder_interval = intervals[i];
+            t.graphs = g;
+            v.push_back(t);
+        }
+    return v;
+}
+static std::vector<emh_tuning> tuning_candidates() {
+    static const uint32_t kIntervals[] = {8u, 64u, 512u, kReorderSubsteps};
+    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]));
+}
+static std::vector<emh_tuning> batch_tuning_candidates() {
+    static const uint32_t kIntervals[] = {4u, kBatchReorderSubsteps, 32u, 256u};
+    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]));
+}
+
+// Instantiated graphs of runs of substep launches, keyed by (launch number
+// mod 6, launches, kernel variant); valid for one kernel argument (`pipe`).
+struct GraphCache {
+    struct Entry {
+        uint32_t r, slots, variant;
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t kMax = 24;
+    std::vector<Entry> entries;
+    std::vector<unsigned char> arg;  // the argument bytes the entries captured
+
+    ~GraphCache() { clear(); }
+    void clear() {
+        for (auto& e : entries) (void)hipGraphExecDestroy(e.exec);
+        entries.clear();
+    }
+    // Drops every entry when the argument differs from the captured one.
+    void check_arg(const void* a, size_t n) {
+        if (arg.size() == n && std::memcmp(arg.data(), a, n) == 0) return;
+        clear();
+        arg.assign((const unsigned char*)a, (const unsigned char*)a + n);
+    }
+    hipGraphExec_t find(uint32_t r, uint32_t slots, uint32_t variant) const {
+        for (const auto& e : entries)
+            if (e.r == r && e.slots == slots && e.variant == variant) return e.exec;
+        return nullptr;
+    }
+    // Captures `launch(k)` for k < slots on `stream` into a new entry.
+    template <class F>
+    hipError_t capture(hipStream_t stream, uint32_t r, uint32_t slots, uint32_t variant, F&& launch,
+                       hipGraphExec_t* out) {
+        *out = nullptr;
+        if (entries.size() >= kMax) {
+            (void)hipGraphExecDestroy(entries.front().exec);
+            entries.erase(entries.begin());
+        }
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (uint32_t k = 0; k < slots && e == hipSuccess; ++k) e = launch(k);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(stream, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            *out = nullptr;
+            return e;
+        }
+        entries.push_back(Entry{r, slots, variant, *out});
+        return hipSuccess;
+    }
+};
+
 struct emh_sim {
     emh_config cfg{};
     std::vector<emh_material_params> mats;
@@ -366,6 +441,13 @@ struct emh_sim {
     uint32_t sg_mode = 0;
     emh_surrogate* sg_attached = nullptr;  // the surrogate of modes 1 and 2
     uint64_t cfg_epoch = 0;                // bumped by in-place edits of pipe.cfg (set_gravity)
+    // hipGraph replay of a step's substep launches (emh_sim_set_graphs): the
+    // kernels read their launch number only modulo 6 (control-state parity
+    // and reduction slot), so a run of `slots` launches captured at residue r
+    // replays for every later run at that residue while the pipeline (the
+    // launches' argument) is unchanged.
+    bool use_graphs = false;
+    GraphCache graphs;
     // sleep partition: on when sleep_threshold > 0 or an uploaded particle sleeps
     bool sleep = false;
     float sleep_sq() const {
@@ -446,6 +528,7 @@ struct emh_sim {
 
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
+        graphs.clear();
         release_events();
         release_particles();
         if (d_mats) (void)hipFree(d_mats);
@@ -796,6 +879,66 @@ struct emh_sim {
         return restart_loop(st.cur_buf);
     }
 
+    // Time each tuning candidate on this scene: from the same saved state,
+    // one untimed step (upload, reorder, graph capture), then `trial` steps
+    // timed on the host clock up to their completion.  The state is put back
+    // afterwards (re-uploaded from the saved copy at the next step, stats
+    // restored) and the fastest candidate applied.
+    int autotune(uint32_t trial, emh_tuning* best, emh_tuning* timed, uint32_t cap, uint32_t* n_timed) {
+        if (trial == 0) return fail(EMH_ERR_INVALID_ARG, "trial_steps must be at least 1");
+        if (profiling) return fail(EMH_ERR_STATE, "turn profiling off before autotuning");
+        if (pipe.n == 0 && host.empty()) return fail(EMH_ERR_STATE, "no particles to tune on");
+        int rc = host_current();
+        if (rc) return rc;
+        const std::vector<emh_particle> saved = host;
+        const emh_step_stats saved_stats = stats;
+        const uint32_t saved_interval = reorder_substeps, saved_since = substeps_since_reorder,
+                       saved_pred = predicted_substeps;
+        const bool saved_graphs = use_graphs;
+        const std::vector<emh_tuning> cands = tuning_candidates();
+        std::vector<float> us(cands.size(), 0.0f);
+        auto restore = [&]() {
+            host = saved;
+            host_dirty = true;
+            device_ahead = false;  // the host copy, not the last trial's device state, is current
+            stats = saved_stats;
+            substeps_since_reorder = saved_since;
+            predicted_substeps = saved_pred;
+        };
+        for (size_t c = 0; c < cands.size() && rc == EMH_OK; ++c) {
+            restore();
+            reorder_substeps = cands[c].reorder_interval;
+            use_graphs = cands[c].graphs != 0;
+            rc = step(1);
+            if (rc) break;
+            const uint64_t s0 = stats.total_substeps;
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = step(trial);
+            const auto t1 = std::chrono::steady_clock::now();
+            const double work = (double)std::max<uint64_t>(stats.total_substeps - s0, 1) * (double)pipe.n;
+            us[c] = (float)(std::chrono::duration<double, std::nano>(t1 - t0).count() / work);
+        }
+        restore();
+        reorder_substeps = saved_interval;
+        use_graphs = saved_graphs;
+        if (rc) return rc;
+        size_t bi = 0;
+        for (size_t c = 1; c < cands.size(); ++c)
+            if (us[c] < us[bi]) bi = c;
+        for (size_t c = 0; c < cands.size() && c < cap && timed; ++c) {
+            timed[c] = cands[c];
+            timed[c].ns_per_particle_substep = us[c];
+        }
+        if (n_timed) *n_timed = (uint32_t)cands.size();
+        reorder_substeps = cands[bi].reorder_interval;
+        use_graphs = cands[bi].graphs != 0;
+        if (best) {
+            *best = cands[bi];
+            best->ns_per_particle_substep = us[bi];
+        }
+        return EMH_OK;
+    }
+
     // The caller index of the particle in each device slot (the stored order).
     int slot_order(uint32_t* out) {
         if (host_dirty) return fail(EMH_ERR_STATE, "the particles are not on the device yet: step first");
@@ -905,6 +1048,27 @@ struct emh_sim {
     }
 
     int host_current() { return download(); }
+
+    // `slots` back-to-back substep launches from launch number seq, replayed
+    // from a captured graph in graph mode.
+    int launch_substep_run(uint32_t slots) {
+        if (!use_graphs) {
+            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
+            return EMH_OK;
+        }
+        graphs.check_arg(&pipe, sizeof(Pipeline));
+        const uint32_t r = (uint32_t)(seq % 6u), variant = sg_mode | (sleep ? 4u : 0u);
+        hipGraphExec_t ex = graphs.find(r, slots, variant);
+        if (!ex) {
+            const uint64_t s0 = seq;
+            HIP_TRY(graphs.capture(
+                stream, r, slots, variant,
+                [&](uint32_t k) { return launch_substep(pipe, s0 + k, stream, sg_mode, sleep); }, &ex));
+        }
+        HIP_TRY(hipGraphLaunch(ex, stream));
+        seq += slots;
+        return EMH_OK;
+    }
 
     // Running per the loop condition of step.rs:34.
     bool running(const CtrlState& st) const {
@@ -1027,7 +1191,10 @@ struct emh_sim {
             }
             uint32_t slots = std::max<uint32_t>(predicted_substeps, 1u);
             slots = std::min<uint32_t>(slots, cfg.max_substeps_per_step);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_substep(pipe, seq++, stream, sg_mode, sleep));
+            {
+                int rc = launch_substep_run(slots);
+                if (rc) return rc;
+            }
             if (profiling) prof_count[0] += 1;
             substeps_since_reorder += slots;
             if ((s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
@@ -1767,6 +1934,29 @@ int emh_sim_set_reorder_interval(emh_sim* s, uint32_t substeps) {
     return EMH_OK;
 }
 
+int emh_sim_get_tuning(emh_sim* s, emh_tuning* out) {
+    if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *out = emh_tuning{};
+    out->reorder_interval = s->reorder_substeps;
+    out->graphs = s->use_graphs ? 1u : 0u;
+    return EMH_OK;
+}
+
+int emh_sim_set_tuning(emh_sim* s, const emh_tuning* t) {
+    if (!s || !t) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (t->reorder_interval == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
+    if (t->graphs > 1) return fail(EMH_ERR_INVALID_ARG, "graphs must be 0 or 1");
+    s->reorder_substeps = t->reorder_interval;
+    s->use_graphs = t->graphs != 0;
+    return EMH_OK;
+}
+
+int emh_sim_autotune(emh_sim* s, uint32_t trial_steps, emh_tuning* chosen, emh_tuning* timed, uint32_t timed_cap,
+                     uint32_t* n_timed) {
+    if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
+    return s->autotune(trial_steps, chosen, timed, timed_cap, n_timed);
+}
+
 int emh_sim_set_profiling(emh_sim* s, int enable) {
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     s->profiling = enable != 0;
@@ -2465,13 +2655,19 @@ struct emh_batch {
     std::vector<Ctrl> h_ctrl;
     BatchLists L{};
     uint64_t seq = 0;
+    uint32_t reorder_substeps = kBatchReorderSubsteps;  // emh_batch_set_tuning
     uint32_t substeps_since_reorder = kBatchReorderSubsteps;
     uint32_t predicted = 1;
     uint32_t max_substeps = 1;
     std::vector<uint64_t> epochs;  // each scene's cfg_epoch in d_pipes
+    // graph replay of runs of batch substep launches (the work lists are the
+    // launches' arguments; the scenes' pipelines are read from d_pipes)
+    bool use_graphs = false;
+    GraphCache graphs;
 
     ~emh_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
+        graphs.clear();
         if (d_pipes) (void)hipFree(d_pipes);
         if (d_starts) (void)hipFree(d_starts);
         if (d_ctrl) (void)hipFree(d_ctrl);
@@ -2533,7 +2729,7 @@ struct emh_batch {
         L.fused_blocks = fused[n];
         L.density_blocks = dens[n];
         seq = 1;
-        substeps_since_reorder = kBatchReorderSubsteps;
+        substeps_since_reorder = reorder_substeps;
         return EMH_OK;
     }
 
@@ -2587,6 +2783,72 @@ struct emh_batch {
         return EMH_OK;
     }
 
+    // The batch counterpart of emh_sim::autotune: every scene is put back to
+    // its saved state between candidates and after the last.
+    int autotune(uint32_t trial, emh_tuning* best, emh_tuning* timed, uint32_t cap, uint32_t* n_timed) {
+        if (trial == 0) return fail(EMH_ERR_INVALID_ARG, "trial_steps must be at least 1");
+        const size_t n = sims.size();
+        std::vector<std::vector<emh_particle>> saved(n);
+        std::vector<emh_step_stats> saved_stats(n);
+        std::vector<uint32_t> saved_pred(n);
+        for (size_t i = 0; i < n; ++i) {
+            int rc = sims[i]->host_current();
+            if (rc) return rc;
+            saved[i] = sims[i]->host;
+            saved_stats[i] = sims[i]->stats;
+            saved_pred[i] = sims[i]->predicted_substeps;
+        }
+        const uint32_t saved_interval = reorder_substeps;
+        const bool saved_graphs = use_graphs;
+        auto restore = [&]() {
+            for (size_t i = 0; i < n; ++i) {
+                sims[i]->host = saved[i];
+                sims[i]->host_dirty = true;
+                sims[i]->device_ahead = false;
+                sims[i]->stats = saved_stats[i];
+                sims[i]->predicted_substeps = saved_pred[i];
+            }
+        };
+        const std::vector<emh_tuning> cands = batch_tuning_candidates();
+        std::vector<float> cost(cands.size(), 0.0f);
+        int rc = EMH_OK;
+        for (size_t c = 0; c < cands.size() && rc == EMH_OK; ++c) {
+            restore();
+            reorder_substeps = cands[c].reorder_interval;
+            use_graphs = cands[c].graphs != 0;
+            rc = step(1);
+            if (rc) break;
+            std::vector<uint64_t> s0(n);
+            for (size_t i = 0; i < n; ++i) s0[i] = sims[i]->stats.total_substeps;
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = step(trial);
+            const auto t1 = std::chrono::steady_clock::now();
+            double work = 0.0;
+            for (size_t i = 0; i < n; ++i)
+                work += (double)(sims[i]->stats.total_substeps - s0[i]) * (double)sims[i]->pipe.n;
+            cost[c] = (float)(std::chrono::duration<double, std::nano>(t1 - t0).count() / std::max(work, 1.0));
+        }
+        restore();
+        reorder_substeps = saved_interval;
+        use_graphs = saved_graphs;
+        if (rc) return rc;
+        size_t bi = 0;
+        for (size_t c = 1; c < cands.size(); ++c)
+            if (cost[c] < cost[bi]) bi = c;
+        for (size_t c = 0; c < cands.size() && c < cap && timed; ++c) {
+            timed[c] = cands[c];
+            timed[c].ns_per_particle_substep = cost[c];
+        }
+        if (n_timed) *n_timed = (uint32_t)cands.size();
+        reorder_substeps = cands[bi].reorder_interval;
+        use_graphs = cands[bi].graphs != 0;
+        if (best) {
+            *best = cands[bi];
+            best->ns_per_particle_substep = cost[bi];
+        }
+        return EMH_OK;
+    }
+
     int step(uint32_t n_steps) {
         if (!in_sync()) {
             int rc = sync();
@@ -2596,11 +2858,24 @@ struct emh_batch {
         if (rc) return rc;
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
-            const bool reorder = substeps_since_reorder >= kBatchReorderSubsteps;
+            const bool reorder = substeps_since_reorder >= reorder_substeps;
             HIP_TRY(launch_batch_begin_step(L, seq++, reorder, stream));
             if (reorder) substeps_since_reorder = 0;
             const uint32_t slots = std::min<uint32_t>(std::max<uint32_t>(predicted, 1u), max_substeps);
-            for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_batch_substep(L, seq++, stream));
+            if (use_graphs) {
+                graphs.check_arg(&L, sizeof(L));
+                const uint32_t r = (uint32_t)(seq % 6u);
+                hipGraphExec_t ex = graphs.find(r, slots, 0u);
+                if (!ex) {
+                    const uint64_t s0 = seq;
+                    HIP_TRY(graphs.capture(stream, r, slots, 0u,
+                                           [&](uint32_t k) { return launch_batch_substep(L, s0 + k, stream); }, &ex));
+                }
+                HIP_TRY(hipGraphLaunch(ex, stream));
+                seq += slots;
+            } else {
+                for (uint32_t k = 0; k < slots; ++k) HIP_TRY(launch_batch_substep(L, seq++, stream));
+            }
             substeps_since_reorder += slots;
             if ((s + 1) % kDrainSteps == 0 || s + 1 == n_steps) {
                 int rc = drain();
@@ -2660,6 +2935,29 @@ int emh_batch_destroy(emh_batch* b) {
 int emh_batch_step(emh_batch* b, uint32_t n_steps) {
     if (!b) return fail(EMH_ERR_INVALID_ARG, "null batch");
     return b->step(n_steps);
+}
+
+int emh_batch_get_tuning(emh_batch* b, emh_tuning* out) {
+    if (!b || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    *out = emh_tuning{};
+    out->reorder_interval = b->reorder_substeps;
+    out->graphs = b->use_graphs ? 1u : 0u;
+    return EMH_OK;
+}
+
+int emh_batch_set_tuning(emh_batch* b, const emh_tuning* t) {
+    if (!b || !t) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    if (t->reorder_interval == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
+    if (t->graphs > 1) return fail(EMH_ERR_INVALID_ARG, "graphs must be 0 or 1");
+    b->reorder_substeps = t->reorder_interval;
+    b->use_graphs = t->graphs != 0;
+    return EMH_OK;
+}
+
+int emh_batch_autotune(emh_batch* b, uint32_t trial_steps, emh_tuning* chosen, emh_tuning* timed, uint32_t timed_cap,
+                       uint32_t* n_timed) {
+    if (!b) return fail(EMH_ERR_INVALID_ARG, "null batch");
+    return b->autotune(trial_steps, chosen, timed, timed_cap, n_timed);
 }
 
 }  // extern "C"

@@ -112,6 +112,11 @@ class CKernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
 
 
+class CTuning(ctypes.Structure):
+    _fields_ = [("reorder_interval", ctypes.c_uint32), ("graphs", ctypes.c_uint32),
+                ("ns_per_particle_substep", ctypes.c_float), ("pad", ctypes.c_uint32)]
+
+
 class CStepStats(ctypes.Structure):
     _fields_ = [("last_substeps", ctypes.c_uint32), ("last_step_dt", ctypes.c_float),
                 ("last_sim_time_dropped", ctypes.c_float), ("last_vel_clamp_count", ctypes.c_uint32),
@@ -282,6 +287,14 @@ def load_library() -> ctypes.CDLL:
         "emh_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(ctypes.c_void_p)]),
         "emh_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
         "emh_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+        "emh_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
+        "emh_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
+        "emh_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CTuning), P(CTuning),
+                                              ctypes.c_uint32, P(ctypes.c_uint32)]),
+        "emh_sim_get_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
+        "emh_sim_set_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
+        "emh_sim_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CTuning), P(CTuning),
+                                            ctypes.c_uint32, P(ctypes.c_uint32)]),
         "emh_cell_key": (ctypes.c_uint32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32]),
         "emh_sort_cell_keys": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p,
                                               ctypes.c_void_p]),
@@ -754,8 +767,61 @@ class StepStats:
     frame_index: int
 
 
-class Simulation:
+def _tuning_dict(t: CTuning) -> dict:
+    return {"reorder_interval": int(t.reorder_interval), "graphs": int(t.graphs),
+            "ns_per_particle_substep": float(t.ns_per_particle_substep)}
+
+
+class _Tunable:
+    """Launch tuning (reorder interval, hipGraph replay) and the autotuner
+    (emh_sim_* / emh_batch_* tuning entry points)."""
+    _tune_prefix = ""
+
+    def _fn(self, name):
+        return getattr(load_library(), f"emh_{self._tune_prefix}_{name}")
+
+    def tuning(self) -> dict:
+        t = CTuning()
+        _check(self._fn("get_tuning")(self._h, ctypes.byref(t)))
+        return _tuning_dict(t)
+
+    def set_tuning(self, reorder_interval: int, graphs: bool) -> None:
+        t = CTuning(int(reorder_interval), 1 if graphs else 0, 0.0, 0)
+        _check(self._fn("set_tuning")(self._h, ctypes.byref(t)))
+
+    def autotune(self, trial_steps: int = 10) -> Tuple[dict, list]:
+        """Time every candidate setting on this scene (state put back afterwards), apply
+        the fastest; returns (chosen, [every candidate with its cost])."""
+        best = CTuning()
+        timed = (CTuning * 64)()
+        n = ctypes.c_uint32()
+        _check(self._fn("autotune")(self._h, int(trial_steps), ctypes.byref(best), timed, 64, ctypes.byref(n)))
+        return _tuning_dict(best), [_tuning_dict(timed[i]) for i in range(min(n.value, 64))]
+
+    def autotune_cached(self, key: str, cache_path: str, trial_steps: int = 10) -> dict:
+        """autotune() once per `key` (a workload signature), the choice kept in the JSON
+        file `cache_path`; later calls with the same key apply the cached choice."""
+        import json
+        cache = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        if key in cache:
+            c = cache[key]
+            self.set_tuning(c["reorder_interval"], bool(c["graphs"]))
+            return dict(c, cached=True)
+        best, timed = self.autotune(trial_steps)
+        cache[key] = dict(best, candidates=timed)
+        tmp = cache_path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(tmp, cache_path)
+        return dict(best, candidates=timed, cached=False)
+
+
+class Simulation(_Tunable):
     """Simulation (spacetime/solver/mod.rs:37-75) on one GPU."""
+    _tune_prefix = "sim"
 
     def __init__(self, handle: ctypes.c_void_p, config: SimConfig):
         self._h = handle
@@ -1078,9 +1144,10 @@ def nacc_return_batch(material: "NaccMaterial", f_ptr: int, alpha_ptr: int, n: i
 
 
 # -------------------------------------------------------------- batched scenes
-class SceneBatch:
+class SceneBatch(_Tunable):
     """Many independent simulations stepped together (one launch per stage for
     all of them); each keeps its own adaptive substeps."""
+    _tune_prefix = "batch"
 
     def __init__(self, sims: Sequence["Simulation"]):
         self._sims = list(sims)  # keeps them alive while batched

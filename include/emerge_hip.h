@@ -443,6 +443,33 @@ int emh_sim_set_reorder_interval(emh_sim* sim, uint32_t substeps);
 int emh_sim_get_slot_order(emh_sim* sim, uint32_t* ids_out, size_t n);
 int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
 
+/* ---- launch tuning and the autotuner --------------------------------------
+ * No counterpart in the reference (its WGSL backend submits up to 64
+ * substeps per frame in one command buffer, gpu/solver/step.rs:404-441).
+ * Two run-time settings change how a step is launched, never its results
+ * beyond f32 summation order:
+ *   reorder_interval  substeps between physical cell-key reorders (>= 1);
+ *   graphs            1: each step's run of substep launches is captured
+ *                     once into a hipGraph (per launch-number residue mod 6,
+ *                     run length and kernel variant) and replayed; 0: the
+ *                     launches are queued one by one.
+ * emh_*_autotune times every candidate (reorder intervals x graphs) on the
+ * scene itself: from the current state, one untimed step, then trial_steps
+ * steps timed on the host clock to their completion; it then puts the state
+ * back (the particles are re-uploaded at the next step, the step counters
+ * restored), applies the fastest candidate and reports it in *chosen (and
+ * every candidate's cost in timed[0 .. min(cap, *n_timed))). */
+typedef struct emh_tuning {
+    uint32_t reorder_interval;
+    uint32_t graphs;
+    float    ns_per_particle_substep; /* set by the autotuners: wall time / (particles x substeps) */
+    uint32_t pad;
+} emh_tuning;
+int emh_sim_get_tuning(emh_sim* sim, emh_tuning* out);
+int emh_sim_set_tuning(emh_sim* sim, const emh_tuning* t);
+int emh_sim_autotune(emh_sim* sim, uint32_t trial_steps, emh_tuning* chosen, emh_tuning* timed,
+                     uint32_t timed_cap, uint32_t* n_timed);
+
 /* ---- the cell-key sort on its own ---------------------------------------
  * No CPU counterpart in the reference (its WGSL sort is a coarse 16x16-block
  * counting sort with a non-deterministic order inside a block,
@@ -494,6 +521,12 @@ typedef struct emh_batch emh_batch;
 int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out);
 int emh_batch_destroy(emh_batch* b);
 int emh_batch_step(emh_batch* b, uint32_t n_steps);
+/* Launch tuning of the batch (see emh_sim_autotune; the batch's default
+ * reorder interval is 8 substeps) and its autotuner. */
+int emh_batch_get_tuning(emh_batch* b, emh_tuning* out);
+int emh_batch_set_tuning(emh_batch* b, const emh_tuning* t);
+int emh_batch_autotune(emh_batch* b, uint32_t trial_steps, emh_tuning* chosen, emh_tuning* timed,
+                       uint32_t timed_cap, uint32_t* n_timed);
 
 /* ---------------------------------------------------------------------
  * Learned NACC return-mapping surrogate.

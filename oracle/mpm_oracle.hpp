@@ -70,6 +70,11 @@ enum Mutation : int {
 inline int g_mutation = kMutNone;
 inline bool mutated(int m) { return g_mutation == m; }
 
+// Threads of G2P phase 1, the one loop the reference runs in parallel (rayon's
+// par_iter over particles, transfer/g2p.rs:189-196).  Each particle reads the
+// grid and writes only its own state, so the result does not depend on it.
+inline int g_threads = 1;
+
 // ---------------------------------------------------------------- glam mirror
 template <class T> struct V2 {
     T x, y;
@@ -1676,7 +1681,9 @@ template <class T> struct Simulation {
         T apic = T(cfg.apic_blend);
         uint32_t R = cfg.grid_res;
         uint32_t clamped = 0;
-        for (size_t i = 0; i < active_count; ++i) {  // phase 1
+        const long n_active = (long)active_count;
+#pragma omp parallel for schedule(static) reduction(+ : clamped) num_threads(g_threads) if (g_threads > 1)
+        for (long i = 0; i < n_active; ++i) {  // phase 1
             V2<T> x = P.x[i];
             Weights<T> w = quadratic_weights(x);
             V2<T> new_v{T(0), T(0)};

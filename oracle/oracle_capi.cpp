@@ -364,6 +364,9 @@ int orc_project_particle(void* hv, size_t i) {
 // Planted mistakes (Mutation in mpm_oracle.hpp) for the oracle's own tests.
 void orc_set_mutation(int m) { g_mutation = m; }
 
+// Threads of G2P phase 1 (the reference's rayon loop); 1 = serial.
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
 // Seconds of wall time for `n` full steps (the CPU baseline timer).
 double orc_time_steps(void* hv, uint32_t n) {
     auto t0 = std::chrono::steady_clock::now();

@@ -83,6 +83,7 @@ def load() -> ctypes.CDLL:
         "orc_choose_substep_dt": (ctypes.c_double, [ctypes.c_void_p, f]),
         "orc_project_particle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
         "orc_set_mutation": (None, [ctypes.c_int]),
+        "orc_set_threads": (None, [ctypes.c_int]),
         "orc_time_steps": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_uint32]),
         "orc_time_substeps": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_uint32, f]),
         "orc_quadratic_weights": (None, [f, f, P(ctypes.c_int32), P(f), P(f)]),
@@ -402,6 +403,11 @@ def material(kind: str, *args) -> Material:
     p, e = CMaterialParams(), CMaterialExt()
     getattr(load(), f"orc_material_{kind}")(*args, ctypes.byref(p), ctypes.byref(e))
     return Material(p, e)
+
+
+def set_threads(n: int) -> None:
+    """Threads of G2P phase 1 (the reference's rayon loop, transfer/g2p.rs:189-196)."""
+    load().orc_set_threads(int(n))
 
 
 # Planted mistakes of the oracle (mpm_oracle.hpp Mutation), for its own tests.

@@ -1,0 +1,127 @@
+"""Launch tuning: hipGraph replay of each step's substep launches and the
+autotuner.  Neither may change what a step computes: a scene stepped with
+graph replay takes the substep schedule of the same scene stepped with plain
+launches, and its state agrees to the f32 summation-order noise of the grid
+atomics, after every step (each step replays captured graphs, with the
+library's own synchronisation and eager launches in between).  The
+autotuner puts the state back after its trials."""
+import numpy as np
+import pytest
+
+import emerge_hip as eh
+import scenes as S
+
+
+def _match(a, b, tol_x=2e-5, tol_f=1e-3):
+    assert b.stats().total_substeps == a.stats().total_substeps
+    assert b.stats().last_substeps == a.stats().last_substeps
+    e = S.rel_errors(S.state_matrix(b.particles()), S.state_matrix(a.particles()))
+    assert e["x"] < tol_x and e["F"] < tol_f, e
+    return e
+
+
+# two runs of a scene differ by the order of their float atomics; NACC's return
+# mapping amplifies that most (one step spreads C by 7e-4 .. 1.4e-2 over
+# particle orders on the CPU oracle alone)
+TOL = {"nacc_block": (1e-4, 1e-2)}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["sand_column", "multi_material", "nacc_block", "terrain_slide"])
+def test_graph_replay_matches_plain_launches(name):
+    plain, graphed = S.SCENES[name].build("gpu"), S.SCENES[name].build("gpu")
+    graphed.set_tuning(plain.tuning()["reorder_interval"], True)
+    assert graphed.tuning()["graphs"] == 1 and plain.tuning()["graphs"] == 0
+    tol_x, tol_f = TOL.get(name, (2e-5, 1e-3))
+    for step in range(12):
+        plain.step_n(1)
+        graphed.step_n(1)
+        _match(plain, graphed, tol_x, tol_f)
+        if step == 5:  # the pipeline argument changes: cached graphs must be recaptured
+            for s in (plain, graphed):
+                s.set_gravity((1.0, -20.0))
+    plain.step_n(7)  # several steps per call: runs replayed back to back
+    graphed.step_n(7)
+    _match(plain, graphed, tol_x, tol_f)
+    plain.close()
+    graphed.close()
+
+
+@pytest.mark.gpu
+def test_graph_replay_in_batch():
+    names = ["quickstart", "sand_column", "snow_drop", "multi_material"]
+    a = [S.SCENES[n].build("gpu") for n in names]
+    b = [S.SCENES[n].build("gpu") for n in names]
+    ba, bb = eh.SceneBatch(a), eh.SceneBatch(b)
+    bb.set_tuning(ba.tuning()["reorder_interval"], True)
+    for _ in range(4):
+        ba.step_n(1)
+        bb.step_n(1)
+        for x, y in zip(a, b):
+            _match(x, y)
+    ba.step_n(5)
+    bb.step_n(5)
+    for x, y in zip(a, b):
+        _match(x, y)
+    ba.close()
+    bb.close()
+    for s in a + b:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_autotune_restores_state_and_applies_choice():
+    ref, tuned = S.sand_column("gpu"), S.sand_column("gpu")
+    ref.step_n(3)
+    tuned.step_n(3)
+    before = tuned.particles()
+    best, timed = tuned.autotune(trial_steps=3)
+    print("chosen", best, "\n", "\n".join(str(t) for t in timed))
+    assert len(timed) == 8
+    assert all(t["ns_per_particle_substep"] > 0 for t in timed)
+    assert best["ns_per_particle_substep"] == min(t["ns_per_particle_substep"] for t in timed)
+    got = tuned.tuning()
+    assert got["reorder_interval"] == best["reorder_interval"] and got["graphs"] == best["graphs"]
+    assert tuned.stats().total_substeps == ref.stats().total_substeps
+    assert np.array_equal(tuned.particles(), before)  # the state it was tuned from
+    ref.step_n(4)
+    tuned.step_n(4)
+    _match(ref, tuned, tol_x=1e-4)
+    ref.close()
+    tuned.close()
+
+
+@pytest.mark.gpu
+def test_batch_autotune_and_cache(tmp_path):
+    names = ["corotated_drop", "snow_drop", "sand_column"]
+    a = [S.SCENES[n].build("gpu") for n in names]
+    b = [S.SCENES[n].build("gpu") for n in names]
+    ba, bb = eh.SceneBatch(a), eh.SceneBatch(b)
+    ba.step_n(2)
+    bb.step_n(2)
+    cache = str(tmp_path / "tuning.json")
+    first = bb.autotune_cached("three_scenes", cache, trial_steps=2)
+    assert not first["cached"] and len(first["candidates"]) == 8
+    second = bb.autotune_cached("three_scenes", cache, trial_steps=2)
+    assert second["cached"] and second["reorder_interval"] == first["reorder_interval"]
+    assert bb.tuning()["reorder_interval"] == first["reorder_interval"]
+    for x, y in zip(a, b):
+        assert x.stats().total_substeps == y.stats().total_substeps
+    ba.step_n(3)
+    bb.step_n(3)
+    for x, y in zip(a, b):
+        _match(x, y, tol_x=1e-4)
+    ba.close()
+    bb.close()
+    for s in a + b:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_tuning_errors():
+    sim = S.sand_column("gpu")
+    with pytest.raises(eh.EmergeError):
+        sim.set_tuning(0, False)
+    with pytest.raises(eh.EmergeError):
+        sim.autotune(trial_steps=0)
+    sim.close()

@@ -390,3 +390,21 @@ def test_nacc_case_b_tip(oracle):
     assert fo[0, 0] == pytest.approx(fo[0, 3], rel=1e-14)
     assert _p_of_j(jn) == pytest.approx(-BETA * p0, rel=1e-9)
     assert a[0] == pytest.approx(alpha + math.log(j / jn), rel=1e-12)
+
+
+# ------------------------------------------------- threaded G2P (CPU baseline)
+def test_parallel_g2p_phase1_is_bit_identical(oracle):
+    """G2P phase 1 on several threads (the reference's rayon loop, g2p.rs:189-196) gives
+    the serial result bit for bit: each particle reads the grid and writes only itself."""
+    import scenes as S
+    runs = []
+    for threads in (1, 8):
+        oracle.set_threads(threads)
+        try:
+            sim = S.multi_material("oracle32")
+            sim.step_n(5)
+            runs.append(sim.particles())
+            sim.close()
+        finally:
+            oracle.set_threads(1)
+    assert runs[0].tobytes() == runs[1].tobytes()
