@@ -138,6 +138,79 @@ def full_run_section(args, tuning) -> dict:
             "note": "includes the first step's upload, the physical reorders and the whole collapse"}
 
 
+def build_multi_material(segments: bool = True):
+    """Four-material scene at its stated size: 1024^2 grid, 370 seeded boxes with
+    sides U[8, 96] at 4 particles per cell (3.98M particles; the stated 40 boxes
+    hold only ~0.54M), materials in turn NeoHookean (400, 200), water (1000, 1e4),
+    Stomakhin snow (E = 1.4e5, nu = 0.2) and corotated (400, 200), initial
+    velocity scale 0.5, gravity (0, -9.81), step dt 0.02.  The water makes every
+    substep run the density re-estimate."""
+    cfg = eh.SimConfig.standard(1024, 0.02, (0.0, -9.81))
+    lam, mu = eh.lame_from_young(1.4e5, 0.2)
+    sim = (eh.Simulation.empty(cfg)
+           .with_default_material(eh.NeoHookeanMaterial.new(400.0, 200.0))
+           .with_material(1, eh.NewtonianFluidMaterial.low_viscosity(1000.0, 1e4))
+           .with_material(2, eh.StomakhinMaterial.new(lam, mu, 10.0, 0.025, 0.0075, 0.6, 20.0))
+           .with_material(3, eh.CorotatedMaterial.new(400.0, 200.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.set_tuning(sim.tuning()["reorder_interval"], False, segments)
+    rng = np.random.default_rng(3)
+    for k in range(370):
+        w, h = (int(v) for v in rng.integers(8, 97, size=2))
+        cx = float(rng.uniform(4 + w / 2 + 1, 1020 - w / 2 - 1))
+        cy = float(rng.uniform(4 + h / 2 + 1, 1020 - h / 2 - 1))
+        sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(w, h), box_center=(cx, cy), material_id=k % 4,
+                                    rng_seed=1000 + k, initial_velocity_scale=0.5, position_jitter=0.1))
+    return sim
+
+
+# Per particle-substep for the NeoHookean / corotated / water / snow mix: the
+# sand figure's fields minus friction_hardening and log_volume_strain (128 B).
+MIXED_BYTES = 128
+
+
+def multi_material_section(args) -> dict:
+    """The four-material scene in the default cell-key order against per-material
+    segments (the reorder groups particles by model, so each wave runs one
+    model's update), each from its upload, same steps; with the share of waves
+    that hold more than one model, where the update's per-model switch diverges."""
+    out = {"workload": "multi_material_4m_1024", "bytes_per_particle_substep": MIXED_BYTES}
+    for name, segments in (("cell_key_order", False), ("segments", True)):
+        sim = build_multi_material(segments)
+        n = sim.num_particles()
+        sim.step_n(2)
+        sim.synchronize()
+        # waves (64 consecutive device slots) holding more than one model after the upload's reorder
+        mid = sim.particles()["material_id"][sim.slot_order()]
+        nw = len(mid) // 64
+        w = mid[:nw * 64].reshape(nw, 64)
+        mixed_waves = float(np.mean(np.any(w != w[:, :1], axis=1)))
+        sim.set_profiling(True)
+        s0 = sim.stats().total_substeps
+        t0 = time.perf_counter()
+        sim.step_n(args.mixed_steps)
+        sim.synchronize()
+        elapsed = time.perf_counter() - t0
+        subs = sim.stats().total_substeps - s0
+        times = {k: v for k, v in sim.kernel_times().items() if v[0] > 0}
+        sim.set_profiling(False)
+        key = max((k for k in times if k != "step_setup"), key=lambda k: times[k][1])
+        launches, total_ms = times[key]
+        avg_s = total_ms / launches / 1e3
+        p = sim.particles()
+        sim.close()
+        out["particles"] = n
+        out[name] = {"particle_substeps_per_s": n * subs / elapsed, "substeps": subs, "seconds": elapsed,
+                     "kernels": key, "avg_substep_us": avg_s * 1e6,
+                     "roofline_frac": MIXED_BYTES * n / avg_s / 1e9 / HBM_PEAK_GBS,
+                     "mixed_model_waves": mixed_waves,
+                     "state_finite": bool(np.all(np.isfinite(p["x"])))}
+    out["steps"] = args.mixed_steps
+    out["segments_speedup"] = (out["segments"]["particle_substeps_per_s"] /
+                               out["cell_key_order"]["particle_substeps_per_s"])
+    return out
+
+
 # ------------------------------------------------------------ NACC surrogate
 # Algorithmic MFMA work of one network evaluation: 2 * (6*64 + 64*64 + 64*64 + 64*3).
 SURROGATE_FLOPS = 2 * (6 * 64 + 64 * 64 + 64 * 64 + 64 * 3)
@@ -430,6 +503,8 @@ def main() -> int:
     ap.add_argument("--tune-steps", type=int, default=10, help="timed steps per autotuner candidate")
     ap.add_argument("--full-run-steps", type=int, default=500,
                     help="the whole collapse from rest, timed as one run (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=20,
+                    help="timed steps of the four-material 1024^2 scene, per ordering (0: skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes section")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -524,6 +599,7 @@ def main() -> int:
         out["cpu_baseline"] = None
     sim.close()
     out["full_run"] = None if args.full_run_steps <= 0 else full_run_section(args, tuning)
+    out["multi_material"] = None if args.mixed_steps <= 0 else multi_material_section(args)
     out["nacc_surrogate"] = None if args.no_surrogate else surrogate_section(args)
     out["batched_scenes"] = None if args.no_batch else batch_section(args)
     print(json.dumps(out), flush=True)

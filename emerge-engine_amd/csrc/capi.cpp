@@ -338,25 +338,28 @@ template <class T> T* dalloc(size_t n) {
 }  // namespace
 
 // ------------------------------------------------------------------ emh_sim
-// The settings the autotuners time: reorder intervals (substeps) x graph replay.
-static std::vector<emh_tuning> tuning_candidates(const uint32_t* intervals, size_t n) {
+// The settings the autotuners time: reorder intervals (substeps) x graph
+// replay, x per-material segments on and off when some scene mixes models.
+static std::vector<emh_tuning> tuning_candidates(const uint32_t* intervals, size_t n, bool mixed) {
     std::vector<emh_tuning> v;
-    for (uint32_t g = 0; g < 2; ++g)
-        for (size_t i = 0; i < n; ++i) {
-            emh_tuning t{};
-            t.reorder_interval = intervals[i];
-            t.graphs = g;
-            v.push_back(t);
-        }
+    for (uint32_t seg = mixed ? 0u : 1u; seg < 2; ++seg)
+        for (uint32_t g = 0; g < 2; ++g)
+            for (size_t i = 0; i < n; ++i) {
+                emh_tuning t{};
+                t.reorder_interval = intervals[i];
+                t.graphs = g;
+                t.model_segments = seg;
+                v.push_back(t);
+            }
     return v;
 }
-static std::vector<emh_tuning> tuning_candidates() {
+static std::vector<emh_tuning> tuning_candidates(bool mixed) {
     static const uint32_t kIntervals[] = {8u, 64u, 512u, kReorderSubsteps};
-    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]));
+    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]), mixed);
 }
-static std::vector<emh_tuning> batch_tuning_candidates() {
+static std::vector<emh_tuning> batch_tuning_candidates(bool mixed) {
     static const uint32_t kIntervals[] = {4u, kBatchReorderSubsteps, 32u, 256u};
-    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]));
+    return tuning_candidates(kIntervals, sizeof(kIntervals) / sizeof(kIntervals[0]), mixed);
 }
 
 // Instantiated graphs of runs of substep launches, keyed by (launch number
@@ -441,6 +444,7 @@ struct emh_sim {
     uint32_t sg_mode = 0;
     emh_surrogate* sg_attached = nullptr;  // the surrogate of modes 1 and 2
     uint64_t cfg_epoch = 0;                // bumped by in-place edits of pipe.cfg (set_gravity)
+    bool model_segments = false;  // emh_tuning::model_segments
     // hipGraph replay of a step's substep launches (emh_sim_set_graphs): the
     // kernels read their launch number only modulo 6 (control-state parity
     // and reduction slot), so a run of `slots` launches captured at residue r
@@ -613,7 +617,7 @@ struct emh_sim {
         pipe.sort.vbuf[0] = (uint32_t*)take(sizeof(uint32_t) * c);
         pipe.sort.vbuf[1] = (uint32_t*)take(sizeof(uint32_t) * c);
         pipe.sort.out = (uint32_t*)take(sizeof(uint32_t) * c);
-        pipe.sort.hist = (uint32_t*)take(sizeof(uint32_t) * sort_hist_len(c, n_bins()));
+        pipe.sort.hist = (uint32_t*)take(sizeof(uint32_t) * sort_hist_len_max(c));
         pipe.sort.perm = 0u;
         for (void* p : particle_allocs)
             if (!p) {
@@ -708,6 +712,10 @@ struct emh_sim {
         }
         d.fields = fields;
         d.only_model = only;
+        // per-material segments (a tuning option): a scene of several models is
+        // reordered by (model, cell key), so a chunk -- and each wave -- runs
+        // one model
+        pipe.sort.model_bits = (only == kModelMixed && model_segments) ? kModelKeyBits : 0u;
     }
 
     // Terrain heights of every heightmap boundary, concatenated in boundary order.
@@ -892,10 +900,13 @@ struct emh_sim {
         if (rc) return rc;
         const std::vector<emh_particle> saved = host;
         const emh_step_stats saved_stats = stats;
-        const uint32_t saved_interval = reorder_substeps, saved_since = substeps_since_reorder,
-                       saved_pred = predicted_substeps;
-        const bool saved_graphs = use_graphs;
-        const std::vector<emh_tuning> cands = tuning_candidates();
+        const uint32_t saved_since = substeps_since_reorder, saved_pred = predicted_substeps;
+        const emh_tuning saved_tuning = tuning();
+        if (host_dirty) {  // the material mix (mixed_models) is known after an upload
+            rc = upload();
+            if (rc) return rc;
+        }
+        const std::vector<emh_tuning> cands = tuning_candidates(mixed_models());
         std::vector<float> us(cands.size(), 0.0f);
         auto restore = [&]() {
             host = saved;
@@ -907,8 +918,7 @@ struct emh_sim {
         };
         for (size_t c = 0; c < cands.size() && rc == EMH_OK; ++c) {
             restore();
-            reorder_substeps = cands[c].reorder_interval;
-            use_graphs = cands[c].graphs != 0;
+            apply_tuning(cands[c]);
             rc = step(1);
             if (rc) break;
             const uint64_t s0 = stats.total_substeps;
@@ -919,8 +929,7 @@ struct emh_sim {
             us[c] = (float)(std::chrono::duration<double, std::nano>(t1 - t0).count() / work);
         }
         restore();
-        reorder_substeps = saved_interval;
-        use_graphs = saved_graphs;
+        apply_tuning(saved_tuning);
         if (rc) return rc;
         size_t bi = 0;
         for (size_t c = 1; c < cands.size(); ++c)
@@ -930,14 +939,31 @@ struct emh_sim {
             timed[c].ns_per_particle_substep = us[c];
         }
         if (n_timed) *n_timed = (uint32_t)cands.size();
-        reorder_substeps = cands[bi].reorder_interval;
-        use_graphs = cands[bi].graphs != 0;
+        apply_tuning(cands[bi]);
         if (best) {
             *best = cands[bi];
             best->ns_per_particle_substep = us[bi];
         }
         return EMH_OK;
     }
+
+    // Launch settings (emh_tuning); the model grouping of the reorder key
+    // takes effect at the next reorder.
+    void apply_tuning(const emh_tuning& t) {
+        reorder_substeps = t.reorder_interval;
+        use_graphs = t.graphs != 0;
+        model_segments = t.model_segments != 0;
+        pipe.sort.model_bits = (pipe.cfg.only_model == kModelMixed && model_segments) ? kModelKeyBits : 0u;
+        ++cfg_epoch;
+    }
+    emh_tuning tuning() const {
+        emh_tuning t{};
+        t.reorder_interval = reorder_substeps;
+        t.graphs = use_graphs ? 1u : 0u;
+        t.model_segments = model_segments ? 1u : 0u;
+        return t;
+    }
+    bool mixed_models() const { return pipe.cfg.only_model == kModelMixed; }
 
     // The caller index of the particle in each device slot (the stored order).
     int slot_order(uint32_t* out) {
@@ -1927,27 +1953,31 @@ int emh_sim_synchronize(emh_sim* s) {
     return EMH_OK;
 }
 
-int emh_sim_set_reorder_interval(emh_sim* s, uint32_t substeps) {
+int emh_sim_set_reorder_interval(emh_sim* s, uint32_t substeps) {  // the tuning's reorder_interval alone
     if (!s) return fail(EMH_ERR_INVALID_ARG, "null sim");
     if (substeps == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
     s->reorder_substeps = substeps;
     return EMH_OK;
 }
 
+static int check_tuning(const emh_tuning* t) {
+    if (t->reorder_interval == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
+    if (t->graphs > 1) return fail(EMH_ERR_INVALID_ARG, "graphs must be 0 or 1");
+    if (t->model_segments > 1) return fail(EMH_ERR_INVALID_ARG, "model_segments must be 0 or 1");
+    return EMH_OK;
+}
+
 int emh_sim_get_tuning(emh_sim* s, emh_tuning* out) {
     if (!s || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
-    *out = emh_tuning{};
-    out->reorder_interval = s->reorder_substeps;
-    out->graphs = s->use_graphs ? 1u : 0u;
+    *out = s->tuning();
     return EMH_OK;
 }
 
 int emh_sim_set_tuning(emh_sim* s, const emh_tuning* t) {
     if (!s || !t) return fail(EMH_ERR_INVALID_ARG, "null argument");
-    if (t->reorder_interval == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
-    if (t->graphs > 1) return fail(EMH_ERR_INVALID_ARG, "graphs must be 0 or 1");
-    s->reorder_substeps = t->reorder_interval;
-    s->use_graphs = t->graphs != 0;
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    s->apply_tuning(*t);
     return EMH_OK;
 }
 
@@ -2710,7 +2740,8 @@ struct emh_batch {
             const bool d = s->pipe.cfg.density_pass != 0;
             one[i + 1] = one[i] + 1;
             sorts[i + 1] = sorts[i] + sort_tiles(s->pipe.n);
-            passes = std::max<uint32_t>(passes, sort_plan(s->n_bins()).passes);
+            passes = std::max<uint32_t>(
+                passes, sort_plan_bits(cell_key_bits(s->n_bins()) + s->pipe.sort.model_bits).passes);
             fused[i + 1] = fused[i] + (d ? 0u : chunks);
             dens[i + 1] = dens[i] + (d ? chunks : 0u);
             max_substeps = std::max<uint32_t>(max_substeps, s->cfg.max_substeps_per_step);
@@ -2783,6 +2814,28 @@ struct emh_batch {
         return EMH_OK;
     }
 
+    // Launch settings of the batch; a change of the scenes' model grouping
+    // rebuilds the work lists (the sort's pass count) at the next step.
+    bool lists_stale = false;
+    void apply_tuning(const emh_tuning& t) {
+        reorder_substeps = t.reorder_interval;
+        use_graphs = t.graphs != 0;
+        for (emh_sim* sim : sims) {
+            if (sim->model_segments == (t.model_segments != 0)) continue;
+            emh_tuning st = sim->tuning();
+            st.model_segments = t.model_segments;
+            sim->apply_tuning(st);
+            lists_stale = true;
+        }
+    }
+    emh_tuning tuning() const {
+        emh_tuning t{};
+        t.reorder_interval = reorder_substeps;
+        t.graphs = use_graphs ? 1u : 0u;
+        t.model_segments = !sims.empty() && sims[0]->model_segments ? 1u : 0u;
+        return t;
+    }
+
     // The batch counterpart of emh_sim::autotune: every scene is put back to
     // its saved state between candidates and after the last.
     int autotune(uint32_t trial, emh_tuning* best, emh_tuning* timed, uint32_t cap, uint32_t* n_timed) {
@@ -2798,8 +2851,7 @@ struct emh_batch {
             saved_stats[i] = sims[i]->stats;
             saved_pred[i] = sims[i]->predicted_substeps;
         }
-        const uint32_t saved_interval = reorder_substeps;
-        const bool saved_graphs = use_graphs;
+        const emh_tuning saved_tuning = tuning();
         auto restore = [&]() {
             for (size_t i = 0; i < n; ++i) {
                 sims[i]->host = saved[i];
@@ -2809,13 +2861,14 @@ struct emh_batch {
                 sims[i]->predicted_substeps = saved_pred[i];
             }
         };
-        const std::vector<emh_tuning> cands = batch_tuning_candidates();
+        bool mixed = false;
+        for (emh_sim* sim : sims) mixed = mixed || sim->mixed_models();
+        const std::vector<emh_tuning> cands = batch_tuning_candidates(mixed);
         std::vector<float> cost(cands.size(), 0.0f);
         int rc = EMH_OK;
         for (size_t c = 0; c < cands.size() && rc == EMH_OK; ++c) {
             restore();
-            reorder_substeps = cands[c].reorder_interval;
-            use_graphs = cands[c].graphs != 0;
+            apply_tuning(cands[c]);
             rc = step(1);
             if (rc) break;
             std::vector<uint64_t> s0(n);
@@ -2829,8 +2882,7 @@ struct emh_batch {
             cost[c] = (float)(std::chrono::duration<double, std::nano>(t1 - t0).count() / std::max(work, 1.0));
         }
         restore();
-        reorder_substeps = saved_interval;
-        use_graphs = saved_graphs;
+        apply_tuning(saved_tuning);
         if (rc) return rc;
         size_t bi = 0;
         for (size_t c = 1; c < cands.size(); ++c)
@@ -2840,8 +2892,7 @@ struct emh_batch {
             timed[c].ns_per_particle_substep = cost[c];
         }
         if (n_timed) *n_timed = (uint32_t)cands.size();
-        reorder_substeps = cands[bi].reorder_interval;
-        use_graphs = cands[bi].graphs != 0;
+        apply_tuning(cands[bi]);
         if (best) {
             *best = cands[bi];
             best->ns_per_particle_substep = cost[bi];
@@ -2850,9 +2901,10 @@ struct emh_batch {
     }
 
     int step(uint32_t n_steps) {
-        if (!in_sync()) {
+        if (!in_sync() || lists_stale) {
             int rc = sync();
             if (rc) return rc;
+            lists_stale = false;
         }
         int rc = refresh_configs();
         if (rc) return rc;
@@ -2939,18 +2991,15 @@ int emh_batch_step(emh_batch* b, uint32_t n_steps) {
 
 int emh_batch_get_tuning(emh_batch* b, emh_tuning* out) {
     if (!b || !out) return fail(EMH_ERR_INVALID_ARG, "null argument");
-    *out = emh_tuning{};
-    out->reorder_interval = b->reorder_substeps;
-    out->graphs = b->use_graphs ? 1u : 0u;
+    *out = b->tuning();
     return EMH_OK;
 }
 
 int emh_batch_set_tuning(emh_batch* b, const emh_tuning* t) {
     if (!b || !t) return fail(EMH_ERR_INVALID_ARG, "null argument");
-    if (t->reorder_interval == 0) return fail(EMH_ERR_INVALID_ARG, "reorder interval must be at least one substep");
-    if (t->graphs > 1) return fail(EMH_ERR_INVALID_ARG, "graphs must be 0 or 1");
-    b->reorder_substeps = t->reorder_interval;
-    b->use_graphs = t->graphs != 0;
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    b->apply_tuning(*t);
     return EMH_OK;
 }
 

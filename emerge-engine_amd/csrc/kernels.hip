@@ -561,33 +561,44 @@ __device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_
 
 
 // P2G contribution of one particle to the 3x3 stencil, one channel at a time
-// (p2g.rs:45-68): mass, momentum m (v + C d) per axis, and the stress moment
-// -4 V0 (tau d) per axis that the grid update multiplies by dt.
+// (p2g.rs:45-68): mass, the momentum m (v + C d) + (-4 V0 dt') (tau d) per
+// axis, and the stress moment -4 V0 (tau d) per axis that the grid update
+// multiplies by dt - dt'.  The momentum is linear in the cell offset d, so it
+// is formed as u + K d with u = m v and K = m C + (-4 V0 dt') tau folded once
+// per particle: per cell and axis two products and two sums before the
+// weight, instead of the reference's three matrix-vector terms.  The offsets
+// d = (cell - x) + 1/2 are exact in f32 (Sterbenz), three per axis.
 struct P2GTerms {
     Wts w;
-    float x, y, vx, vy, mass, coeff, coeff_dt;  // coeff = -4 V0, coeff_dt = coeff dt'
-    M2 c, tau;
+    float x, y, mass;
+    float ux, uy;  // m v
+    M2 k;          // m C + coeff dt' tau   (coeff = -4 V0)
+    M2 st;         // coeff tau
 };
 template <int CH>
 __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
+    float dx[3], dy[3];
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        dx[g] = ((float)(t.w.bx + g - 1) - t.x) + 0.5f;
+        dy[g] = ((float)(t.w.by + g - 1) - t.y) + 0.5f;
+    }
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
         for (int gy = 0; gy < 3; ++gy) {
             const float weight = t.w.wx[gx] * t.w.wy[gy];
-            const float dx = ((float)(t.w.bx + gx - 1) - t.x) + 0.5f;
-            const float dy = ((float)(t.w.by + gy - 1) - t.y) + 0.5f;
             float r;
             if (CH == kAccM) {
                 r = weight * t.mass;
             } else if (CH == kAccPx) {
-                r = weight * (t.mass * (t.vx + (t.c.a * dx + t.c.c * dy)) + t.coeff_dt * (t.tau.a * dx + t.tau.c * dy));
+                r = weight * (t.ux + (t.k.a * dx[gx] + t.k.c * dy[gy]));
             } else if (CH == kAccPy) {
-                r = weight * (t.mass * (t.vy + (t.c.b * dx + t.c.d * dy)) + t.coeff_dt * (t.tau.b * dx + t.tau.d * dy));
+                r = weight * (t.uy + (t.k.b * dx[gx] + t.k.d * dy[gy]));
             } else if (CH == kAccSx) {
-                r = weight * (t.coeff * (t.tau.a * dx + t.tau.c * dy));
+                r = weight * (t.st.a * dx[gx] + t.st.c * dy[gy]);
             } else {
-                r = weight * (t.coeff * (t.tau.b * dx + t.tau.d * dy));
+                r = weight * (t.st.b * dx[gx] + t.st.d * dy[gy]);
             }
             v[gx * 3 + gy] = r;
         }
@@ -696,7 +707,11 @@ __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
 struct SortJob {
     const float2* x;        // positions whose cell keys pass 0 computes
     const DevConfig* proj;  // project those positions first (nullptr: no projection)
+    const uint32_t* mat;    // material words (model segments only)
+    const DevMat* mats;     // material table (model segments only)
+    uint32_t n_mats;
     uint32_t n, grid_res, tpd, n_bins;
+    uint32_t cell_bits, key_bits;  // key = model << cell_bits | cell key when s.model_bits
     SortBuf s;
 };
 
@@ -729,7 +744,11 @@ __device__ __forceinline__ uint32_t sort_key_in(const SortJob& J, uint32_t pass,
     if (pass > 0) return J.s.kbuf[(pass - 1u) & 1u][i];
     float2 x = J.x[i];
     if (J.proj) project_x(x.x, x.y, *J.proj);
-    return cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.tpd);
+    const uint32_t key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.tpd);
+    if (!J.s.model_bits) return key;
+    const uint32_t slot = J.mat[i] & kMatMask;  // the slot mat_of reads (slot 0 for an unregistered id)
+    const uint32_t model = min(J.mats[slot < J.n_mats ? slot : 0u].model, (1u << kModelKeyBits) - 1u);
+    return (model << J.cell_bits) | key;
 }
 
 // Ranks of workgroup blk's keys on the digit of `pass`.  On return s_cnt[w][d]
@@ -782,7 +801,7 @@ __device__ __forceinline__ void sort_rank_tile(const SortJob& J, uint32_t pass, 
 
 __device__ __forceinline__ void sort_count_body(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t nblk) {
     __shared__ uint32_t s_cnt[kWaves][kSortBuckets];
-    const SortPlan pl = sort_plan(J.n_bins);
+    const SortPlan pl = sort_plan_bits(J.key_bits);
     if (pass >= pl.passes) return;
     uint32_t shift, bits;
     pass_digit(pl, pass, shift, bits);
@@ -794,7 +813,7 @@ __device__ __forceinline__ void sort_count_body(const SortJob& J, uint32_t pass,
 // Exclusive scan of one pass's counts in place (one workgroup of 1024).
 __device__ __forceinline__ void sort_scan_body(const SortJob& J, uint32_t pass) {
     __shared__ uint32_t s_scan[17];
-    const SortPlan pl = sort_plan(J.n_bins);
+    const SortPlan pl = sort_plan_bits(J.key_bits);
     if (pass >= pl.passes) return;
     uint32_t shift, bits;
     pass_digit(pl, pass, shift, bits);
@@ -822,7 +841,7 @@ __device__ __forceinline__ void sort_scan_body(const SortJob& J, uint32_t pass) 
 
 __device__ __forceinline__ void sort_scatter_body(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t nblk) {
     __shared__ uint32_t s_cnt[kWaves][kSortBuckets];
-    const SortPlan pl = sort_plan(J.n_bins);
+    const SortPlan pl = sort_plan_bits(J.key_bits);
     if (pass >= pl.passes) return;
     uint32_t shift, bits;
     pass_digit(pl, pass, shift, bits);
@@ -860,12 +879,18 @@ __device__ __forceinline__ SortJob pipeline_sort_job(const Pipeline& p, uint64_t
     const Ctrl* c = p.ctrl;
     const bool project = p.cfg.project && c->red[seq_mod3(seq + 2u)].dirty;
     SortJob J;
-    J.x = sel_buf(p, c->st[seq & 1u].cur_buf).x;
+    const PBuf b = sel_buf(p, c->st[seq & 1u].cur_buf);
+    J.x = b.x;
     J.proj = project ? &p.cfg : nullptr;
+    J.mat = b.mat;
+    J.mats = p.mats;
+    J.n_mats = p.n_mats;
     J.n = p.n;
     J.grid_res = p.cfg.grid_res;
     J.tpd = p.cfg.tiles_per_dim;
     J.n_bins = p.cfg.n_bins;
+    J.cell_bits = cell_key_bits(p.cfg.n_bins);
+    J.key_bits = J.cell_bits + p.sort.model_bits;
     J.s = p.sort;
     return J;
 }
@@ -1021,14 +1046,15 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
     t.w = weights(s.x, s.y);
     t.x = s.x;
     t.y = s.y;
-    t.vx = s.vx;
-    t.vy = s.vy;
     t.mass = s.mass;
-    t.c = s.c;
     const DevMat& mt = mat_of(p, s.mat);
-    t.tau = kirchhoff<ONLY>(mt, s);
-    t.coeff = (-stress_volume<ONLY>(mt, s)) * kDInv;
-    t.coeff_dt = t.coeff * pred_dt;
+    const M2 tau = kirchhoff<ONLY>(mt, s);
+    const float coeff = (-stress_volume<ONLY>(mt, s)) * kDInv;
+    const float coeff_dt = coeff * pred_dt;
+    t.ux = s.mass * s.vx;
+    t.uy = s.mass * s.vy;
+    t.k = madd(mscale(s.c, s.mass), mscale(tau, coeff_dt));
+    t.st = mscale(tau, coeff);
     return t;
 }
 
@@ -1039,10 +1065,11 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
                                             float (*s_acc)[kLdsCells], float* const* g, int R) {
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
     const ScanPred sq = scan_pred(sm);
-    float big = fmaxf(fmaxf(fabsf(t.mass), fmaxf(fabsf(t.coeff), fabsf(t.coeff_dt))), fmaxf(fabsf(t.vx), fabsf(t.vy)));
-    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.c.a), fabsf(t.c.b)), fmaxf(fabsf(t.c.c), fabsf(t.c.d))));
-    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.tau.a), fabsf(t.tau.b)), fmaxf(fabsf(t.tau.c), fabsf(t.tau.d))));
-    const bool small = !valid || (big < 1e18f && !__builtin_isnan(t.mass + t.coeff + t.vx + t.vy));
+    float big = fmaxf(fabsf(t.mass), fmaxf(fabsf(t.ux), fabsf(t.uy)));
+    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.k.a), fabsf(t.k.b)), fmaxf(fabsf(t.k.c), fabsf(t.k.d))));
+    big = fmaxf(big, fmaxf(fmaxf(fabsf(t.st.a), fabsf(t.st.b)), fmaxf(fabsf(t.st.c), fabsf(t.st.d))));
+    const bool small = !valid || (big < 1e18f && !__builtin_isnan(t.mass + t.ux + t.uy + t.k.a + t.k.b + t.k.c + t.k.d +
+                                                                  t.st.a + t.st.b + t.st.c + t.st.d));
     const bool fma_ok = __ballot(!small) == 0ull;
     const int at = box.stencil_at(t.w.bx, t.w.by);
     scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccM], g[kAccM], R);
@@ -1399,6 +1426,8 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         } else {
             P2GTerms t;
             t.w = weights(sp.x, sp.y);
+            t.x = sp.x;
+            t.y = sp.y;
             t.mass = sp.mass;
             const SegMask sm = run_mask(scat ? t.w.bx : INT_MIN + (int)lane_id(), scat ? t.w.by : 0);
             const bool fma_ok = __ballot(scat && !(fabsf(t.mass) < 1e18f)) == 0ull;
@@ -1814,7 +1843,8 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
     k_begin_step<<<1, 1, 0, stream>>>(p, seq, reorder ? 1u : 0u);
     if (reorder) {
         // the keys are those of the state launch seq + 1 will move
-        const uint32_t nb = sort_tiles(p.n), passes = sort_plan(p.cfg.n_bins).passes;
+        const uint32_t nb = sort_tiles(p.n);
+        const uint32_t passes = sort_plan_bits(cell_key_bits(p.cfg.n_bins) + p.sort.model_bits).passes;
         for (uint32_t pass = 0; pass < passes; ++pass) {
             k_sort_count<<<nb, kBlock, 0, stream>>>(p, seq + 1u, pass);
             k_sort_scan<<<1, 1024, 0, stream>>>(p, seq + 1u, pass);
@@ -1891,10 +1921,15 @@ hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res
     J.grid_res = grid_res;
     J.tpd = (grid_res + 15u) / 16u;
     J.n_bins = J.tpd * J.tpd * (uint32_t)kTileCells;
+    J.mat = nullptr;
+    J.mats = nullptr;
+    J.n_mats = 0;
+    J.cell_bits = J.key_bits = cell_key_bits(J.n_bins);
     J.s = s;
     J.s.key = keys_out;
     J.s.out = perm_out;
     J.s.perm = 1u;
+    J.s.model_bits = 0u;
     const uint32_t nb = sort_tiles(n), passes = sort_plan(J.n_bins).passes;
     for (uint32_t pass = 0; pass < passes; ++pass) {
         k_sortx_count<<<nb, kBlock, 0, stream>>>(J, pass);

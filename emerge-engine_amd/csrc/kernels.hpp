@@ -71,19 +71,30 @@ constexpr uint32_t kSortMaxDigit = 10;                         // at most 1024 b
 struct SortPlan {
     uint32_t key_bits, passes, digit;
 };
-__host__ __device__ inline SortPlan sort_plan(uint32_t n_bins) {
+__host__ __device__ inline uint32_t cell_key_bits(uint32_t n_bins) {
     uint32_t b = 1;
     while (b < 32u && (1u << b) < n_bins) ++b;
+    return b;
+}
+__host__ __device__ inline SortPlan sort_plan_bits(uint32_t bits) {
     SortPlan s;
-    s.key_bits = b;
-    s.passes = (b + kSortMaxDigit - 1) / kSortMaxDigit;
-    s.digit = (b + s.passes - 1) / s.passes;
+    s.key_bits = bits;
+    s.passes = (bits + kSortMaxDigit - 1) / kSortMaxDigit;
+    s.digit = (bits + s.passes - 1) / s.passes;
     return s;
 }
+__host__ __device__ inline SortPlan sort_plan(uint32_t n_bins) { return sort_plan_bits(cell_key_bits(n_bins)); }
+// Bits of the constitutive model above the cell key when the reorder groups
+// particles by model (per-material segments): model ids are below 16.
+constexpr uint32_t kModelKeyBits = 4;
 __host__ __device__ inline uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
 // Counts of one pass: (1 << digit) buckets x sort_tiles(n) workgroups.
 __host__ __device__ inline size_t sort_hist_len(uint32_t n, uint32_t n_bins) {
     return ((size_t)1 << sort_plan(n_bins).digit) * (size_t)sort_tiles(n);
+}
+// Enough for any key width: at most 2^kSortMaxDigit buckets per workgroup.
+__host__ __device__ inline size_t sort_hist_len_max(uint32_t n) {
+    return ((size_t)1 << kSortMaxDigit) * (size_t)sort_tiles(n);
 }
 
 struct SortBuf {
@@ -91,8 +102,9 @@ struct SortBuf {
     uint32_t* kbuf[2];  // N each: keys in the order after pass 0, 1 (kbuf[1] may alias key)
     uint32_t* vbuf[2];  // N each: the slot each of those keys came from
     uint32_t* out;      // N: the result (see perm)
-    uint32_t* hist;     // sort_hist_len(N, n_bins): per-pass counts, then start slots
+    uint32_t* hist;     // per-pass counts, then start slots (sort_hist_len_max(N) for the pipeline)
     uint32_t perm;      // 1: out[sorted position] = slot; 0: out[slot] = sorted position
+    uint32_t model_bits;  // pipeline reorder: kModelKeyBits to sort by (model, cell key), 0 by cell key
 };
 
 // Device-resident control: the adaptive substep loop runs on the device so

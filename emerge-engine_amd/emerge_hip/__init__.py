@@ -114,7 +114,7 @@ class CKernelTime(ctypes.Structure):
 
 class CTuning(ctypes.Structure):
     _fields_ = [("reorder_interval", ctypes.c_uint32), ("graphs", ctypes.c_uint32),
-                ("ns_per_particle_substep", ctypes.c_float), ("pad", ctypes.c_uint32)]
+                ("ns_per_particle_substep", ctypes.c_float), ("model_segments", ctypes.c_uint32)]
 
 
 class CStepStats(ctypes.Structure):
@@ -769,7 +769,7 @@ class StepStats:
 
 def _tuning_dict(t: CTuning) -> dict:
     return {"reorder_interval": int(t.reorder_interval), "graphs": int(t.graphs),
-            "ns_per_particle_substep": float(t.ns_per_particle_substep)}
+            "model_segments": int(t.model_segments), "ns_per_particle_substep": float(t.ns_per_particle_substep)}
 
 
 class _Tunable:
@@ -785,8 +785,10 @@ class _Tunable:
         _check(self._fn("get_tuning")(self._h, ctypes.byref(t)))
         return _tuning_dict(t)
 
-    def set_tuning(self, reorder_interval: int, graphs: bool) -> None:
-        t = CTuning(int(reorder_interval), 1 if graphs else 0, 0.0, 0)
+    def set_tuning(self, reorder_interval: int, graphs: bool, model_segments: Optional[bool] = None) -> None:
+        if model_segments is None:
+            model_segments = bool(self.tuning()["model_segments"])
+        t = CTuning(int(reorder_interval), 1 if graphs else 0, 0.0, 1 if model_segments else 0)
         _check(self._fn("set_tuning")(self._h, ctypes.byref(t)))
 
     def autotune(self, trial_steps: int = 10) -> Tuple[dict, list]:
@@ -808,7 +810,7 @@ class _Tunable:
                 cache = json.load(f)
         if key in cache:
             c = cache[key]
-            self.set_tuning(c["reorder_interval"], bool(c["graphs"]))
+            self.set_tuning(c["reorder_interval"], bool(c["graphs"]), bool(c.get("model_segments", 1)))
             return dict(c, cached=True)
         best, timed = self.autotune(trial_steps)
         cache[key] = dict(best, candidates=timed)

@@ -437,23 +437,32 @@ int emh_sim_set_profiling(emh_sim* sim, int enable);
 int emh_sim_set_reorder_interval(emh_sim* sim, uint32_t substeps);
 /* The stored order itself: ids_out[j] is the caller index of the particle in
  * device slot j (n = the particle count).  Right after a reorder this is the
- * stable sort of the cell keys of the positions the reorder saw -- the same
- * permutation emh_sort_cell_keys returns for them.  EMH_ERR_STATE before the
- * first step after an upload (the particles are not on the device yet). */
+ * stable sort of the keys of the positions the reorder saw, by stored slot:
+ * the cell key (emh_sort_cell_keys' permutation of them, for an upload) or,
+ * with per-material segments in a scene of several models, model << bits |
+ * cell key (emh_tuning).  EMH_ERR_STATE before the first step after an upload
+ * (the particles are not on the device yet). */
 int emh_sim_get_slot_order(emh_sim* sim, uint32_t* ids_out, size_t n);
 int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
 
 /* ---- launch tuning and the autotuner --------------------------------------
  * No counterpart in the reference (its WGSL backend submits up to 64
  * substeps per frame in one command buffer, gpu/solver/step.rs:404-441).
- * Two run-time settings change how a step is launched, never its results
+ * Three run-time settings change how a step is launched, never its results
  * beyond f32 summation order:
  *   reorder_interval  substeps between physical cell-key reorders (>= 1);
  *   graphs            1: each step's run of substep launches is captured
  *                     once into a hipGraph (per launch-number residue mod 6,
  *                     run length and kernel variant) and replayed; 0: the
- *                     launches are queued one by one.
- * emh_*_autotune times every candidate (reorder intervals x graphs) on the
+ *                     launches are queued one by one;
+ *   model_segments    1: a scene whose particles use several constitutive
+ *                     models is reordered by (model, cell key), so that a
+ *                     chunk of the substep -- and each wave of it -- runs one
+ *                     model's update; 0 (default): by cell key alone, which
+ *                     keeps each chunk spatially compact.  It takes effect at
+ *                     the next reorder.
+ * emh_*_autotune times every candidate (reorder intervals x graphs, x model
+ * segments on / off for scenes that mix models) on the
  * scene itself: from the current state, one untimed step, then trial_steps
  * steps timed on the host clock to their completion; it then puts the state
  * back (the particles are re-uploaded at the next step, the step counters
@@ -463,7 +472,7 @@ typedef struct emh_tuning {
     uint32_t reorder_interval;
     uint32_t graphs;
     float    ns_per_particle_substep; /* set by the autotuners: wall time / (particles x substeps) */
-    uint32_t pad;
+    uint32_t model_segments;
 } emh_tuning;
 int emh_sim_get_tuning(emh_sim* sim, emh_tuning* out);
 int emh_sim_set_tuning(emh_sim* sim, const emh_tuning* t);

@@ -94,22 +94,39 @@ def test_pipeline_reorder_permutation_is_the_stable_sort(oracle):
     assert np.array_equal(orders[1], orders[0])
 
 
+# constitutive model id of each material slot of scenes.multi_material:
+# NeoHookean (2), Newtonian water (1), Stomakhin snow (4), corotated (3)
+MULTI_MATERIAL_MODELS = np.array([2, 1, 4, 3], np.uint32)
+
+
+def _cell_bits(res):
+    n_bins = ((res + 15) // 16) ** 2 * 256
+    return int(np.ceil(np.log2(n_bins)))
+
+
 @pytest.mark.gpu
-def test_pipeline_mid_run_reorder_is_stable_over_the_stored_order(oracle):
-    """A reorder at a later step start sorts the stored slots by the cell key of their
+@pytest.mark.parametrize("segments", [True, False], ids=["model_segments", "cell_key_only"])
+def test_pipeline_mid_run_reorder_is_stable_over_the_stored_order(segments, oracle):
+    """A reorder at a later step start sorts the stored slots by the key of their
     current positions, ties in slot order: new order = old order composed with the
     stable argsort of the keys read in old slot order (multi-material scene with a
-    density pass, reordering at every step)."""
+    density pass, reordering at every step).  With per-material segments the key is
+    model << cell bits | cell key, and the stored order comes out grouped by model."""
     import scenes as S
     sim = S.multi_material("gpu")
-    sim.set_reorder_interval(1)
+    sim.set_tuning(1, False, segments)
     res = sim.config.grid_res
     sim.step_n(3)
     for _ in range(3):
         old = sim.slot_order()
-        xy = sim.particles()["x"]
-        keys = oracle.stable_sort_cell_keys(xy, res)[0]  # keys in caller order
+        p = sim.particles()
+        keys = oracle.stable_sort_cell_keys(p["x"], res)[0].astype(np.uint64)  # cell keys in caller order
+        if segments:
+            keys |= MULTI_MATERIAL_MODELS[p["material_id"]].astype(np.uint64) << _cell_bits(res)
         sim.step_n(1)
         new = sim.slot_order()
         assert np.array_equal(new, old[_stable_order(keys[old])])
+        if segments:
+            models = MULTI_MATERIAL_MODELS[p["material_id"][new]]
+            assert np.all(np.diff(models.astype(np.int64)) >= 0)  # one contiguous segment per model
     sim.close()
