@@ -83,7 +83,7 @@ typedef struct emh_config {
     uint32_t max_substeps_per_step;       /* usize */
     float    apic_blend;
     float    j_max;
-    float    sleep_threshold;             /* must be 0: sleep partition is not supported */
+    float    sleep_threshold;             /* > 0: the sleep partition (see emh_sim_sleep_tag) */
     float    contact_friction;
     float    asflip_blend;                /* must be 0: ASFLIP is not supported */
     float    mixture_drag_coefficient;
