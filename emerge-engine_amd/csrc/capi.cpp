@@ -342,7 +342,7 @@ template <class T> T* dalloc(size_t n) {
 // replay, x per-material segments on and off when some scene mixes models.
 static std::vector<emh_tuning> tuning_candidates(const uint32_t* intervals, size_t n, bool mixed) {
     std::vector<emh_tuning> v;
-    for (uint32_t seg = mixed ? 0u : 1u; seg < 2; ++seg)
+    for (uint32_t seg = 0; seg < (mixed ? 2u : 1u); ++seg)
         for (uint32_t g = 0; g < 2; ++g)
             for (size_t i = 0; i < n; ++i) {
                 emh_tuning t{};

@@ -207,9 +207,22 @@ __device__ __forceinline__ void project_x(float& x, float& y, const DevConfig& c
 }
 
 // Would projecting `s` change it?  (`sp` receives the projected state.)
+// Every lane of the wave calls this.  A state that project_state leaves
+// untouched is recognised directly (finite fields, positive scalars, x inside
+// the clamp range, J within (proj_min_j, j_max], density at or above its
+// floor); only a wave with some other lane runs project_state itself.
 __device__ __forceinline__ bool needs_projection(const PState& s, PState& sp, const DevConfig& cfg) {
     sp = s;
     if (!cfg.project) return false;
+    const float j = mdet(s.f);
+    const bool ok = s.x >= cfg.proj_lo && s.x <= cfg.proj_hi && s.y >= cfg.proj_lo && s.y <= cfg.proj_hi &&
+                    __builtin_isfinite(s.vx) && __builtin_isfinite(s.vy) && mfinite(s.c) && mfinite(s.f) &&
+                    j > cfg.proj_min_j && j <= cfg.j_max && s.jp > 0.0f && __builtin_isfinite(s.jp) && s.h > 0.0f &&
+                    __builtin_isfinite(s.h) && __builtin_isfinite(s.q) && __builtin_isfinite(s.lvs) &&
+                    s.mass > 0.0f && __builtin_isfinite(s.mass) && s.v0 > 0.0f && __builtin_isfinite(s.v0) &&
+                    s.vol > 0.0f && __builtin_isfinite(s.vol) && s.dens >= cfg.proj_min_density &&
+                    __builtin_isfinite(s.dens);
+    if (__ballot(!ok) == 0ull) return false;
     const bool reset = project_state(sp, cfg);
     return reset || __float_as_uint(sp.x) != __float_as_uint(s.x) || __float_as_uint(sp.y) != __float_as_uint(s.y) ||
            __float_as_uint(sp.dens) != __float_as_uint(s.dens);
@@ -429,15 +442,20 @@ struct ScanPred {
     bool p[6];   // per step: the source lane is in this lane's run
     float f[6];  // the same as 1.0f / 0.0f
 };
+// Source lane src < lane is in this lane's run exactly when src >= the run's
+// first lane (the highest run head at or below this lane); the row-broadcast
+// steps read lane 15 (into rows 1 and 3) and lane 31 (into rows 2 and 3).
 __device__ __forceinline__ ScanPred scan_pred(const SegMask& m) {
     ScanPred q;
     const int l = (int)m.lane, r = l & 15;
-    q.p[0] = r >= 1 && same_run(m, l - 1);
-    q.p[1] = r >= 2 && same_run(m, l - 2);
-    q.p[2] = r >= 4 && same_run(m, l - 4);
-    q.p[3] = r >= 8 && same_run(m, l - 8);
-    q.p[4] = (l >= 16) && same_run(m, (l & ~15) - 1) && ((l >> 4) == 1 || (l >> 4) == 3);
-    q.p[5] = (l >= 32) && same_run(m, 31);
+    const uint64_t upto = (m.lane == 63) ? ~0ull : ((2ull << m.lane) - 1ull);
+    const int start = 63 - __clzll((long long)(m.heads & upto));  // lane 0 is always a head
+    q.p[0] = r >= 1 && l - 1 >= start;
+    q.p[1] = r >= 2 && l - 2 >= start;
+    q.p[2] = r >= 4 && l - 4 >= start;
+    q.p[3] = r >= 8 && l - 8 >= start;
+    q.p[4] = ((l >> 4) == 1 || (l >> 4) == 3) && (l & ~15) - 1 >= start;
+    q.p[5] = l >= 32 && 31 >= start;
 #pragma unroll
     for (int k = 0; k < 6; ++k) q.f[k] = q.p[k] ? 1.0f : 0.0f;
     return q;
