@@ -48,7 +48,8 @@ MEASURED_TRAFFIC = {"k_substep": (27728.2 * 2.000 + 80394.4 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu"):
-    """The 1M-particle sand column on either backend (the oracle for the CPU baseline)."""
+    """The 1M-particle sand column on the GPU ("gpu") or the CPU oracle ("oracle" = f32,
+    "oracle64" = the FP64 oracle)."""
     cfg = eh.SimConfig.standard(512, 1.0 / 60.0, (0.0, -9.81))
     spawn = eh.SpawnRegion(spacing=0.25, box_size=(256, 256), box_center=(256.0, 130.0), position_jitter=1.0,
                            rng_seed=42)
@@ -63,7 +64,8 @@ def build_workload(backend: str = "gpu"):
         lam, mu = oracle_py.lame_from_young(3.5e5, 0.3)
         mat = oracle_py.material("drucker_prager", lam, mu)
         mat.params.dp_h0 = oracle_py.to_radians(30.0)
-        sim = oracle_py.OracleSim.empty(cfg, 32).with_default_material(mat).with_boundary(eh.SlipBoundary.new(2))
+        sim = oracle_py.OracleSim.empty(cfg, 64 if backend == "oracle64" else 32).with_default_material(
+            mat).with_boundary(eh.SlipBoundary.new(2))
     sim.add_body(spawn)
     return sim
 

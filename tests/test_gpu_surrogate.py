@@ -304,3 +304,44 @@ def test_shared_harvest_buffer_cannot_grow_under_another_harvester():
     for s in (a, b):
         s.close()
     sg.close()
+
+
+@pytest.mark.gpu
+def test_device_adam_matches_reference_over_steps():
+    """Four full-batch Adam steps on the device (k_sg_fwd_bwd + k_sg_wgrad + k_sg_adam,
+    cosine learning rate) against the CPU reference: each step's gradient by the chain
+    rule with the device's BF16 operand rounding emulated, then Adam with bias
+    correction 1 / (1 - beta^t) at t = 1..4.  A planted mistake in the reference
+    (bias correction left at t = 1) moves the weights far more than the device differs."""
+    import surrogate_ref as R
+    rng = np.random.default_rng(21)
+    n = 256
+    x = rng.normal(0.0, 1.0, (n, 6)).astype(np.float32)
+    x[:, 3:] = x[:, 3:] * 0.1 + np.array([1.85, 0.5, 0.8], np.float32)
+    y = (x[:, :3] + 0.05 * np.tanh(x[:, :3]) + rng.normal(0.0, 0.01, (n, 3))).astype(np.float32)
+    sg = eh.NaccSurrogate(9)
+    sg.set_training_set(x, y)
+    w0, norm = sg.params()
+    steps, lr, lr_final, b1, b2, eps = 4, 1e-2, 1e-3, 0.9, 0.999, 1e-8
+    loss = sg.train(epochs=steps, batch=n, lr=lr, lr_final=lr_final, beta1=b1, beta2=b2, eps=eps, seed=5)
+    assert len(loss) == steps
+    w_dev, _ = sg.params()
+    sg.close()
+
+    def reference(bias_steps):
+        w = w0.astype(np.float64)
+        m = np.zeros_like(w)
+        v = np.zeros_like(w)
+        for t in range(1, steps + 1):
+            g, _ = R.gradients(w.astype(np.float32), norm, x, y, exact=False)
+            lr_t = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * (t - 1) / (steps - 1)))
+            w, m, v = R.adam_step(w, m, v, g, bias_steps(t), lr_t, b1, b2, eps)
+        return w
+
+    w_ref = reference(lambda t: t)
+    moved = np.linalg.norm(w_ref - w0)
+    err = np.linalg.norm(w_dev - w_ref) / moved
+    w_bad = reference(lambda t: 1)  # bias correction frozen at the first step
+    err_bad = np.linalg.norm(w_bad - w_ref) / moved
+    print(f"device vs reference after {steps} Adam steps: {err:.2e} of the update; planted mistake {err_bad:.2e}")
+    assert err < 2e-2 and err_bad > 10 * err
