@@ -140,6 +140,54 @@ def full_run_section(args, tuning) -> dict:
             "note": "includes the first step's upload, the physical reorders and the whole collapse"}
 
 
+SORT_BYTES_PER_KEY = 16  # compulsory: the position read (8 B), the key and the permutation entry written
+
+
+def sort_section(args) -> dict:
+    """The cell-key sort on its own (emh_sort_cell_keys_noalloc on device buffers):
+    the headline scene's 1,048,576 positions on its 512^2 grid (18-bit keys, two
+    9-bit passes), in the order the scene generator writes them and in a random
+    order, each sorted `--sort-reps` times back to back and timed with HIP events
+    on the stream the sorts are enqueued on.  Every result is checked on the host
+    (keys non-decreasing along the permutation, equal keys in increasing slot
+    order, the permutation a bijection)."""
+    import torch
+    sim = build_workload("gpu")
+    x = np.ascontiguousarray(sim.particles()["x"], dtype=np.float32)
+    sim.close()
+    n, R = len(x), 512
+    stream = torch.cuda.current_stream()
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    perm = torch.empty(n, dtype=torch.int32, device="cuda")
+    out = {"workload": "cell_key_sort_1m_512", "keys": n, "grid_res": R,
+           "bytes_per_key": SORT_BYTES_PER_KEY, "reps": args.sort_reps}
+    orders = {"generated_order": np.arange(n), "random_order": np.random.default_rng(11).permutation(n)}
+    for name, order in orders.items():
+        xy = torch.from_numpy(np.ascontiguousarray(x[order])).cuda()
+
+        def run():
+            eh.sort_cell_keys_noalloc(xy.data_ptr(), n, R, keys.data_ptr(), perm.data_ptr(), stream.cuda_stream)
+
+        run()
+        torch.cuda.synchronize()
+        k = keys.cpu().numpy().view(np.uint32).astype(np.int64)
+        p = perm.cpu().numpy().view(np.uint32).astype(np.int64)
+        ks = k[p]
+        dk, dp = np.diff(ks), np.diff(p)
+        ok = bool(np.all(dk >= 0) and np.all((dk > 0) | (dp > 0)) and np.array_equal(np.sort(p), np.arange(n)))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.sort_reps):
+            run()
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / args.sort_reps
+        gbs = SORT_BYTES_PER_KEY * n / us / 1e3
+        out[name] = {"us_per_sort": us, "keys_per_s": n / us * 1e6, "achieved_GBs": gbs,
+                     "roofline_frac": gbs / HBM_PEAK_GBS, "stable_permutation": ok}
+    return out
+
+
 def build_multi_material(segments: bool = True):
     """Four-material scene at its stated size: 1024^2 grid, 370 seeded boxes with
     sides U[8, 96] at 4 particles per cell (3.98M particles; the stated 40 boxes
@@ -507,6 +555,8 @@ def main() -> int:
                     help="the whole collapse from rest, timed as one run (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=20,
                     help="timed steps of the four-material 1024^2 scene, per ordering (0: skip)")
+    ap.add_argument("--sort-reps", type=int, default=50,
+                    help="timed sorts of the headline positions, per input order (0: skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes section")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -595,19 +645,30 @@ def main() -> int:
         "setup_s": build_s,
         "tuning": tuning,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, first_dt)
-    else:
-        out["cpu_baseline"] = None
+    _progress(f"headline: {rate:.4g} particle-substeps/s, {dominant} {avg_s * 1e6:.1f} us")
     sim.close()
-    out["full_run"] = None if args.full_run_steps <= 0 else full_run_section(args, tuning)
-    out["multi_material"] = None if args.mixed_steps <= 0 else multi_material_section(args)
-    out["nacc_surrogate"] = None if args.no_surrogate else surrogate_section(args)
-    out["batched_scenes"] = None if args.no_batch else batch_section(args)
+    sections = [("cpu_baseline", not args.no_cpu_baseline, lambda: cpu_baseline(args.cpu_seconds, first_dt)),
+                ("cell_sort", args.sort_reps > 0, lambda: sort_section(args)),
+                ("full_run", args.full_run_steps > 0, lambda: full_run_section(args, tuning)),
+                ("multi_material", args.mixed_steps > 0, lambda: multi_material_section(args)),
+                ("nacc_surrogate", not args.no_surrogate, lambda: surrogate_section(args)),
+                ("batched_scenes", not args.no_batch, lambda: batch_section(args))]
+    for name, on, run in sections:
+        if on:
+            _progress(f"{name} ...")
+            t_sec = time.perf_counter()
+        out[name] = run() if on else None
+        if on:
+            _progress(f"{name} done in {time.perf_counter() - t_sec:.1f} s")
     print(json.dumps(out), flush=True)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)
     return 0
+
+
+def _progress(msg: str) -> None:
+    """One line per finished part of the run, on stderr (stdout carries the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def copy_calibration(nbytes: int) -> None:

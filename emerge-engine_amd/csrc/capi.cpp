@@ -2680,7 +2680,7 @@ struct emh_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     Pipeline* d_pipes = nullptr;
-    uint32_t* d_starts = nullptr;  // four work lists of n + 1 entries
+    uint32_t* d_starts = nullptr;  // five work lists of n + 1 entries
     Ctrl* d_ctrl = nullptr;        // gathered control blocks
     std::vector<Ctrl> h_ctrl;
     BatchLists L{};
@@ -2710,11 +2710,12 @@ struct emh_batch {
         const uint32_t n = (uint32_t)sims.size();
         std::vector<Pipeline> pipes(n);
         epochs.assign(n, 0);
-        std::vector<uint32_t> st(4 * (n + 1), 0u);
+        std::vector<uint32_t> st(5 * (n + 1), 0u);
         uint32_t* one = st.data();
         uint32_t* sorts = one + (n + 1);
         uint32_t* fused = sorts + (n + 1);
         uint32_t* dens = fused + (n + 1);
+        uint32_t* scans = dens + (n + 1);
         max_substeps = 1;
         predicted = 1;
         uint32_t passes = 0;
@@ -2740,8 +2741,9 @@ struct emh_batch {
             const bool d = s->pipe.cfg.density_pass != 0;
             one[i + 1] = one[i] + 1;
             sorts[i + 1] = sorts[i] + sort_tiles(s->pipe.n);
-            passes = std::max<uint32_t>(
-                passes, sort_plan_bits(cell_key_bits(s->n_bins()) + s->pipe.sort.model_bits).passes);
+            const uint32_t key_bits = cell_key_bits(s->n_bins()) + s->pipe.sort.model_bits;
+            scans[i + 1] = scans[i] + sort_scan_groups(key_bits);
+            passes = std::max<uint32_t>(passes, sort_plan_bits(key_bits).passes);
             fused[i + 1] = fused[i] + (d ? 0u : chunks);
             dens[i + 1] = dens[i] + (d ? chunks : 0u);
             max_substeps = std::max<uint32_t>(max_substeps, s->cfg.max_substeps_per_step);
@@ -2755,7 +2757,9 @@ struct emh_batch {
         L.sort = view(1);
         L.fused = view(2);
         L.density = view(3);
+        L.scan = view(4);
         L.sort_blocks = sorts[n];
+        L.scan_blocks = scans[n];
         L.sort_passes = passes;
         L.fused_blocks = fused[n];
         L.density_blocks = dens[n];
@@ -2969,7 +2973,7 @@ int emh_batch_create(emh_sim* const* sims, uint32_t n, emh_batch** out) {
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(EMH_ERR_DEVICE, "stream creation failed"));
     b->d_pipes = dalloc<Pipeline>(n);
-    b->d_starts = dalloc<uint32_t>(4 * ((size_t)n + 1));
+    b->d_starts = dalloc<uint32_t>(5 * ((size_t)n + 1));
     b->d_ctrl = dalloc<Ctrl>(n);
     if (!b->d_pipes || !b->d_starts || !b->d_ctrl) return bail(fail(EMH_ERR_OUT_OF_MEMORY, "device allocation failed"));
     b->h_ctrl.resize(n);

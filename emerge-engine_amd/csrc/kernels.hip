@@ -427,13 +427,6 @@ struct SegMask {
     uint64_t heads;  // bit L: lane L starts a run
     uint32_t lane;
 };
-// Lanes in (src, lane] contain no run head <=> lane src is in this lane's run.
-__device__ __forceinline__ bool same_run(const SegMask& m, int src) {
-    if (src < 0) return false;
-    const uint64_t upto_lane = (m.lane == 63) ? ~0ull : ((2ull << m.lane) - 1ull);
-    const uint64_t upto_src = (2ull << (uint32_t)src) - 1ull;
-    return (m.heads & (upto_lane & ~upto_src)) == 0ull;
-}
 template <int CTRL, int ROWS>
 __device__ __forceinline__ float dpp_take(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, true));
@@ -712,10 +705,11 @@ __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
 // result is the order std::stable_sort gives -- by key, then by slot.
 //   count   per (digit, workgroup) counts of one pass (pass 0 also computes
 //           the keys, from the stored and, when flagged, projected positions);
-//   scan    exclusive scan of the counts in (digit, workgroup) order: the
-//           first output position of each digit of each workgroup;
-//   scatter each key's position = that start + its rank among the keys of
-//           the workgroup with the same digit and a lower slot.
+//   scan    one wave per digit: exclusive scan of the digit's counts over
+//           the workgroups in place, and the digit's total;
+//   scatter each key's position = the totals of the lower digits + the
+//           digit's start in its workgroup + its rank among the keys of the
+//           workgroup with the same digit and a lower slot.
 // A workgroup's kSortTile keys are 64 x kSortItems rows; wave w takes the
 // kSortItems consecutive rows from w * kSortItems, so wave order, row order
 // and lane order together are slot order.  A key's rank among the equal
@@ -828,33 +822,43 @@ __device__ __forceinline__ void sort_count_body(const SortJob& J, uint32_t pass,
                    [&](uint32_t d, uint32_t t) { J.s.hist[(size_t)d * nblk + blk] = t; });
 }
 
-// Exclusive scan of one pass's counts in place (one workgroup of 1024).
-__device__ __forceinline__ void sort_scan_body(const SortJob& J, uint32_t pass) {
-    __shared__ uint32_t s_scan[17];
+// The digit totals of a pass follow its (1 << digit) x nblk counts.
+__device__ __forceinline__ uint32_t* sort_totals(const SortJob& J, const SortPlan& pl, uint32_t nblk) {
+    return J.s.hist + ((size_t)1 << pl.digit) * nblk;
+}
+
+// Scan group g of a pass: wave w scans the counts of digit g * kSortRowsPerGroup
+// + w over the workgroups (exclusive, in place) and writes the digit's total.
+// Waves past the pass's digits return at once (the body has no barrier).
+__device__ __forceinline__ void sort_scan_body(const SortJob& J, uint32_t pass, uint32_t g) {
     const SortPlan pl = sort_plan_bits(J.key_bits);
     if (pass >= pl.passes) return;
     uint32_t shift, bits;
     pass_digit(pl, pass, shift, bits);
-    const uint32_t len = (1u << bits) * sort_tiles(J.n);
-    constexpr uint32_t kPerThread = 16;
+    const uint32_t d = g * kSortRowsPerGroup + (threadIdx.x >> 6);
+    if (d >= (1u << bits)) return;
+    const uint32_t nblk = sort_tiles(J.n), lane = lane_id();
+    uint32_t* row = J.s.hist + (size_t)d * nblk;
+    constexpr uint32_t kPerLane = 8;  // 512 workgroups per round: all of them up to 1M keys
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < len; base += blockDim.x * kPerThread) {
-        const uint32_t i0 = base + threadIdx.x * kPerThread;
-        uint32_t v[kPerThread], sum = 0;
+    for (uint32_t base = 0; base < nblk; base += 64u * kPerLane) {
+        const uint32_t t0 = base + lane * kPerLane;
+        uint32_t v[kPerLane], sum = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            v[j] = i0 + j < len ? J.s.hist[i0 + j] : 0u;
+        for (uint32_t j = 0; j < kPerLane; ++j) {
+            v[j] = t0 + j < nblk ? row[t0 + j] : 0u;
             sum += v[j];
         }
-        uint32_t total;
-        uint32_t run = carry + block_excl_scan(sum, s_scan, total);
+        const uint32_t incl = wave_incl_scan(sum);
+        uint32_t run = carry + incl - sum;
 #pragma unroll
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            if (i0 + j < len) J.s.hist[i0 + j] = run;
+        for (uint32_t j = 0; j < kPerLane; ++j) {
+            if (t0 + j < nblk) row[t0 + j] = run;
             run += v[j];
         }
-        carry += total;
+        carry += __shfl(incl, 63, 64);
     }
+    if (lane == 0) sort_totals(J, pl, nblk)[d] = carry;
 }
 
 __device__ __forceinline__ void sort_scatter_body(const SortJob& J, uint32_t pass, uint32_t blk, uint32_t nblk) {
@@ -864,10 +868,32 @@ __device__ __forceinline__ void sort_scatter_body(const SortJob& J, uint32_t pas
     uint32_t shift, bits;
     pass_digit(pl, pass, shift, bits);
     const bool last = pass + 1u == pl.passes;
+    // the first slot of each digit: exclusive scan of the digit totals
+    __shared__ uint32_t s_base[kSortBuckets];
+    __shared__ uint32_t s_scan[17];
+    static_assert(kBlock * 4 == kSortBuckets, "four digit totals per thread");
+    {
+        const uint32_t* tot = sort_totals(J, pl, nblk);
+        const uint32_t d0 = threadIdx.x * 4u, buckets = 1u << bits;
+        uint32_t v[4], sum = 0, total;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            v[j] = d0 + j < buckets ? tot[d0 + j] : 0u;
+            sum += v[j];
+        }
+        uint32_t run = block_excl_scan(sum, s_scan, total);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if (d0 + j < buckets) s_base[d0 + j] = run;
+            run += v[j];
+        }
+        // block_excl_scan ends on a barrier and sort_rank_tile barriers
+        // before its first read of s_base
+    }
     uint32_t key[kSortItems], loc[kSortItems];
     const uint32_t* hist = J.s.hist;
     sort_rank_tile(J, pass, blk, shift, bits, s_cnt, key, loc,
-                   [&](uint32_t d) { return hist[(size_t)d * nblk + blk]; }, [](uint32_t, uint32_t) {});
+                   [&](uint32_t d) { return s_base[d] + hist[(size_t)d * nblk + blk]; }, [](uint32_t, uint32_t) {});
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6, mask = (1u << bits) - 1u;
     const uint32_t row0 = blk * kSortTile + w * (uint32_t)kSortItems * 64u;
     const uint32_t* vin = pass > 0 ? J.s.vbuf[(pass - 1u) & 1u] : nullptr;
@@ -915,8 +941,8 @@ __device__ __forceinline__ SortJob pipeline_sort_job(const Pipeline& p, uint64_t
 __global__ __launch_bounds__(kBlock) void k_sort_count(const Pipeline p, uint64_t seq, uint32_t pass) {
     sort_count_body(pipeline_sort_job(p, seq), pass, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(1024) void k_sort_scan(const Pipeline p, uint64_t seq, uint32_t pass) {
-    sort_scan_body(pipeline_sort_job(p, seq), pass);
+__global__ __launch_bounds__(kBlock) void k_sort_scan(const Pipeline p, uint64_t seq, uint32_t pass) {
+    sort_scan_body(pipeline_sort_job(p, seq), pass, blockIdx.x);
 }
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const Pipeline p, uint64_t seq, uint32_t pass) {
     sort_scatter_body(pipeline_sort_job(p, seq), pass, blockIdx.x, gridDim.x);
@@ -1607,10 +1633,10 @@ __global__ __launch_bounds__(kBlock) void kb_sort_count(BatchView b, uint64_t se
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
     sort_count_body(pipeline_sort_job(b.pipes[s], seq), pass, blk, nblk);
 }
-__global__ __launch_bounds__(1024) void kb_sort_scan(BatchView b, uint64_t seq, uint32_t pass) {
+__global__ __launch_bounds__(kBlock) void kb_sort_scan(BatchView b, uint64_t seq, uint32_t pass) {
     uint32_t blk, nblk;
     const uint32_t s = batch_scene(b, blockIdx.x, blk, nblk);
-    sort_scan_body(pipeline_sort_job(b.pipes[s], seq), pass);
+    sort_scan_body(pipeline_sort_job(b.pipes[s], seq), pass, blk);
 }
 __global__ __launch_bounds__(kBlock) void kb_sort_scatter(BatchView b, uint64_t seq, uint32_t pass) {
     uint32_t blk, nblk;
@@ -1796,7 +1822,9 @@ __global__ __launch_bounds__(kBlock) void k_snap_grid(const Pipeline p, uint32_t
 __global__ __launch_bounds__(kBlock) void k_sortx_count(const SortJob J, uint32_t pass) {
     sort_count_body(J, pass, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(1024) void k_sortx_scan(const SortJob J, uint32_t pass) { sort_scan_body(J, pass); }
+__global__ __launch_bounds__(kBlock) void k_sortx_scan(const SortJob J, uint32_t pass) {
+    sort_scan_body(J, pass, blockIdx.x);
+}
 __global__ __launch_bounds__(kBlock) void k_sortx_scatter(const SortJob J, uint32_t pass) {
     sort_scatter_body(J, pass, blockIdx.x, gridDim.x);
 }
@@ -1835,7 +1863,7 @@ hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reord
         // the keys are those of the state launch seq + 1 will move
         for (uint32_t pass = 0; pass < L.sort_passes; ++pass) {
             kb_sort_count<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
-            kb_sort_scan<<<L.one.n, 1024, 0, stream>>>(L.one, seq + 1u, pass);
+            kb_sort_scan<<<L.scan_blocks, kBlock, 0, stream>>>(L.scan, seq + 1u, pass);
             kb_sort_scatter<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
         }
     }
@@ -1862,10 +1890,11 @@ hipError_t launch_begin_step(const Pipeline& p, uint64_t seq, bool reorder, hipS
     if (reorder) {
         // the keys are those of the state launch seq + 1 will move
         const uint32_t nb = sort_tiles(p.n);
-        const uint32_t passes = sort_plan_bits(cell_key_bits(p.cfg.n_bins) + p.sort.model_bits).passes;
+        const uint32_t key_bits = cell_key_bits(p.cfg.n_bins) + p.sort.model_bits;
+        const uint32_t passes = sort_plan_bits(key_bits).passes, ng = sort_scan_groups(key_bits);
         for (uint32_t pass = 0; pass < passes; ++pass) {
             k_sort_count<<<nb, kBlock, 0, stream>>>(p, seq + 1u, pass);
-            k_sort_scan<<<1, 1024, 0, stream>>>(p, seq + 1u, pass);
+            k_sort_scan<<<ng, kBlock, 0, stream>>>(p, seq + 1u, pass);
             k_sort_scatter<<<nb, kBlock, 0, stream>>>(p, seq + 1u, pass);
         }
     }
@@ -1948,10 +1977,10 @@ hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res
     J.s.out = perm_out;
     J.s.perm = 1u;
     J.s.model_bits = 0u;
-    const uint32_t nb = sort_tiles(n), passes = sort_plan(J.n_bins).passes;
+    const uint32_t nb = sort_tiles(n), passes = sort_plan(J.n_bins).passes, ng = sort_scan_groups(J.key_bits);
     for (uint32_t pass = 0; pass < passes; ++pass) {
         k_sortx_count<<<nb, kBlock, 0, stream>>>(J, pass);
-        k_sortx_scan<<<1, 1024, 0, stream>>>(J, pass);
+        k_sortx_scan<<<ng, kBlock, 0, stream>>>(J, pass);
         k_sortx_scatter<<<nb, kBlock, 0, stream>>>(J, pass);
     }
     return hipGetLastError();

@@ -62,10 +62,11 @@ struct GridBuf {
 // Stable LSD radix sort by cell key (the pipeline's reorder and the
 // stand-alone sort share it).  A pass ranks kSortTile keys per workgroup on
 // one digit of at most kSortMaxDigit bits: a per-(digit, workgroup) count, an
-// exclusive scan of those counts in (digit, workgroup) order, and a scatter
-// that ranks each key among the equal digits before it in index order.
+// exclusive scan of each digit's counts over the workgroups with the digit's
+// total beside it, and a scatter that adds the digits' scanned totals and
+// ranks each key among the equal digits before it in index order.
 // Keys of k bits take ceil(k / kSortMaxDigit) passes of equal width.
-constexpr int kSortItems = 16;                                 // keys per lane
+constexpr int kSortItems = 8;                                 // keys per lane
 constexpr uint32_t kSortTile = (uint32_t)kBlock * kSortItems;  // keys per workgroup
 constexpr uint32_t kSortMaxDigit = 10;                         // at most 1024 buckets
 struct SortPlan {
@@ -87,14 +88,20 @@ __host__ __device__ inline SortPlan sort_plan(uint32_t n_bins) { return sort_pla
 // Bits of the constitutive model above the cell key when the reorder groups
 // particles by model (per-material segments): model ids are below 16.
 constexpr uint32_t kModelKeyBits = 4;
-__host__ __device__ inline uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
-// Counts of one pass: (1 << digit) buckets x sort_tiles(n) workgroups.
+__host__ __device__ constexpr uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+// Counts of one pass, (1 << digit) buckets x sort_tiles(n) workgroups, then
+// the (1 << digit) digit totals.
 __host__ __device__ inline size_t sort_hist_len(uint32_t n, uint32_t n_bins) {
-    return ((size_t)1 << sort_plan(n_bins).digit) * (size_t)sort_tiles(n);
+    return ((size_t)1 << sort_plan(n_bins).digit) * ((size_t)sort_tiles(n) + 1u);
 }
-// Enough for any key width: at most 2^kSortMaxDigit buckets per workgroup.
-__host__ __device__ inline size_t sort_hist_len_max(uint32_t n) {
-    return ((size_t)1 << kSortMaxDigit) * (size_t)sort_tiles(n);
+// Enough for any key width: at most 2^kSortMaxDigit buckets.
+__host__ __device__ constexpr size_t sort_hist_len_max(uint32_t n) {
+    return ((size_t)1 << kSortMaxDigit) * ((size_t)sort_tiles(n) + 1u);
+}
+// Workgroups of a pass's scan: one wave per digit row.
+constexpr uint32_t kSortRowsPerGroup = (uint32_t)kBlock / 64u;
+__host__ __device__ inline uint32_t sort_scan_groups(uint32_t key_bits) {
+    return ((1u << sort_plan_bits(key_bits).digit) + kSortRowsPerGroup - 1u) / kSortRowsPerGroup;
 }
 
 struct SortBuf {
@@ -102,7 +109,7 @@ struct SortBuf {
     uint32_t* kbuf[2];  // N each: keys in the order after pass 0, 1 (kbuf[1] may alias key)
     uint32_t* vbuf[2];  // N each: the slot each of those keys came from
     uint32_t* out;      // N: the result (see perm)
-    uint32_t* hist;     // per-pass counts, then start slots (sort_hist_len_max(N) for the pipeline)
+    uint32_t* hist;     // per-pass counts, then start slots, and the digit totals (sort_hist_len*)
     uint32_t perm;      // 1: out[sorted position] = slot; 0: out[slot] = sorted position
     uint32_t model_bits;  // pipeline reorder: kModelKeyBits to sort by (model, cell key), 0 by cell key
 };
@@ -188,11 +195,12 @@ struct BatchView {
     uint32_t n;
 };
 struct BatchLists {
-    BatchView one;      // one workgroup per scene (step start, sort scans, control gather)
+    BatchView one;      // one workgroup per scene (step start, control gather)
     BatchView sort;     // sort_tiles(n_s) workgroups (cell-key sort counts and scatters)
+    BatchView scan;     // sort_scan_groups(key bits of s) workgroups (the sort's count scans)
     BatchView fused;    // chunks of the scenes without a density pass (fused substep)
     BatchView density;  // chunks of the scenes with one (density + P2G, then G2P + mass)
-    uint32_t sort_blocks, fused_blocks, density_blocks;
+    uint32_t sort_blocks, scan_blocks, fused_blocks, density_blocks;
     uint32_t sort_passes;  // the most radix passes any scene's key needs
 };
 // One launch per stage over every scene of the batch (the single-scene

@@ -21,8 +21,9 @@ namespace {
 
 constexpr uint32_t kMaxKeys = EMH_SORT_NOALLOC_MAX_KEYS;
 // the radix sort's scratch: two key and two slot buffers of kMaxKeys, then
-// the counts of one pass (at most 2^kSortMaxDigit buckets per workgroup)
-constexpr size_t kHistWords = ((size_t)1 << kSortMaxDigit) * (size_t)((kMaxKeys + kSortTile - 1) / kSortTile);
+// the counts of one pass (at most 2^kSortMaxDigit buckets per workgroup) and
+// its digit totals
+constexpr size_t kHistWords = sort_hist_len_max(kMaxKeys);
 constexpr size_t kScratchWords = 4 * (size_t)kMaxKeys + kHistWords;
 static_assert(kScratchWords * 4 == EMH_SORT_NOALLOC_SCRATCH_BYTES, "header and source disagree on the scratch size");
 

@@ -507,7 +507,7 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,
  * or threads) race, so the caller serialises them.  Nothing in the scratch
  * needs clearing between calls. */
 #define EMH_SORT_NOALLOC_MAX_KEYS 1048576u
-#define EMH_SORT_NOALLOC_SCRATCH_BYTES 17825792u /* (4 x 1048576 keys + 1024 x 256 counts) x 4 B */
+#define EMH_SORT_NOALLOC_SCRATCH_BYTES 18878464u /* (4 x 1048576 keys + 1024 x 513 counts, totals) x 4 B */
 int emh_sort_cell_keys_noalloc(const float* xy_dev, size_t n, uint32_t grid_res, uint32_t* keys_out_dev,
                                uint32_t* perm_out_dev, void* stream);
 
