@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 SUBSTEP_BYTES = 136
 ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 # HBM bytes per k_substep launch on this workload, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE (one run each, mean over launches) and scaled by
-# the same counters' reading for a 256 MiB device-to-device copy
-# (profiles/k_substep_sand_1m_512.md): 27728 KiB x 2.000 + 80394 KiB x 1.000.
-MEASURED_TRAFFIC = {"k_substep": (27728.2 * 2.000 + 80394.4 * 1.000) * 1024.0}
+# FETCH_SIZE and WRITE_SIZE (one run each, median over the launches) and
+# scaled by the same counters' reading for a 256 MiB device-to-device copy
+# (profiles/k_substep_sand_1m_512.md): 26150 KiB x 2.000 + 76468 KiB x 1.000.
+MEASURED_TRAFFIC = {"k_substep": (26149.625 * 2.000 + 76468.0 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu"):
@@ -140,6 +140,30 @@ def full_run_section(args, tuning) -> dict:
             "note": "includes the first step's upload, the physical reorders and the whole collapse"}
 
 
+def _hip_time_ms(fn, reps: int, stream: int) -> float:
+    """Milliseconds between two HIP events recorded on `stream` around `reps` calls of fn."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        if hip.hipEventCreate(ctypes.byref(e)):
+            raise RuntimeError("hipEventCreate failed")
+    try:
+        if hip.hipEventRecord(ev[0], ctypes.c_void_p(stream)):
+            raise RuntimeError("hipEventRecord failed")
+        for _ in range(reps):
+            fn()
+        if hip.hipEventRecord(ev[1], ctypes.c_void_p(stream)) or hip.hipEventSynchronize(ev[1]):
+            raise RuntimeError("hipEventRecord / hipEventSynchronize failed")
+        ms = ctypes.c_float()
+        if hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1]):
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+    finally:
+        for e in ev:
+            hip.hipEventDestroy(e)
+
+
 SORT_BYTES_PER_KEY = 16  # compulsory: the position read (8 B), the key and the permutation entry written
 
 
@@ -156,7 +180,7 @@ def sort_section(args) -> dict:
     x = np.ascontiguousarray(sim.particles()["x"], dtype=np.float32)
     sim.close()
     n, R = len(x), 512
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # a stream of its own (default priority), not the null stream
     keys = torch.empty(n, dtype=torch.int32, device="cuda")
     perm = torch.empty(n, dtype=torch.int32, device="cuda")
     out = {"workload": "cell_key_sort_1m_512", "keys": n, "grid_res": R,
@@ -164,6 +188,7 @@ def sort_section(args) -> dict:
     orders = {"generated_order": np.arange(n), "random_order": np.random.default_rng(11).permutation(n)}
     for name, order in orders.items():
         xy = torch.from_numpy(np.ascontiguousarray(x[order])).cuda()
+        torch.cuda.synchronize()  # the upload ran on torch's stream
 
         def run():
             eh.sort_cell_keys_noalloc(xy.data_ptr(), n, R, keys.data_ptr(), perm.data_ptr(), stream.cuda_stream)
@@ -175,13 +200,7 @@ def sort_section(args) -> dict:
         ks = k[p]
         dk, dp = np.diff(ks), np.diff(p)
         ok = bool(np.all(dk >= 0) and np.all((dk > 0) | (dp > 0)) and np.array_equal(np.sort(p), np.arange(n)))
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.sort_reps):
-            run()
-        e1.record(stream)
-        e1.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / args.sort_reps
+        us = _hip_time_ms(run, args.sort_reps, stream.cuda_stream) * 1e3 / args.sort_reps
         gbs = SORT_BYTES_PER_KEY * n / us / 1e3
         out[name] = {"us_per_sort": us, "keys_per_s": n / us * 1e6, "achieved_GBs": gbs,
                      "roofline_frac": gbs / HBM_PEAK_GBS, "stable_permutation": ok}

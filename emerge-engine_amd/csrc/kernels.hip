@@ -195,15 +195,18 @@ __device__ __forceinline__ void store_fields(const PBuf& b, uint32_t i, const PS
 }
 
 // Position part of project_particle_state_to_admissible (step.rs:357-362).
-__device__ __forceinline__ void project_x(float& x, float& y, const DevConfig& cfg) {
+__device__ __forceinline__ void project_x(float& x, float& y, float lo, float hi) {
     if (!(__builtin_isfinite(x) && __builtin_isfinite(y))) {
-        float c = (cfg.proj_lo + cfg.proj_hi) * 0.5f;
+        float c = (lo + hi) * 0.5f;
         x = c;
         y = c;
     } else {
-        x = rmin(rmax(x, cfg.proj_lo), cfg.proj_hi);
-        y = rmin(rmax(y, cfg.proj_lo), cfg.proj_hi);
+        x = rmin(rmax(x, lo), hi);
+        y = rmin(rmax(y, lo), hi);
     }
+}
+__device__ __forceinline__ void project_x(float& x, float& y, const DevConfig& cfg) {
+    project_x(x, y, cfg.proj_lo, cfg.proj_hi);
 }
 
 // Would projecting `s` change it?  (`sp` receives the projected state.)
@@ -716,9 +719,12 @@ __global__ void k_begin_step(const Pipeline p, uint64_t seq, uint32_t reorder) {
 // digits of its row comes from `bits` ballots (the lanes agreeing on every
 // bit), its rank among the wave's earlier rows from a wave-private LDS count
 // that the row's lowest such lane advances: no atomics, no arrival order.
+// Plain values only: a pointer into a kernel argument (say &p.cfg) would make
+// the compiler copy the whole argument to every lane's scratch.
 struct SortJob {
-    const float2* x;        // positions whose cell keys pass 0 computes
-    const DevConfig* proj;  // project those positions first (nullptr: no projection)
+    const float2* x;         // positions whose cell keys pass 0 computes
+    uint32_t project;        // clamp those positions to [proj_lo, proj_hi] first
+    float proj_lo, proj_hi;
     const uint32_t* mat;    // material words (model segments only)
     const DevMat* mats;     // material table (model segments only)
     uint32_t n_mats;
@@ -751,11 +757,15 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t digit, uint32_t bits, b
     return peers;
 }
 
+// Buffer k of a pair, by select: a runtime index into a SortJob built in
+// registers would send the whole job to scratch.
+__device__ __forceinline__ uint32_t* pick(uint32_t* const (&buf)[2], uint32_t k) { return k ? buf[1] : buf[0]; }
+
 // Key of slot i as pass `pass` reads it.
 __device__ __forceinline__ uint32_t sort_key_in(const SortJob& J, uint32_t pass, uint32_t i) {
-    if (pass > 0) return J.s.kbuf[(pass - 1u) & 1u][i];
+    if (pass > 0) return pick(J.s.kbuf, (pass - 1u) & 1u)[i];
     float2 x = J.x[i];
-    if (J.proj) project_x(x.x, x.y, *J.proj);
+    if (J.project) project_x(x.x, x.y, J.proj_lo, J.proj_hi);
     const uint32_t key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.tpd);
     if (!J.s.model_bits) return key;
     const uint32_t slot = J.mat[i] & kMatMask;  // the slot mat_of reads (slot 0 for an unregistered id)
@@ -896,9 +906,9 @@ __device__ __forceinline__ void sort_scatter_body(const SortJob& J, uint32_t pas
                    [&](uint32_t d) { return s_base[d] + hist[(size_t)d * nblk + blk]; }, [](uint32_t, uint32_t) {});
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6, mask = (1u << bits) - 1u;
     const uint32_t row0 = blk * kSortTile + w * (uint32_t)kSortItems * 64u;
-    const uint32_t* vin = pass > 0 ? J.s.vbuf[(pass - 1u) & 1u] : nullptr;
-    uint32_t* kout = J.s.kbuf[pass & 1u];
-    uint32_t* vout = J.s.vbuf[pass & 1u];
+    const uint32_t* vin = pass > 0 ? pick(J.s.vbuf, (pass - 1u) & 1u) : nullptr;
+    uint32_t* kout = pick(J.s.kbuf, pass & 1u);
+    uint32_t* vout = pick(J.s.vbuf, pass & 1u);
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
         const uint32_t i = row0 + (uint32_t)k * 64u + lane;
@@ -925,7 +935,9 @@ __device__ __forceinline__ SortJob pipeline_sort_job(const Pipeline& p, uint64_t
     SortJob J;
     const PBuf b = sel_buf(p, c->st[seq & 1u].cur_buf);
     J.x = b.x;
-    J.proj = project ? &p.cfg : nullptr;
+    J.project = project ? 1u : 0u;
+    J.proj_lo = p.cfg.proj_lo;
+    J.proj_hi = p.cfg.proj_hi;
     J.mat = b.mat;
     J.mats = p.mats;
     J.n_mats = p.n_mats;
@@ -1963,7 +1975,8 @@ hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res
     if (n == 0) return hipSuccess;
     SortJob J;
     J.x = xy;
-    J.proj = nullptr;
+    J.project = 0u;
+    J.proj_lo = J.proj_hi = 0.0f;
     J.n = n;
     J.grid_res = grid_res;
     J.tpd = (grid_res + 15u) / 16u;
