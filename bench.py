@@ -239,44 +239,47 @@ MIXED_BYTES = 128
 
 
 def multi_material_section(args) -> dict:
-    """The four-material scene in the default cell-key order against per-material
-    segments (the reorder groups particles by model, so each wave runs one
-    model's update), each from its upload, same steps; with the share of waves
-    that hold more than one model, where the update's per-model switch diverges."""
+    """The four-material scene (density pass every substep) from its upload:
+    the autotuner times reorder interval x hipGraph replay x per-material
+    segments (the reorder grouping particles by model, so each wave runs one
+    model's update) from the state after two steps and puts that state back;
+    then `--mixed-steps` steps are timed with its choice.  Reports the share of
+    waves that hold more than one model (where the update's per-model switch
+    diverges) in the chosen order, and every candidate's cost."""
     out = {"workload": "multi_material_4m_1024", "bytes_per_particle_substep": MIXED_BYTES}
-    for name, segments in (("cell_key_order", False), ("segments", True)):
-        sim = build_multi_material(segments)
-        n = sim.num_particles()
-        sim.step_n(2)
-        sim.synchronize()
-        # waves (64 consecutive device slots) holding more than one model after the upload's reorder
-        mid = sim.particles()["material_id"][sim.slot_order()]
-        nw = len(mid) // 64
-        w = mid[:nw * 64].reshape(nw, 64)
-        mixed_waves = float(np.mean(np.any(w != w[:, :1], axis=1)))
-        sim.set_profiling(True)
-        s0 = sim.stats().total_substeps
-        t0 = time.perf_counter()
-        sim.step_n(args.mixed_steps)
-        sim.synchronize()
-        elapsed = time.perf_counter() - t0
-        subs = sim.stats().total_substeps - s0
-        times = {k: v for k, v in sim.kernel_times().items() if v[0] > 0}
-        sim.set_profiling(False)
-        key = max((k for k in times if k != "step_setup"), key=lambda k: times[k][1])
-        launches, total_ms = times[key]
-        avg_s = total_ms / launches / 1e3
-        p = sim.particles()
-        sim.close()
-        out["particles"] = n
-        out[name] = {"particle_substeps_per_s": n * subs / elapsed, "substeps": subs, "seconds": elapsed,
-                     "kernels": key, "avg_substep_us": avg_s * 1e6,
-                     "roofline_frac": MIXED_BYTES * n / avg_s / 1e9 / HBM_PEAK_GBS,
-                     "mixed_model_waves": mixed_waves,
-                     "state_finite": bool(np.all(np.isfinite(p["x"])))}
-    out["steps"] = args.mixed_steps
-    out["segments_speedup"] = (out["segments"]["particle_substeps_per_s"] /
-                               out["cell_key_order"]["particle_substeps_per_s"])
+    sim = build_multi_material(False)
+    n = sim.num_particles()
+    sim.step_n(2)
+    sim.synchronize()
+    chosen, timed = sim.autotune(args.tune_steps)
+    sim.step_n(1)
+    sim.synchronize()
+    # waves (64 consecutive device slots) holding more than one model
+    mid = sim.particles()["material_id"][sim.slot_order()]
+    nw = len(mid) // 64
+    w = mid[:nw * 64].reshape(nw, 64)
+    mixed_waves = float(np.mean(np.any(w != w[:, :1], axis=1)))
+    sim.set_profiling(True)
+    s0 = sim.stats().total_substeps
+    t0 = time.perf_counter()
+    sim.step_n(args.mixed_steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    subs = sim.stats().total_substeps - s0
+    times = {k: v for k, v in sim.kernel_times().items() if v[0] > 0}
+    sim.set_profiling(False)
+    key = max((k for k in times if k != "step_setup"), key=lambda k: times[k][1])
+    launches, total_ms = times[key]
+    avg_s = total_ms / launches / 1e3
+    p = sim.particles()
+    sim.close()
+    best = {seg: min(t["ns_per_particle_substep"] for t in timed if t["model_segments"] == seg) for seg in (0, 1)}
+    out.update({"particles": n, "steps": args.mixed_steps, "particle_substeps_per_s": n * subs / elapsed,
+                "substeps": subs, "seconds": elapsed, "kernels": key, "avg_substep_us": avg_s * 1e6,
+                "roofline_frac": MIXED_BYTES * n / avg_s / 1e9 / HBM_PEAK_GBS, "mixed_model_waves": mixed_waves,
+                "state_finite": bool(np.all(np.isfinite(p["x"]))),
+                "tuning": {"chosen": chosen, "candidates": timed, "trial_steps": args.tune_steps},
+                "segments_speedup": best[0] / best[1]})
     return out
 
 

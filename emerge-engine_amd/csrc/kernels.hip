@@ -396,9 +396,14 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
     const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
     const float m = p.grid.acc[set][kAccM][gi];
     if (!(m > 0.0f)) return make_float2(0.0f, 0.0f);
-    // momentum = A + (dt - dt') S (ddt = dt - dt' is 0 when the prediction held)
-    float vx = p.grid.acc[set][kAccPx][gi] + ddt * p.grid.acc[set][kAccSx][gi];
-    float vy = p.grid.acc[set][kAccPy][gi] + ddt * p.grid.acc[set][kAccSy][gi];
+    // momentum = A + (dt - dt') S (ddt = dt - dt' is 0 when the prediction held,
+    // and always for scenes with a density pass, which never write S)
+    float vx = p.grid.acc[set][kAccPx][gi];
+    float vy = p.grid.acc[set][kAccPy][gi];
+    if (ddt != 0.0f) {
+        vx = vx + ddt * p.grid.acc[set][kAccSx][gi];
+        vy = vy + ddt * p.grid.acc[set][kAccSy][gi];
+    }
     vx = vx / m;
     vy = vy / m;
     vx = vx + cfg.gx * dt;
@@ -1114,11 +1119,15 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
     return t;
 }
 
-// All five channels, one at a time (keeps 9 partial sums live, not 45).
-// Terms below 1e18 in magnitude keep every stencil value below 1e37 and every
-// partial sum of 64 of them finite, so the fma form of the scan is exact.
+// The first NCH channels, one at a time (keeps 9 partial sums live, not 45):
+// all five, or mass and momentum alone when the dt the terms were formed with
+// is the substep's own (the density pass).  Terms below 1e18 in magnitude
+// keep every stencil value below 1e37 and every partial sum of 64 of them
+// finite, so the fma form of the scan is exact.
+template <int NCH = kAccChannels>
 __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
                                             float (*s_acc)[kLdsCells], float* const* g, int R) {
+    static_assert(NCH == kAccChannels || NCH == kAccPy + 1, "all channels, or mass and momentum");
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
     const ScanPred sq = scan_pred(sm);
     float big = fmaxf(fabsf(t.mass), fmaxf(fabsf(t.ux), fabsf(t.uy)));
@@ -1131,8 +1140,10 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
     scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccM], g[kAccM], R);
     scatter_channel<kAccPx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPx], g[kAccPx], R);
     scatter_channel<kAccPy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPy], g[kAccPy], R);
-    scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSx], g[kAccSx], R);
-    scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSy], g[kAccSy], R);
+    if (NCH == kAccChannels) {
+        scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSx], g[kAccSx], R);
+        scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSy], g[kAccSy], R);
+    }
 }
 
 // Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
@@ -1503,9 +1514,10 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         const uint32_t R2 = (uint32_t)R * (uint32_t)R;
         const uint32_t per = (R2 + nblk - 1) / nblk;
         const uint32_t lo = blk * per, hi = min(R2, lo + per);
+        // (a density-pass scene leaves the S channels zero from the start)
+        const int nch = cfg.density_pass ? kAccPy + 1 : kAccChannels;
         for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-#pragma unroll
-            for (int ch = 0; ch < kAccChannels; ++ch) p.grid.acc[old][ch][i] = 0.0f;
+            for (int ch = 0; ch < nch; ++ch) p.grid.acc[old][ch][i] = 0.0f;
             if (cfg.density_pass) p.grid.dm[old][i] = 0.0f;
         }
     }
@@ -1530,7 +1542,11 @@ __global__ __launch_bounds__(kBlock, (MODE & kModeSurrogate) ? 2 : 4) void k_sub
 // read the substep's state, so it also projects (and persists) it if flagged.
 template <bool BATCH, bool SLEEP>
 __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq, uint32_t blk, uint32_t nblk) {
-    __shared__ float s_acc[kAccChannels][kLdsCells];
+    // dt is decided here, before the terms are formed: the S channels (the
+    // correction for a mispredicted dt) stay empty, and the grid update of a
+    // density-pass scene never reads them
+    constexpr int kDensCh = kAccPy + 1;
+    __shared__ float s_acc[kDensCh][kLdsCells];
     __shared__ float s_dm[kLdsCells];
     __shared__ int s_box[4 * kWaves];
     Ctrl* c = p.ctrl;
@@ -1548,15 +1564,15 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     const uint32_t chunk = BATCH ? blk : xcd_chunk(blk, nblk);
     const uint32_t base = chunk * kChunk;
     const float* dm = p.grid.dm[cur];
-    float* gdst[kAccChannels];
+    float* gdst[kDensCh];
 #pragma unroll
-    for (int ch = 0; ch < kAccChannels; ++ch) gdst[ch] = p.grid.acc[cur][ch];
+    for (int ch = 0; ch < kDensCh; ++ch) gdst[ch] = p.grid.acc[cur][ch];
 
     const Box box = chunk_box(p, a, base, project_in, 1, s_box);
     if (box.lds) {
         for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
 #pragma unroll
-            for (int ch = 0; ch < kAccChannels; ++ch) s_acc[ch][i] = 0.0f;
+            for (int ch = 0; ch < kDensCh; ++ch) s_acc[ch][i] = 0.0f;
             const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
             s_dm[i] = (cx >= 0 && cy >= 0 && cx < R && cy < R) ? dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] : 0.0f;
         }
@@ -1604,9 +1620,9 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
             }
         }
         const P2GTerms t = p2g_terms<-1>(p, s, N.dt);
-        scatter_p2g(t, valid && !asleep, box, s_acc, gdst, R);
+        scatter_p2g<kDensCh>(t, valid && !asleep, box, s_acc, gdst, R);
     }
-    flush_box<kAccChannels>(box, s_acc, gdst, R);
+    flush_box<kDensCh>(box, s_acc, gdst, R);
     const uint32_t wp = wave_sum(n_proj);
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
