@@ -896,6 +896,9 @@ struct emh_sim {
         if (trial == 0) return fail(EMH_ERR_INVALID_ARG, "trial_steps must be at least 1");
         if (profiling) return fail(EMH_ERR_STATE, "turn profiling off before autotuning");
         if (pipe.n == 0 && host.empty()) return fail(EMH_ERR_STATE, "no particles to tune on");
+        if (sg_mode != 0)
+            return fail(EMH_ERR_STATE, "switch the surrogate mode to plain before autotuning (the trial steps "
+                                       "would add harvest records or counts to the surrogate)");
         int rc = host_current();
         if (rc) return rc;
         const std::vector<emh_particle> saved = host;

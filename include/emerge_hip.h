@@ -467,7 +467,10 @@ int emh_sim_kernel_times(emh_sim* sim, emh_kernel_time* out, uint32_t cap, uint3
  * steps timed on the host clock to their completion; it then puts the state
  * back (the particles are re-uploaded at the next step, the step counters
  * restored), applies the fastest candidate and reports it in *chosen (and
- * every candidate's cost in timed[0 .. min(cap, *n_timed))). */
+ * every candidate's cost in timed[0 .. min(cap, *n_timed))).  A scene in a
+ * surrogate mode (harvest or inference) is refused with EMH_ERR_STATE: its
+ * trial steps would add records or counts to the surrogate that the restore
+ * cannot take back. */
 typedef struct emh_tuning {
     uint32_t reorder_interval;
     uint32_t graphs;

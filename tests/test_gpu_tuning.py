@@ -144,3 +144,13 @@ def test_tuning_errors():
     with pytest.raises(eh.EmergeError):
         sim.autotune(trial_steps=0)
     sim.close()
+    # a scene harvesting for a surrogate: trial steps would add records the restore cannot remove
+    nacc, sg = S.SCENES["nacc_block"].build("gpu"), eh.NaccSurrogate(1)
+    nacc.set_surrogate_mode(sg, "harvest", 4 * nacc.num_particles(), 1)
+    with pytest.raises(eh.EmergeError, match="surrogate"):
+        nacc.autotune(trial_steps=1)
+    assert sg.harvest_count()[0] == 0
+    nacc.set_surrogate_mode(None, "plain")
+    nacc.autotune(trial_steps=1)
+    nacc.close()
+    sg.close()
