@@ -140,28 +140,64 @@ def full_run_section(args, tuning) -> dict:
             "note": "includes the first step's upload, the physical reorders and the whole collapse"}
 
 
-def _hip_time_ms(fn, reps: int, stream: int) -> float:
-    """Milliseconds between two HIP events recorded on `stream` around `reps` calls of fn."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
-    for e in ev:
-        if hip.hipEventCreate(ctypes.byref(e)):
-            raise RuntimeError("hipEventCreate failed")
-    try:
-        if hip.hipEventRecord(ev[0], ctypes.c_void_p(stream)):
-            raise RuntimeError("hipEventRecord failed")
-        for _ in range(reps):
-            fn()
-        if hip.hipEventRecord(ev[1], ctypes.c_void_p(stream)) or hip.hipEventSynchronize(ev[1]):
-            raise RuntimeError("hipEventRecord / hipEventSynchronize failed")
-        ms = ctypes.c_float()
-        if hip.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1]):
-            raise RuntimeError("hipEventElapsedTime failed")
-        return float(ms.value)
-    finally:
+class _Hip:
+    """The HIP runtime the library itself loaded (libamdhip64), through ctypes:
+    device buffers, a stream and events for the stand-alone sort."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.lib = ctypes.CDLL("libamdhip64.so")
+
+    def check(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def malloc(self, nbytes: int) -> int:
+        p = self.ct.c_void_p()
+        self.check(self.lib.hipMalloc(self.ct.byref(p), self.ct.c_size_t(nbytes)), "hipMalloc")
+        return p.value
+
+    def free(self, p: int) -> None:
+        self.lib.hipFree(self.ct.c_void_p(p))
+
+    def h2d(self, dst: int, a: np.ndarray) -> None:
+        self.check(self.lib.hipMemcpy(self.ct.c_void_p(dst), self.ct.c_void_p(a.ctypes.data),
+                                      self.ct.c_size_t(a.nbytes), 1), "hipMemcpy H2D")
+
+    def d2h(self, a: np.ndarray, src: int) -> None:
+        self.check(self.lib.hipMemcpy(self.ct.c_void_p(a.ctypes.data), self.ct.c_void_p(src),
+                                      self.ct.c_size_t(a.nbytes), 2), "hipMemcpy D2H")
+
+    def stream(self) -> int:
+        s = self.ct.c_void_p()
+        self.check(self.lib.hipStreamCreateWithFlags(self.ct.byref(s), 1), "hipStreamCreateWithFlags")
+        return s.value
+
+    def destroy_stream(self, s: int) -> None:
+        self.lib.hipStreamDestroy(self.ct.c_void_p(s))
+
+    def sync(self, s: int) -> None:
+        self.check(self.lib.hipStreamSynchronize(self.ct.c_void_p(s)), "hipStreamSynchronize")
+
+    def time_ms(self, fn, reps: int, stream: int) -> float:
+        """Milliseconds between two events recorded on `stream` around `reps` calls of fn."""
+        ct, hip = self.ct, self.lib
+        ev = [ct.c_void_p(), ct.c_void_p()]
         for e in ev:
-            hip.hipEventDestroy(e)
+            self.check(hip.hipEventCreate(ct.byref(e)), "hipEventCreate")
+        try:
+            self.check(hip.hipEventRecord(ev[0], ct.c_void_p(stream)), "hipEventRecord")
+            for _ in range(reps):
+                fn()
+            self.check(hip.hipEventRecord(ev[1], ct.c_void_p(stream)), "hipEventRecord")
+            self.check(hip.hipEventSynchronize(ev[1]), "hipEventSynchronize")
+            ms = ct.c_float()
+            self.check(hip.hipEventElapsedTime(ct.byref(ms), ev[0], ev[1]), "hipEventElapsedTime")
+            return float(ms.value)
+        finally:
+            for e in ev:
+                hip.hipEventDestroy(e)
 
 
 SORT_BYTES_PER_KEY = 16  # compulsory: the position read (8 B), the key and the permutation entry written
@@ -175,35 +211,39 @@ def sort_section(args) -> dict:
     on the stream the sorts are enqueued on.  Every result is checked on the host
     (keys non-decreasing along the permutation, equal keys in increasing slot
     order, the permutation a bijection)."""
-    import torch
     sim = build_workload("gpu")
     x = np.ascontiguousarray(sim.particles()["x"], dtype=np.float32)
     sim.close()
     n, R = len(x), 512
-    stream = torch.cuda.Stream()  # a stream of its own (default priority), not the null stream
-    keys = torch.empty(n, dtype=torch.int32, device="cuda")
-    perm = torch.empty(n, dtype=torch.int32, device="cuda")
+    hip = _Hip()
+    stream = hip.stream()
+    d_xy, d_keys, d_perm = hip.malloc(8 * n), hip.malloc(4 * n), hip.malloc(4 * n)
     out = {"workload": "cell_key_sort_1m_512", "keys": n, "grid_res": R,
            "bytes_per_key": SORT_BYTES_PER_KEY, "reps": args.sort_reps}
-    orders = {"generated_order": np.arange(n), "random_order": np.random.default_rng(11).permutation(n)}
-    for name, order in orders.items():
-        xy = torch.from_numpy(np.ascontiguousarray(x[order])).cuda()
-        torch.cuda.synchronize()  # the upload ran on torch's stream
+    try:
+        orders = {"generated_order": np.arange(n), "random_order": np.random.default_rng(11).permutation(n)}
+        for name, order in orders.items():
+            hip.h2d(d_xy, np.ascontiguousarray(x[order]))
 
-        def run():
-            eh.sort_cell_keys_noalloc(xy.data_ptr(), n, R, keys.data_ptr(), perm.data_ptr(), stream.cuda_stream)
+            def run():
+                eh.sort_cell_keys_noalloc(d_xy, n, R, d_keys, d_perm, stream)
 
-        run()
-        torch.cuda.synchronize()
-        k = keys.cpu().numpy().view(np.uint32).astype(np.int64)
-        p = perm.cpu().numpy().view(np.uint32).astype(np.int64)
-        ks = k[p]
-        dk, dp = np.diff(ks), np.diff(p)
-        ok = bool(np.all(dk >= 0) and np.all((dk > 0) | (dp > 0)) and np.array_equal(np.sort(p), np.arange(n)))
-        us = _hip_time_ms(run, args.sort_reps, stream.cuda_stream) * 1e3 / args.sort_reps
-        gbs = SORT_BYTES_PER_KEY * n / us / 1e3
-        out[name] = {"us_per_sort": us, "keys_per_s": n / us * 1e6, "achieved_GBs": gbs,
-                     "roofline_frac": gbs / HBM_PEAK_GBS, "stable_permutation": ok}
+            run()
+            hip.sync(stream)
+            k, p = np.empty(n, np.uint32), np.empty(n, np.uint32)
+            hip.d2h(k, d_keys)
+            hip.d2h(p, d_perm)
+            ks, pi = k.astype(np.int64)[p], p.astype(np.int64)
+            dk, dp = np.diff(ks), np.diff(pi)
+            ok = bool(np.all(dk >= 0) and np.all((dk > 0) | (dp > 0)) and np.array_equal(np.sort(pi), np.arange(n)))
+            us = hip.time_ms(run, args.sort_reps, stream) * 1e3 / args.sort_reps
+            gbs = SORT_BYTES_PER_KEY * n / us / 1e3
+            out[name] = {"us_per_sort": us, "keys_per_s": n / us * 1e6, "achieved_GBs": gbs,
+                         "roofline_frac": gbs / HBM_PEAK_GBS, "stable_permutation": ok}
+    finally:
+        for d in (d_xy, d_keys, d_perm):
+            hip.free(d)
+        hip.destroy_stream(stream)
     return out
 
 
@@ -508,8 +548,11 @@ def batch_section(args) -> dict:
     batch.step_n(1)
     tuning = None
     if not args.no_autotune:
-        chosen, timed = batch.autotune(3)
-        tuning = {"chosen": chosen, "candidates": timed, "trial_steps": 3}
+        # trials as long as the other sections': over fewer steps the decay of
+        # the particle order between reorders does not show, and every
+        # candidate times the same
+        chosen, timed = batch.autotune(args.tune_steps)
+        tuning = {"chosen": chosen, "candidates": timed, "trial_steps": args.tune_steps}
         batch.step_n(1)  # the re-upload after the trials, outside the timed region
         for s in alone:
             s.set_tuning(chosen["reorder_interval"], bool(chosen["graphs"]))
