@@ -1261,8 +1261,8 @@ __device__ __forceinline__ void harvest_nacc(const SgDev& sg, const DevMat& m, c
 }
 
 // update_particle with the NACC return mapping through the learned
-// surrogate (nacc.rs:255-269 otherwise unchanged).  Every lane of the wave
-// calls this.
+// surrogate (nacc.rs:255-269 otherwise unchanged).  Every lane of the
+// workgroup calls this.
 template <int ONLY>
 __device__ __forceinline__ void surrogate_update(const SgDev& sg, const SgLds& net, const DevMat& m, bool valid,
                                                  PState& s, float dt, float* s_io) {
@@ -1271,7 +1271,7 @@ __device__ __forceinline__ void surrogate_update(const SgDev& sg, const SgLds& n
     if (nacc && (m.flags & kMatNaccAdvance)) fin = mmul(madd(M2{1.0f, 0.0f, 0.0f, 1.0f}, mscale(s.c, dt)), fin);
     M2 nf;
     float na;
-    sg_return_wave(m, sg, net, nacc, fin, s.lvs, nf, na, s_io);
+    sg_return<1>(m, sg, net, nacc, fin, s.lvs, nf, na, s_io);
     if (nacc) {
         s.f = nf;
         s.lvs = na;
@@ -1297,7 +1297,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     constexpr bool SLEEP = (MODE & kModeSleep) != 0;
     static_assert(!(SURR && SLEEP), "the surrogate substep has no sleep partition");
     constexpr int NCH = P2G ? kAccChannels : 1;
-    __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgWaveLds : 1];
+    __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgQueueLds<1> : 1];
     __shared__ std::conditional_t<SURR, SgLds, char> s_net;
     __shared__ float s_acc[NCH][kLdsCells];
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];

@@ -293,7 +293,8 @@ __global__ __launch_bounds__(kSgBlock, 4) void k_nacc_return_surrogate(DevMat m,
                                                                     const float* __restrict__ alpha,
                                                                     unsigned long long n, float4* __restrict__ fo,
                                                                     float* __restrict__ ao, SgDev sg) {
-    __shared__ float s_io[kSgBlock / 64][kSgWaveLds];
+    static_assert(kSgBlock == kSgBlockLanes, "sg_return<4> queues one workgroup of kSgBlockLanes lanes");
+    __shared__ float s_io[kSgQueueLds<4>];
     __shared__ SgLds s_net;
     sg_stage(sg.pk, sg.bias, s_net);  // once per workgroup: the grid strides over the particles
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(kSgBlock, 4) void k_nacc_return_surrogate(DevMat m,
         }
         M2 nf;
         float na;
-        sg_return_wave(m, sg, s_net, valid, fin, a, nf, na, s_io[threadIdx.x >> 6]);
+        sg_return<4>(m, sg, s_net, valid, fin, a, nf, na, s_io);
         if (valid) {
             fo[i] = make_float4(nf.a, nf.b, nf.c, nf.d);
             ao[i] = na;

@@ -10,24 +10,39 @@
 
 namespace emh {
 
-// Floats per wave of the LDS exchange buffer sg_return_wave needs.
-constexpr int kSgWaveLds = 64 * 12;
+// Floats of LDS the network queue of sg_return<POOL> needs: 12 per lane of
+// the POOL waves that share it.
+constexpr int kSgBlockLanes = 256;  // the lanes of a workgroup (kernels using POOL = 4)
+template <uint32_t POOL>
+constexpr int kSgQueueLds = (int)POOL * 64 * 12;
 
-// NaccMaterial::project (nacc.rs:129-223) for the 64 particles of a wave,
-// with the exact yield-function check as the gate: a state inside the yield
-// surface keeps F; the tension and compression caps (Cases A and B, closed
-// forms of one sqrt and one pow) keep the plain return; a state outside the
-// yield surface between them (Case C, the projection onto the surface with
-// the hardening update) goes through the network when its singular values
-// are positive and every normalised input lies within z_max, and through the
-// plain return mapping (nacc_return) otherwise.
-// Every lane of the wave calls this (it runs MFMAs over compacted lanes and
-// synchronises the workgroup); `valid` marks the lanes with a particle.
-// net: the network staged in LDS (sg_stage); s_io: this wave's kSgWaveLds
-// floats of LDS.
-__device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg, const SgLds& net, bool valid,
-                                               const M2& fin, float alpha, M2& fo, float& ao, float* s_io) {
-    const uint32_t lane = threadIdx.x & 63u, c = lane & 15u, g = lane >> 4;
+// NaccMaterial::project (nacc.rs:129-223) with the exact yield-function check
+// as the gate: a state inside the yield surface keeps F; the tension and
+// compression caps (Cases A and B, closed forms of one sqrt and one pow) keep
+// the plain return; a state outside the yield surface between them (Case C,
+// the projection onto the surface with the hardening update) goes through the
+// network when its singular values are positive and every normalised input
+// lies within z_max, and through the plain return mapping (nacc_return)
+// otherwise.
+// The network's lanes of POOL waves are queued together in LDS and the queue
+// is evaluated in groups of 16 (one MFMA column group), dealt to the POOL
+// waves in turn, so the groups are full except the last.  POOL = 4 (the
+// workgroup) for the batch return mapping: about 41 of 256 lanes reach the
+// network in the NACC slope scene, 3 groups where four separate waves run 4
+// part-filled ones (112-117 -> 87-89 us per 1.97M states).  POOL = 1 inside
+// the substep, where waiting for the workgroup's other waves costs more than
+// the fuller groups save (284-287 us per substep with one-wave queues against
+// 295-321 us with workgroup queues).
+// Every lane of the workgroup calls this (it synchronises the workgroup);
+// `valid` marks the lanes with a particle.  net: the network staged in LDS
+// (sg_stage); s_io: kSgQueueLds<POOL> floats of LDS, this wave's own (POOL = 1)
+// or the workgroup's (POOL = 4).
+template <uint32_t POOL>
+__device__ __forceinline__ void sg_return(const DevMat& m, const SgDev& sg, const SgLds& net, bool valid,
+                                          const M2& fin, float alpha, M2& fo, float& ao, float* s_io) {
+    static_assert(POOL == 1 || POOL * 64 == kSgBlockLanes, "one wave or the whole workgroup");
+    __shared__ uint32_t s_wcnt[POOL];
+    const uint32_t lane = threadIdx.x & 63u, w = POOL == 1 ? 0u : threadIdx.x >> 6, c = lane & 15u, g = lane >> 4;
     M2 u{1.0f, 0.0f, 0.0f, 1.0f}, vt{1.0f, 0.0f, 0.0f, 1.0f};
     NaccTrial tr{};
     tr.yield = 0;
@@ -52,17 +67,32 @@ __device__ __forceinline__ void sg_return_wave(const DevMat& m, const SgDev& sg,
             }
         }
     }
+    // queue slot: the network lanes of the waves before this one, then this
+    // wave's in lane order
     const uint64_t mask = __ballot(use);
     const uint32_t cnt = (uint32_t)__popcll(mask);
-    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t off = 0, total = cnt;
+    if (POOL > 1) {
+        if (lane == 0) s_wcnt[w] = cnt;
+        __syncthreads();
+        total = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < POOL; ++k) {
+            const uint32_t v = s_wcnt[k];
+            off += k < w ? v : 0u;
+            total += v;
+        }
+    }
+    const uint32_t rank = off + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     if (use) {
 #pragma unroll
         for (int k = 0; k < kSgIn; ++k) s_io[rank * 12 + k] = (x[k] - sg.norm[kSgInMu + k]) / sg.norm[kSgInSd + k];
     }
     __syncthreads();
-    for (uint32_t g0 = 0; g0 < cnt; g0 += 16) {  // wave-uniform
-        const uint32_t slot = g0 + c;
-        const bool mine = g == 0 && slot < cnt;
+    const uint32_t groups = (total + 15u) / 16u;
+    for (uint32_t grp = w; grp < groups; grp += POOL) {  // wave-uniform
+        const uint32_t slot = grp * 16u + c;
+        const bool mine = g == 0 && slot < total;
         float z[kSgIn] = {};
         if (mine) {
 #pragma unroll
