@@ -609,11 +609,11 @@ struct emh_sim {
             pb.mat = (uint32_t*)take(sizeof(uint32_t) * c);
             pb.id = (uint32_t*)take(sizeof(uint32_t) * c);
         }
-        // the reorder's radix sort: pass 1 writes its keys over the input-order
-        // keys of pass 0, which no later pass reads
-        pipe.sort.key = (uint32_t*)take(sizeof(uint32_t) * c);
+        // the reorder's radix sort; nothing reads its input-order keys, so
+        // pass 0 does not write them (key = null)
+        pipe.sort.key = nullptr;
         pipe.sort.kbuf[0] = (uint32_t*)take(sizeof(uint32_t) * c);
-        pipe.sort.kbuf[1] = pipe.sort.key;
+        pipe.sort.kbuf[1] = (uint32_t*)take(sizeof(uint32_t) * c);
         pipe.sort.vbuf[0] = (uint32_t*)take(sizeof(uint32_t) * c);
         pipe.sort.vbuf[1] = (uint32_t*)take(sizeof(uint32_t) * c);
         pipe.sort.out = (uint32_t*)take(sizeof(uint32_t) * c);

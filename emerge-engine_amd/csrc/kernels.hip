@@ -800,7 +800,7 @@ __device__ __forceinline__ void sort_rank_tile(const SortJob& J, uint32_t pass, 
         const uint32_t i = row0 + (uint32_t)k * 64u + lane;
         const bool valid = i < J.n;
         key[k] = valid ? sort_key_in(J, pass, i) : 0u;
-        if (pass == 0 && valid) J.s.key[i] = key[k];
+        if (pass == 0 && valid && J.s.key) J.s.key[i] = key[k];
         const uint32_t d = (key[k] >> shift) & mask;
         const uint64_t peers = digit_peers(d, bits, valid);
         const uint32_t before = (uint32_t)__popcll(peers & below);

@@ -105,8 +105,8 @@ __host__ __device__ inline uint32_t sort_scan_groups(uint32_t key_bits) {
 }
 
 struct SortBuf {
-    uint32_t* key;      // N: cell key of the particle in each slot (written by pass 0)
-    uint32_t* kbuf[2];  // N each: keys in the order after pass 0, 1 (kbuf[1] may alias key)
+    uint32_t* key;      // N: cell key of the particle in each slot (written by pass 0 unless null)
+    uint32_t* kbuf[2];  // N each: keys in the order after pass 0, 1
     uint32_t* vbuf[2];  // N each: the slot each of those keys came from
     uint32_t* out;      // N: the result (see perm)
     uint32_t* hist;     // per-pass counts, then start slots, and the digit totals (sort_hist_len*)
