@@ -47,12 +47,21 @@ ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 MEASURED_TRAFFIC = {"k_substep": (26111.6875 * 2.000 + 76468.125 * 1.000) * 1024.0}
 
 
-def build_workload(backend: str = "gpu"):
+def build_workload(backend: str = "gpu", geometry: str = "column"):
     """The 1M-particle sand column on the GPU ("gpu") or the CPU oracle ("oracle" = f32,
-    "oracle64" = the FP64 oracle)."""
+    "oracle64" = the FP64 oracle).  geometry "column": 1,048,576 particles at 16 per
+    cell in a 256 x 256-cell column on the floor (the headline); "interior": the other
+    way to hold ~1M particles at 512^2, 4 per cell filling the whole 508 x 508-cell
+    interior inside the slip walls (1,032,256 particles, jitter +-0.25 cell)."""
     cfg = eh.SimConfig.standard(512, 1.0 / 60.0, (0.0, -9.81))
-    spawn = eh.SpawnRegion(spacing=0.25, box_size=(256, 256), box_center=(256.0, 130.0), position_jitter=1.0,
-                           rng_seed=42)
+    if geometry == "column":
+        spawn = eh.SpawnRegion(spacing=0.25, box_size=(256, 256), box_center=(256.0, 130.0), position_jitter=1.0,
+                               rng_seed=42)
+    elif geometry == "interior":
+        spawn = eh.SpawnRegion(spacing=0.5, box_size=(508, 508), box_center=(256.0, 256.0), position_jitter=0.5,
+                               rng_seed=42)
+    else:
+        raise ValueError(f"unknown geometry {geometry!r}")
     if backend == "gpu":
         lam, mu = eh.lame_from_young(3.5e5, 0.3)
         mat = eh.DruckerPragerMaterial.new(lam, mu)
@@ -198,6 +207,36 @@ class _Hip:
         finally:
             for e in ev:
                 hip.hipEventDestroy(e)
+
+
+def interior_section(args, tuning) -> dict:
+    """The same material and grid at 4 particles per cell over the whole interior
+    (1,032,256 particles in 508 x 508 cells): `--interior-steps` steps after 20,
+    with the headline's tuned launch settings and the substep kernel's live mean."""
+    sim = build_workload("gpu", "interior")
+    n = sim.num_particles()
+    if tuning is not None:
+        sim.set_tuning(tuning["chosen"]["reorder_interval"], bool(tuning["chosen"]["graphs"]))
+    sim.step_n(20)
+    sim.synchronize()
+    sim.set_profiling(True)
+    s0 = sim.stats().total_substeps
+    t0 = time.perf_counter()
+    sim.step_n(args.interior_steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    substeps = sim.stats().total_substeps - s0
+    times = {k: v for k, v in sim.kernel_times().items() if v[0] > 0}
+    sim.set_profiling(False)
+    p = sim.particles()
+    sim.close()
+    launches, total_ms = times["k_substep"]
+    avg_s = total_ms / launches / 1e3
+    achieved = SUBSTEP_BYTES * n / avg_s / 1e9
+    return {"workload": "drucker_prager_interior_4ppc_512", "particles": n, "steps": args.interior_steps,
+            "substeps": substeps, "particle_substeps_per_s": n * substeps / elapsed,
+            "k_substep_avg_us": avg_s * 1e6, "roofline_frac": achieved / HBM_PEAK_GBS,
+            "state_finite": bool(np.all(np.isfinite(p["x"])))}
 
 
 SORT_BYTES_PER_KEY = 16  # compulsory: the position read (8 B), the key and the permutation entry written
@@ -620,6 +659,8 @@ def main() -> int:
                     help="the whole collapse from rest, timed as one run (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=20,
                     help="timed steps of the four-material 1024^2 scene, per ordering (0: skip)")
+    ap.add_argument("--interior-steps", type=int, default=50,
+                    help="timed steps of the 4-per-cell whole-interior geometry (0: skip)")
     ap.add_argument("--sort-reps", type=int, default=50,
                     help="timed sorts of the headline positions, per input order (0: skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes section")
@@ -715,6 +756,7 @@ def main() -> int:
     sections = [("cpu_baseline", not args.no_cpu_baseline, lambda: cpu_baseline(args.cpu_seconds, first_dt)),
                 ("cell_sort", args.sort_reps > 0, lambda: sort_section(args)),
                 ("full_run", args.full_run_steps > 0, lambda: full_run_section(args, tuning)),
+                ("interior_4ppc", args.interior_steps > 0, lambda: interior_section(args, tuning)),
                 ("multi_material", args.mixed_steps > 0, lambda: multi_material_section(args)),
                 ("nacc_surrogate", not args.no_surrogate, lambda: surrogate_section(args)),
                 ("batched_scenes", not args.no_batch, lambda: batch_section(args))]
