@@ -603,9 +603,12 @@ def batch_section(args) -> dict:
     for s in alone:
         s.step_n(args.batch_steps)
     t_alone = time.perf_counter() - t0
+    batch.set_profiling(True)  # per-stage HIP-event spans, recorded while the steps run
     t0 = time.perf_counter()
     batch.step_n(args.batch_steps)
     t_batch = time.perf_counter() - t0
+    stages = {k: {"launches": v[0], "ms": v[1]} for k, v in batch.kernel_times().items()}
+    batch.set_profiling(False)
     sa = np.array([s.stats().total_substeps for s in alone]) - sa0
     sb = np.array([s.stats().total_substeps for s in together]) - sb0
     dx = 0.0
@@ -624,7 +627,7 @@ def batch_section(args) -> dict:
         "substeps_per_scene_min_max": [int(sb.min()), int(sb.max())],
         "substeps_equal": bool(np.array_equal(sa, sb)),
         "max_abs_dx_cells": dx,
-        "batch": {"particle_substeps_per_s": work_b / t_batch, "seconds": t_batch},
+        "batch": {"particle_substeps_per_s": work_b / t_batch, "seconds": t_batch, "stages": stages},
         "separate": {"particle_substeps_per_s": work_a / t_alone, "seconds": t_alone},
         "speedup": (work_b / t_batch) / (work_a / t_alone),
         "build_s": build_s,

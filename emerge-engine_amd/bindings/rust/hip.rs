@@ -350,6 +350,8 @@ extern "C" {
     fn emh_batch_set_tuning(b: *mut EmhBatch, t: *const EmhTuning) -> c_int;
     fn emh_batch_autotune(b: *mut EmhBatch, trial_steps: u32, chosen: *mut EmhTuning, timed: *mut EmhTuning,
                           timed_cap: u32, n_timed: *mut u32) -> c_int;
+    fn emh_batch_set_profiling(b: *mut EmhBatch, enable: c_int) -> c_int;
+    fn emh_batch_kernel_times(b: *mut EmhBatch, out: *mut EmhKernelTime, cap: u32, n_out: *mut u32) -> c_int;
     fn emh_surrogate_create(seed: u64, out: *mut *mut EmhSurrogate) -> c_int;
     fn emh_surrogate_destroy(sg: *mut EmhSurrogate) -> c_int;
     fn emh_surrogate_get_params(sg: *mut EmhSurrogate, params: *mut f32, norm: *mut f32) -> c_int;
@@ -712,6 +714,16 @@ impl<'a> HipBatch<'a> {
         let mut n = 0u32;
         check(unsafe { emh_batch_autotune(self.raw, trial_steps, &mut best, timed.as_mut_ptr(), 64, &mut n) })?;
         Ok((best, timed[..(n as usize).min(64)].to_vec()))
+    }
+    /// Live per-stage timing of the batch steps (resets the totals).
+    pub fn set_profiling(&mut self, enable: bool) -> Result<(), HipError> {
+        check(unsafe { emh_batch_set_profiling(self.raw, enable as c_int) })
+    }
+    pub fn kernel_times(&mut self) -> Result<Vec<EmhKernelTime>, HipError> {
+        let mut out = [EmhKernelTime::default(); 8];
+        let mut n = 0u32;
+        check(unsafe { emh_batch_kernel_times(self.raw, out.as_mut_ptr(), out.len() as u32, &mut n) })?;
+        Ok(out[..(n as usize).min(out.len())].to_vec())
     }
 }
 

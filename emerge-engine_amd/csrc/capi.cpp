@@ -2698,9 +2698,47 @@ struct emh_batch {
     bool use_graphs = false;
     GraphCache graphs;
 
+    // Live timing of the batch's stages (emh_batch_set_profiling): event marks
+    // on the batch stream before each stage's launches, recorded without
+    // stopping the queue and read at the drains, whose synchronisation has
+    // completed them.  A span runs from a mark to the next one.
+    enum : uint8_t { kStBegin = 0, kStSort = 1, kStSubsteps = 2, kStGather = 3, kStages = 4, kStIdle = 4 };
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<uint8_t> prof_kind;
+    size_t prof_used = 0;
+    double prof_ms[kStages] = {};
+    uint64_t prof_count[kStages] = {};
+    int prof_mark(uint8_t kind) {
+        if (!profiling) return EMH_OK;
+        if (prof_used == prof_pool.size()) {
+            hipEvent_t e = nullptr;  // timing only: no system-scope fence (see emh_sim::prof_mark)
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            prof_pool.push_back(e);
+            prof_kind.push_back(kStIdle);
+        }
+        HIP_TRY(hipEventRecord(prof_pool[prof_used], stream));
+        prof_kind[prof_used++] = kind;
+        return EMH_OK;
+    }
+    int prof_fold() {
+        for (size_t i = 0; i + 1 < prof_used; ++i) {
+            if (prof_kind[i] >= kStages) continue;
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, prof_pool[i], prof_pool[i + 1]));
+            prof_ms[prof_kind[i]] += ms;
+        }
+        prof_used = 0;
+        return EMH_OK;
+    }
+    void prof_count_add(uint8_t kind, uint64_t n) {
+        if (profiling) prof_count[kind] += n;
+    }
+
     ~emh_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         graphs.clear();
+        for (auto& e : prof_pool) (void)hipEventDestroy(e);
         if (d_pipes) (void)hipFree(d_pipes);
         if (d_starts) (void)hipFree(d_starts);
         if (d_ctrl) (void)hipFree(d_ctrl);
@@ -2784,7 +2822,12 @@ struct emh_batch {
         uint32_t extra = 0;
         const uint32_t limit = (max_substeps + 1) * (kStepRing + 1);
         while (true) {
+            int rc = prof_mark(kStGather);
+            if (rc) return rc;
             HIP_TRY(launch_batch_gather_ctrl(L, d_ctrl, stream));
+            rc = prof_mark(kStIdle);
+            if (rc) return rc;
+            prof_count_add(kStGather, 1);
             HIP_TRY(hipMemcpyAsync(h_ctrl.data(), d_ctrl, sizeof(Ctrl) * n, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
             uint32_t more = 0;
@@ -2797,8 +2840,15 @@ struct emh_batch {
             if (more == 0) break;
             more = std::min<uint32_t>(more, 64u);
             if (extra + more > limit) return fail(EMH_ERR_DEVICE, "batched substep loop did not terminate");
+            rc = prof_mark(kStSubsteps);
+            if (rc) return rc;
+            prof_count_add(kStSubsteps, more);
             for (uint32_t k = 0; k < more; ++k) HIP_TRY(launch_batch_substep(L, seq++, stream));
             extra += more;
+        }
+        {
+            int rc = prof_fold();  // the last synchronisation completed every mark
+            if (rc) return rc;
         }
         predicted = 1;
         for (uint32_t i = 0; i < n; ++i) {
@@ -2847,6 +2897,7 @@ struct emh_batch {
     // its saved state between candidates and after the last.
     int autotune(uint32_t trial, emh_tuning* best, emh_tuning* timed, uint32_t cap, uint32_t* n_timed) {
         if (trial == 0) return fail(EMH_ERR_INVALID_ARG, "trial_steps must be at least 1");
+        if (profiling) return fail(EMH_ERR_STATE, "turn profiling off before autotuning");
         const size_t n = sims.size();
         std::vector<std::vector<emh_particle>> saved(n);
         std::vector<emh_step_stats> saved_stats(n);
@@ -2918,9 +2969,22 @@ struct emh_batch {
         constexpr uint32_t kDrainSteps = kStepRing / 2;
         for (uint32_t s = 0; s < n_steps; ++s) {
             const bool reorder = substeps_since_reorder >= reorder_substeps;
-            HIP_TRY(launch_batch_begin_step(L, seq++, reorder, stream));
-            if (reorder) substeps_since_reorder = 0;
+            rc = prof_mark(kStBegin);
+            if (rc) return rc;
+            prof_count_add(kStBegin, 1);
+            HIP_TRY(launch_batch_begin_step(L, seq, reorder, stream));
+            if (reorder) {
+                rc = prof_mark(kStSort);
+                if (rc) return rc;
+                prof_count_add(kStSort, 1);
+                HIP_TRY(launch_batch_sort(L, seq, stream));
+                substeps_since_reorder = 0;
+            }
+            ++seq;
             const uint32_t slots = std::min<uint32_t>(std::max<uint32_t>(predicted, 1u), max_substeps);
+            rc = prof_mark(kStSubsteps);
+            if (rc) return rc;
+            prof_count_add(kStSubsteps, slots);
             if (use_graphs) {
                 graphs.check_arg(&L, sizeof(L));
                 const uint32_t r = (uint32_t)(seq % 6u);
@@ -3014,6 +3078,35 @@ int emh_batch_autotune(emh_batch* b, uint32_t trial_steps, emh_tuning* chosen, e
                        uint32_t* n_timed) {
     if (!b) return fail(EMH_ERR_INVALID_ARG, "null batch");
     return b->autotune(trial_steps, chosen, timed, timed_cap, n_timed);
+}
+
+int emh_batch_set_profiling(emh_batch* b, int enable) {
+    if (!b) return fail(EMH_ERR_INVALID_ARG, "null batch");
+    b->profiling = enable != 0;
+    b->prof_used = 0;
+    for (int k = 0; k < emh_batch::kStages; ++k) {
+        b->prof_ms[k] = 0.0;
+        b->prof_count[k] = 0;
+    }
+    return EMH_OK;
+}
+
+int emh_batch_kernel_times(emh_batch* b, emh_kernel_time* out, uint32_t cap, uint32_t* n_out) {
+    if (!b || (!out && cap)) return fail(EMH_ERR_INVALID_ARG, "null argument");
+    static const char* const names[emh_batch::kStages] = {"kb_begin_step", "kb_sort", "kb_substep",
+                                                          "kb_gather_ctrl"};
+    uint32_t n = 0;
+    for (int k = 0; k < emh_batch::kStages; ++k) {
+        if (n < cap) {
+            std::memset(&out[n], 0, sizeof(out[n]));
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", names[k]);
+            out[n].launches = b->prof_count[k];
+            out[n].total_ms = b->prof_ms[k];
+        }
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    return EMH_OK;
 }
 
 }  // extern "C"

@@ -1892,13 +1892,16 @@ hipError_t launch_prologue(const Pipeline& p, uint64_t seq, hipStream_t stream, 
 
 hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reorder, hipStream_t stream) {
     kb_begin_step<<<L.one.n, 64, 0, stream>>>(L.one, seq, reorder ? 1u : 0u);
-    if (reorder && L.sort_blocks) {
-        // the keys are those of the state launch seq + 1 will move
-        for (uint32_t pass = 0; pass < L.sort_passes; ++pass) {
-            kb_sort_count<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
-            kb_sort_scan<<<L.scan_blocks, kBlock, 0, stream>>>(L.scan, seq + 1u, pass);
-            kb_sort_scatter<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
-        }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_sort(const BatchLists& L, uint64_t seq, hipStream_t stream) {
+    if (!L.sort_blocks) return hipSuccess;
+    // the keys are those of the state launch seq + 1 will move
+    for (uint32_t pass = 0; pass < L.sort_passes; ++pass) {
+        kb_sort_count<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
+        kb_sort_scan<<<L.scan_blocks, kBlock, 0, stream>>>(L.scan, seq + 1u, pass);
+        kb_sort_scatter<<<L.sort_blocks, kBlock, 0, stream>>>(L.sort, seq + 1u, pass);
     }
     return hipGetLastError();
 }

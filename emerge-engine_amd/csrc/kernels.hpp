@@ -206,6 +206,8 @@ struct BatchLists {
 // One launch per stage over every scene of the batch (the single-scene
 // kernels' bodies, ONLY = mixed).
 hipError_t launch_batch_begin_step(const BatchLists& L, uint64_t seq, bool reorder, hipStream_t stream);
+// The radix sort of every scene's keys, after a reorder step's begin_step (same seq).
+hipError_t launch_batch_sort(const BatchLists& L, uint64_t seq, hipStream_t stream);
 hipError_t launch_batch_substep(const BatchLists& L, uint64_t seq, hipStream_t stream);
 hipError_t launch_batch_gather_ctrl(const BatchLists& L, Ctrl* out, hipStream_t stream);
 

@@ -291,6 +291,9 @@ def load_library() -> ctypes.CDLL:
         "emh_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
         "emh_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CTuning), P(CTuning),
                                               ctypes.c_uint32, P(ctypes.c_uint32)]),
+        "emh_batch_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "emh_batch_kernel_times": (ctypes.c_int, [ctypes.c_void_p, P(CKernelTime), ctypes.c_uint32,
+                                                  P(ctypes.c_uint32)]),
         "emh_sim_get_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
         "emh_sim_set_tuning": (ctypes.c_int, [ctypes.c_void_p, P(CTuning)]),
         "emh_sim_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CTuning), P(CTuning),
@@ -1160,6 +1163,18 @@ class SceneBatch(_Tunable):
 
     def step_n(self, steps: int) -> None:
         _check(load_library().emh_batch_step(self._h, int(steps)))
+
+    def set_profiling(self, enable: bool) -> None:
+        """Live per-stage timing of the batch steps (resets the totals)."""
+        _check(load_library().emh_batch_set_profiling(self._h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        """{"kb_begin_step" | "kb_sort" | "kb_substep" | "kb_gather_ctrl": (launches, ms)}
+        since profiling was enabled."""
+        arr = (CKernelTime * 8)()
+        n = ctypes.c_uint32()
+        _check(load_library().emh_batch_kernel_times(self._h, arr, 8, ctypes.byref(n)))
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(min(n.value, 8))}
 
     def close(self) -> None:
         if self._h:

@@ -539,6 +539,17 @@ int emh_batch_get_tuning(emh_batch* b, emh_tuning* out);
 int emh_batch_set_tuning(emh_batch* b, const emh_tuning* t);
 int emh_batch_autotune(emh_batch* b, uint32_t trial_steps, emh_tuning* chosen, emh_tuning* timed,
                        uint32_t timed_cap, uint32_t* n_timed);
+/* Live timing of the batch's stages, the counterpart of emh_sim_set_profiling
+ * for a batch: HIP events on the batch stream before each stage's launches,
+ * read at the drains.  Four entries, summed over the batch steps since
+ * enabling: "kb_begin_step" (launches = batch steps), "kb_sort" (the radix
+ * sort of every scene, launches = reorder steps), "kb_substep" (the density
+ * pass where a scene has one and the substep kernels; launches = substep
+ * launch slots, each covering every scene) and "kb_gather_ctrl" (the control
+ * gather of each drain).  Enabling resets the totals; the autotuner refuses
+ * to run while profiling is on. */
+int emh_batch_set_profiling(emh_batch* b, int enable);
+int emh_batch_kernel_times(emh_batch* b, emh_kernel_time* out, uint32_t cap, uint32_t* n_out);
 
 /* ---------------------------------------------------------------------
  * Learned NACC return-mapping surrogate.

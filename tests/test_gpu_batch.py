@@ -129,3 +129,37 @@ def test_batch_errors():
     batch.close()
     _close(sims + [empty, nacc])
     sg.close()
+
+
+@pytest.mark.gpu
+def test_batch_profiling_counts_stages():
+    """Per-stage timing of a batch: one begin_step per batch step, one sort on the
+    first step (every scene's upload reorder), substep launch slots at least the
+    longest scene's substeps, a control gather per drain; the timed steps take the
+    same schedules as an untimed batch of the same scenes; the autotuner refuses to
+    run while profiling is on."""
+    names = ["quickstart", "sand_column", "multi_material"]
+    a = [S.SCENES[k].build("gpu") for k in names]
+    b = [S.SCENES[k].build("gpu") for k in names]
+    ba, bb = eh.SceneBatch(a), eh.SceneBatch(b)
+    bb.set_profiling(True)
+    ba.step_n(3)
+    bb.step_n(3)
+    t = bb.kernel_times()
+    print(t)
+    assert set(t) == {"kb_begin_step", "kb_sort", "kb_substep", "kb_gather_ctrl"}
+    assert t["kb_begin_step"][0] == 3
+    assert t["kb_sort"][0] >= 1
+    assert t["kb_substep"][0] >= max(s.stats().total_substeps for s in b)
+    assert t["kb_gather_ctrl"][0] >= 1
+    assert all(ms > 0.0 for _, ms in t.values())
+    for x, y in zip(a, b):
+        assert x.stats().total_substeps == y.stats().total_substeps
+    with pytest.raises(eh.EmergeError):
+        bb.autotune(1)
+    bb.set_profiling(False)
+    assert all(v == (0, 0.0) for v in bb.kernel_times().values())
+    ba.close()
+    bb.close()
+    for s in a + b:
+        s.close()
