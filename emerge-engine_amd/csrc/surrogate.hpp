@@ -51,8 +51,12 @@ __host__ __device__ inline uint32_t sg_bf16_bits(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return u >> 16;
 }
+// Two f32 -> packed bf16 pair: one v_cvt_pk_bf16_f32 on gfx950 (round to
+// nearest even, as sg_bf16_bits for finite inputs).
+typedef __bf16 sg_bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t sg_pack2(float lo, float hi) {
-    return sg_bf16_bits(lo) | (sg_bf16_bits(hi) << 16);
+    const sg_bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
 }
 __device__ __forceinline__ bf16x8 sg_pack8(float a0, float a1, float a2, float a3, float a4, float a5, float a6,
                                            float a7) {
