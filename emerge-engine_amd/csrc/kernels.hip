@@ -623,23 +623,16 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
         }
 }
 // Segmented wave sum of one channel, then the run tails add it to the LDS box
-// (or to the global grid when the stencil leaves the box).
+// (or to the global grid when the stencil leaves the box).  Lanes that do not
+// scatter carry zero terms (scatter_terms), so their values are zero.
 // at: the stencil's LDS index (Box::stencil_at), -1 for the global grid.
-template <int CH>
-__device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, const SegMask& sm, const ScanPred& sq,
-                                                bool fma_ok, const Box& box, int at, float* s_ch, float* g_ch, int R) {
+template <bool FMA, int CH>
+__device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool tail, const ScanPred& sq, const Box& box, int at,
+                                                float* s_ch, float* g_ch, int R) {
     float v[9];
-    if (valid) {
-        p2g_channel<CH>(t, v);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) v[q] = 0.0f;
-    }
-    if (fma_ok)
-        seg_scan<true>(v, sq);
-    else
-        seg_scan<false>(v, sq);
-    if (valid && run_tail(sm)) {
+    p2g_channel<CH>(t, v);
+    seg_scan<FMA>(v, sq);
+    if (tail) {
         const int bx = t.w.bx, by = t.w.by;
         if (at >= 0) {
             float* c = s_ch + at;
@@ -658,6 +651,20 @@ __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool valid, c
                 }
         }
     }
+}
+// Terms as scattered: a lane that does not scatter (no particle, or a
+// sleeping one) gets zero weights, position and terms, so that every stencil
+// value it forms is exactly zero, whatever its state held.
+__device__ __forceinline__ P2GTerms scatter_terms(const P2GTerms& t, bool valid) {
+    P2GTerms z = t;
+    if (!valid) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) z.w.wx[g] = z.w.wy[g] = 0.0f;
+        z.x = z.y = z.mass = z.ux = z.uy = 0.0f;
+        z.k = M2{0.0f, 0.0f, 0.0f, 0.0f};
+        z.st = M2{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    return z;
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
@@ -1119,6 +1126,20 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
     return t;
 }
 
+template <bool FMA, int NCH>
+__device__ __forceinline__ void scatter_channels(const P2GTerms& z, bool tail, const ScanPred& sq, const Box& box, int at,
+                                                 float (*s_acc)[kLdsCells], float* const* g, int R) {
+    scatter_channel<FMA, kAccM>(z, tail, sq, box, at, s_acc[kAccM], g[kAccM], R);
+    if constexpr (NCH > 1) {
+        scatter_channel<FMA, kAccPx>(z, tail, sq, box, at, s_acc[kAccPx], g[kAccPx], R);
+        scatter_channel<FMA, kAccPy>(z, tail, sq, box, at, s_acc[kAccPy], g[kAccPy], R);
+    }
+    if constexpr (NCH == kAccChannels) {
+        scatter_channel<FMA, kAccSx>(z, tail, sq, box, at, s_acc[kAccSx], g[kAccSx], R);
+        scatter_channel<FMA, kAccSy>(z, tail, sq, box, at, s_acc[kAccSy], g[kAccSy], R);
+    }
+}
+
 // The first NCH channels, one at a time (keeps 9 partial sums live, not 45):
 // all five, or mass and momentum alone when the dt the terms were formed with
 // is the substep's own (the density pass).  Terms below 1e18 in magnitude
@@ -1127,7 +1148,7 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
 template <int NCH = kAccChannels>
 __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
                                             float (*s_acc)[kLdsCells], float* const* g, int R) {
-    static_assert(NCH == kAccChannels || NCH == kAccPy + 1, "all channels, or mass and momentum");
+    static_assert(NCH == kAccChannels || NCH == kAccPy + 1 || NCH == 1, "all channels, mass and momentum, or mass");
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
     const ScanPred sq = scan_pred(sm);
     float big = fmaxf(fabsf(t.mass), fmaxf(fabsf(t.ux), fabsf(t.uy)));
@@ -1137,13 +1158,14 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
                                                                   t.st.a + t.st.b + t.st.c + t.st.d));
     const bool fma_ok = __ballot(!small) == 0ull;
     const int at = box.stencil_at(t.w.bx, t.w.by);
-    scatter_channel<kAccM>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccM], g[kAccM], R);
-    scatter_channel<kAccPx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPx], g[kAccPx], R);
-    scatter_channel<kAccPy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccPy], g[kAccPy], R);
-    if (NCH == kAccChannels) {
-        scatter_channel<kAccSx>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSx], g[kAccSx], R);
-        scatter_channel<kAccSy>(t, valid, sm, sq, fma_ok, box, at, s_acc[kAccSy], g[kAccSy], R);
-    }
+    const bool tail = valid && run_tail(sm);
+    const P2GTerms z = scatter_terms(t, valid);
+    // one branch per particle round: the scans of the fast path feed the
+    // atomics directly
+    if (fma_ok)
+        scatter_channels<true, NCH>(z, tail, sq, box, at, s_acc, g, R);
+    else
+        scatter_channels<false, NCH>(z, tail, sq, box, at, s_acc, g, R);
 }
 
 // Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
@@ -1496,10 +1518,9 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
             t.x = sp.x;
             t.y = sp.y;
             t.mass = sp.mass;
-            const SegMask sm = run_mask(scat ? t.w.bx : INT_MIN + (int)lane_id(), scat ? t.w.by : 0);
-            const bool fma_ok = __ballot(scat && !(fabsf(t.mass) < 1e18f)) == 0ull;
-            scatter_channel<kAccM>(t, scat, sm, scan_pred(sm), fma_ok, box, box.stencil_at(t.w.bx, t.w.by), s_acc[0],
-                                   gdst[0], R);
+            t.ux = t.uy = 0.0f;
+            t.k = t.st = M2{0.0f, 0.0f, 0.0f, 0.0f};
+            scatter_p2g<1>(t, scat, box, s_acc, gdst, R);
         }
         if (sliced) {
             flush_box<NCH>(box, s_acc, gdst, R);
