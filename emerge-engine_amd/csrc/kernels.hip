@@ -60,10 +60,13 @@ __device__ __forceinline__ void gatomic_add(float* addr, float v) {
     __hip_atomic_fetch_add((__attribute__((address_space(1))) float*)addr, v, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
-// LDS float add, workgroup scope.  Keeping it distinct from gatomic_add also
-// stops the compiler from merging an LDS add and a global add on two sides of
-// a branch into one flat atomic.
-__device__ __forceinline__ void latomic_add(float* addr, float v) {
+// LDS add into a box accumulator, workgroup scope.  The box accumulates in
+// f64: gfx950's LDS executes an f64 add (ds_add_f64) at about 0.14 CU cycles
+// per lane, its f32 add (ds_add_f32) one lane at a time at about 3 cycles per
+// lane; each box cell is rounded to f32 once, when it is flushed.  Keeping it
+// distinct from gatomic_add also stops the compiler from merging an LDS add
+// and a global add on two sides of a branch into one flat atomic.
+__device__ __forceinline__ void latomic_add(double* addr, double v) {
     __hip_atomic_fetch_add(addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -628,18 +631,18 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
 // at: the stencil's LDS index (Box::stencil_at), -1 for the global grid.
 template <bool FMA, int CH>
 __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool tail, const ScanPred& sq, const Box& box, int at,
-                                                float* s_ch, float* g_ch, int R) {
+                                                double* s_ch, float* g_ch, int R) {
     float v[9];
     p2g_channel<CH>(t, v);
     seg_scan<FMA>(v, sq);
     if (tail) {
         const int bx = t.w.bx, by = t.w.by;
         if (at >= 0) {
-            float* c = s_ch + at;
+            double* c = s_ch + at;
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-                for (int gy = 0; gy < 3; ++gy) latomic_add(&c[gx * box.h + gy], v[gx * 3 + gy]);
+                for (int gy = 0; gy < 3; ++gy) latomic_add(&c[gx * box.h + gy], (double)v[gx * 3 + gy]);
         } else {
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
@@ -1036,10 +1039,10 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
 // shorten its runs and costs about 5% of the substep.
 constexpr uint32_t kSortRunsPerWave = 8;
 __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
-                                            uint32_t nvalid, uint32_t* s_cnt, uint32_t* s_perm, uint32_t* s_scan) {
+                                            uint32_t nvalid, uint32_t* s_cnt, uint16_t* s_perm, uint32_t* s_scan) {
     if (!box.lds) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = k * kBlock + threadIdx.x;
+        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = (uint16_t)(k * kBlock + threadIdx.x);
         __syncthreads();
         return;
     }
@@ -1068,7 +1071,7 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
     __syncthreads();
     if (s_runs <= kSortRunsPerWave * (kChunk / 64)) {
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = k * kBlock + threadIdx.x;
+        for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = (uint16_t)(k * kBlock + threadIdx.x);
         __syncthreads();
         return;
     }
@@ -1100,7 +1103,7 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t q = k * kBlock + threadIdx.x;
-        if (q < nvalid) s_perm[s_cnt[cell[k]] + rank[k]] = q;
+        if (q < nvalid) s_perm[s_cnt[cell[k]] + rank[k]] = (uint16_t)q;
     }
     __syncthreads();
 }
@@ -1128,7 +1131,7 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
 
 template <bool FMA, int NCH>
 __device__ __forceinline__ void scatter_channels(const P2GTerms& z, bool tail, const ScanPred& sq, const Box& box, int at,
-                                                 float (*s_acc)[kLdsCells], float* const* g, int R) {
+                                                 double (*s_acc)[kLdsCells], float* const* g, int R) {
     scatter_channel<FMA, kAccM>(z, tail, sq, box, at, s_acc[kAccM], g[kAccM], R);
     if constexpr (NCH > 1) {
         scatter_channel<FMA, kAccPx>(z, tail, sq, box, at, s_acc[kAccPx], g[kAccPx], R);
@@ -1147,7 +1150,7 @@ __device__ __forceinline__ void scatter_channels(const P2GTerms& z, bool tail, c
 // finite, so the fma form of the scan is exact.
 template <int NCH = kAccChannels>
 __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
-                                            float (*s_acc)[kLdsCells], float* const* g, int R) {
+                                            double (*s_acc)[kLdsCells], float* const* g, int R) {
     static_assert(NCH == kAccChannels || NCH == kAccPy + 1 || NCH == 1, "all channels, mass and momentum, or mass");
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
     const ScanPred sq = scan_pred(sm);
@@ -1171,7 +1174,7 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
 // Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
 // sorted by base cell inside the slice's box: s_perm[k*kBlock + r].
 __device__ __forceinline__ void slice_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
-                                            uint32_t k, uint32_t nvalid, uint32_t* s_cnt, uint32_t* s_perm,
+                                            uint32_t k, uint32_t nvalid, uint32_t* s_cnt, uint16_t* s_perm,
                                             uint32_t* s_scan) {
     const uint32_t q = k * kBlock + threadIdx.x;
     const bool valid = q < nvalid;
@@ -1207,7 +1210,7 @@ __device__ __forceinline__ void slice_order(const Pipeline& p, const PBuf& a, ui
         off += cnt[i];
     }
     __syncthreads();
-    if (valid) s_perm[k * kBlock + s_cnt[cell] + rank] = q;
+    if (valid) s_perm[k * kBlock + s_cnt[cell] + rank] = (uint16_t)q;
     __syncthreads();
 }
 
@@ -1227,11 +1230,11 @@ __device__ __forceinline__ Box slice_box(const Pipeline& p, const PBuf& a, uint3
 // Clear the box's LDS accumulators and stage its updated grid velocity.
 template <bool G2P, int NCH>
 __device__ __forceinline__ void stage_box(const Pipeline& p, const Box& box, uint32_t cur, float dt, float ddt,
-                                          float (*s_acc)[kLdsCells], float2* s_vel) {
+                                          double (*s_acc)[kLdsCells], float2* s_vel) {
     if (!box.lds) return;
     for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0f;
+        for (int ch = 0; ch < NCH; ++ch) s_acc[ch][i] = 0.0;
         if (G2P) s_vel[i] = cell_velocity(p, cur, dt, ddt, box.ox + i / box.h, box.oy + i % box.h);
     }
     __syncthreads();
@@ -1239,7 +1242,7 @@ __device__ __forceinline__ void stage_box(const Pipeline& p, const Box& box, uin
 
 // Add the LDS box into the global grid: one atomic per touched cell and channel.
 template <int NCH>
-__device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCells], float* const* g, int R) {
+__device__ __forceinline__ void flush_box(const Box& box, double (*s_acc)[kLdsCells], float* const* g, int R) {
     if (!box.lds) return;
     __syncthreads();
     for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
@@ -1248,7 +1251,7 @@ __device__ __forceinline__ void flush_box(const Box& box, float (*s_acc)[kLdsCel
         const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
-            const float v = s_acc[ch][i];
+            const float v = (float)s_acc[ch][i];
             if (v != 0.0f) gatomic_add(&g[ch][gi], v);
         }
     }
@@ -1321,9 +1324,9 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     constexpr int NCH = P2G ? kAccChannels : 1;
     __shared__ float s_sg[SURR ? kWaves : 1][SURR ? kSgQueueLds<1> : 1];
     __shared__ std::conditional_t<SURR, SgLds, char> s_net;
-    __shared__ float s_acc[NCH][kLdsCells];
+    __shared__ double s_acc[NCH][kLdsCells];
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];
-    __shared__ uint32_t s_perm[kChunk];
+    __shared__ uint16_t s_perm[kChunk];
     __shared__ uint32_t s_scan[17];
     __shared__ int s_box[4 * kWaves];
     __shared__ float s_red[2 * kWaves];
@@ -1567,7 +1570,7 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     // correction for a mispredicted dt) stay empty, and the grid update of a
     // density-pass scene never reads them
     constexpr int kDensCh = kAccPy + 1;
-    __shared__ float s_acc[kDensCh][kLdsCells];
+    __shared__ double s_acc[kDensCh][kLdsCells];
     __shared__ float s_dm[kLdsCells];
     __shared__ int s_box[4 * kWaves];
     Ctrl* c = p.ctrl;
@@ -1593,7 +1596,7 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     if (box.lds) {
         for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
 #pragma unroll
-            for (int ch = 0; ch < kDensCh; ++ch) s_acc[ch][i] = 0.0f;
+            for (int ch = 0; ch < kDensCh; ++ch) s_acc[ch][i] = 0.0;
             const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
             s_dm[i] = (cx >= 0 && cy >= 0 && cx < R && cy < R) ? dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] : 0.0f;
         }

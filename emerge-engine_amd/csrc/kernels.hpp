@@ -13,7 +13,10 @@ constexpr int kBlock = 256;        // threads per workgroup for particle kernels
 constexpr int kWaves = kBlock / 64;
 constexpr int kChunk = 1024;       // particles per substep workgroup (4 per thread)
 constexpr int kPer = kChunk / kBlock;
-constexpr int kLdsCells = 1024;    // LDS box budget in grid cells (7 floats each: 28 KiB)
+// LDS box budget in grid cells: five f64 accumulators and the staged f32
+// velocity per cell (48 B: 36 KiB, with the u16 processing order under the
+// 40 KiB that keeps four 256-lane workgroups per CU).  A multiple of kBlock.
+constexpr int kLdsCells = 768;
 constexpr int kTileCells = 256;    // 16 x 16 cells per sort tile
 // Default interval of the physical cell-key reorder, in substeps (the upload
 // is always reordered at the first step).  The in-chunk sort keeps the runs
