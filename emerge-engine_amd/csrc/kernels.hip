@@ -434,6 +434,11 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
 // with p = 1 or 0, which is exactly v + t or v when t is finite (one
 // v_fmac_f32_dpp per step), or as a select (3 instructions) for waves whose
 // terms are not all small enough for every partial sum to stay finite.
+// kScanSteps = 4 splits runs at the 16-lane row boundaries (every row start
+// is a run head), so that the scan needs only the four row shifts; 6 keeps
+// runs across rows with the two row-broadcast steps.
+constexpr int kScanSteps = 4;
+constexpr uint64_t kRowHeads = kScanSteps == 4 ? 0x0001000100010001ull : 1ull;
 struct SegMask {
     uint64_t heads;  // bit L: lane L starts a run
     uint32_t lane;
@@ -485,16 +490,27 @@ __device__ __forceinline__ float scan_step(float v, bool p, float f) {
     "v_fmac_f32_dpp %6, %6, " F " " DPP "\n\tv_fmac_f32_dpp %7, %7, " F " " DPP "\n\t"                     \
     "v_fmac_f32_dpp %8, %8, " F " " DPP "\n\t"
 __device__ __forceinline__ void seg_scan_fma9(float (&v)[9], const ScanPred& q) {
-    asm("s_nop 1\n\t"
-        EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
-        EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
-        EMH_SCAN9("%11", "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
-        EMH_SCAN9("%12", "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1")
-        EMH_SCAN9("%13", "row_bcast:15 row_mask:0xa bank_mask:0xf")
-        EMH_SCAN9("%14", "row_bcast:31 row_mask:0xc bank_mask:0xf")
-        : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
-          "+&v"(v[8])
-        : "v"(q.f[0]), "v"(q.f[1]), "v"(q.f[2]), "v"(q.f[3]), "v"(q.f[4]), "v"(q.f[5]));
+    if constexpr (kScanSteps == 4) {
+        asm("s_nop 1\n\t"
+            EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%11", "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%12", "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
+              "+&v"(v[8])
+            : "v"(q.f[0]), "v"(q.f[1]), "v"(q.f[2]), "v"(q.f[3]));
+    } else {
+        asm("s_nop 1\n\t"
+            EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%11", "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%12", "row_shr:8 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%13", "row_bcast:15 row_mask:0xa bank_mask:0xf")
+            EMH_SCAN9("%14", "row_bcast:31 row_mask:0xc bank_mask:0xf")
+            : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
+              "+&v"(v[8])
+            : "v"(q.f[0]), "v"(q.f[1]), "v"(q.f[2]), "v"(q.f[3]), "v"(q.f[4]), "v"(q.f[5]));
+    }
 }
 #undef EMH_SCAN9
 template <bool FMA, int N>
@@ -510,8 +526,10 @@ __device__ __forceinline__ void seg_scan(float (&v)[N], const ScanPred& q) {
         x = scan_step<FMA, 0x112, 0xf>(x, q.p[1], q.f[1]);  // row_shr:2
         x = scan_step<FMA, 0x114, 0xf>(x, q.p[2], q.f[2]);  // row_shr:4
         x = scan_step<FMA, 0x118, 0xf>(x, q.p[3], q.f[3]);  // row_shr:8
-        x = scan_step<FMA, 0x142, 0xa>(x, q.p[4], q.f[4]);  // row_bcast:15 into rows 1 and 3
-        x = scan_step<FMA, 0x143, 0xc>(x, q.p[5], q.f[5]);  // row_bcast:31 into rows 2 and 3
+        if constexpr (kScanSteps == 6) {
+            x = scan_step<FMA, 0x142, 0xa>(x, q.p[4], q.f[4]);  // row_bcast:15 into rows 1 and 3
+            x = scan_step<FMA, 0x143, 0xc>(x, q.p[5], q.f[5]);  // row_bcast:31 into rows 2 and 3
+        }
         v[i] = x;
     }
 }
@@ -522,7 +540,7 @@ __device__ __forceinline__ SegMask run_mask(int bx, int by) {
     m.lane = lane_id();
     const int pbx = __shfl_up(bx, 1, 64), pby = __shfl_up(by, 1, 64);
     const bool head = (m.lane == 0) || pbx != bx || pby != by;
-    m.heads = __ballot(head);
+    m.heads = __ballot(head) | kRowHeads;
     return m;
 }
 __device__ __forceinline__ bool run_tail(const SegMask& m) {
