@@ -438,7 +438,10 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
 // is a run head), so that the scan needs only the four row shifts; 6 keeps
 // runs across rows with the two row-broadcast steps.
 constexpr int kScanSteps = 4;
-constexpr uint64_t kRowHeads = kScanSteps == 4 ? 0x0001000100010001ull : 1ull;
+constexpr uint64_t kRowHeads = kScanSteps == 2   ? 0x1111111111111111ull
+                               : kScanSteps == 3 ? 0x0101010101010101ull
+                               : kScanSteps == 4 ? 0x0001000100010001ull
+                                                 : 1ull;
 struct SegMask {
     uint64_t heads;  // bit L: lane L starts a run
     uint32_t lane;
@@ -490,7 +493,22 @@ __device__ __forceinline__ float scan_step(float v, bool p, float f) {
     "v_fmac_f32_dpp %6, %6, " F " " DPP "\n\tv_fmac_f32_dpp %7, %7, " F " " DPP "\n\t"                     \
     "v_fmac_f32_dpp %8, %8, " F " " DPP "\n\t"
 __device__ __forceinline__ void seg_scan_fma9(float (&v)[9], const ScanPred& q) {
-    if constexpr (kScanSteps == 4) {
+    if constexpr (kScanSteps == 2) {
+        asm("s_nop 1\n\t"
+            EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
+              "+&v"(v[8])
+            : "v"(q.f[0]), "v"(q.f[1]));
+    } else if constexpr (kScanSteps == 3) {
+        asm("s_nop 1\n\t"
+            EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            EMH_SCAN9("%11", "row_shr:4 row_mask:0xf bank_mask:0xf bound_ctrl:1")
+            : "+&v"(v[0]), "+&v"(v[1]), "+&v"(v[2]), "+&v"(v[3]), "+&v"(v[4]), "+&v"(v[5]), "+&v"(v[6]), "+&v"(v[7]),
+              "+&v"(v[8])
+            : "v"(q.f[0]), "v"(q.f[1]), "v"(q.f[2]));
+    } else if constexpr (kScanSteps == 4) {
         asm("s_nop 1\n\t"
             EMH_SCAN9("%9", "row_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1")
             EMH_SCAN9("%10", "row_shr:2 row_mask:0xf bank_mask:0xf bound_ctrl:1")
@@ -524,8 +542,8 @@ __device__ __forceinline__ void seg_scan(float (&v)[N], const ScanPred& q) {
         float x = v[i];
         x = scan_step<FMA, 0x111, 0xf>(x, q.p[0], q.f[0]);  // row_shr:1
         x = scan_step<FMA, 0x112, 0xf>(x, q.p[1], q.f[1]);  // row_shr:2
-        x = scan_step<FMA, 0x114, 0xf>(x, q.p[2], q.f[2]);  // row_shr:4
-        x = scan_step<FMA, 0x118, 0xf>(x, q.p[3], q.f[3]);  // row_shr:8
+        if constexpr (kScanSteps >= 3) x = scan_step<FMA, 0x114, 0xf>(x, q.p[2], q.f[2]);  // row_shr:4
+        if constexpr (kScanSteps >= 4) x = scan_step<FMA, 0x118, 0xf>(x, q.p[3], q.f[3]);  // row_shr:8
         if constexpr (kScanSteps == 6) {
             x = scan_step<FMA, 0x142, 0xa>(x, q.p[4], q.f[4]);  // row_bcast:15 into rows 1 and 3
             x = scan_step<FMA, 0x143, 0xc>(x, q.p[5], q.f[5]);  // row_bcast:31 into rows 2 and 3
@@ -1052,10 +1070,15 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
 // that the lanes of a wave hold long runs of one cell however far the stored
 // order has drifted since the last physical reorder.  Lane t's k-th particle
 // is slot s_perm[k * kBlock + t] of the chunk.  s_cnt: kLdsCells scratch.
-// A chunk whose stored order still has few runs (right after a physical
-// reorder, or where particles barely move) keeps it: the sort would not
-// shorten its runs and costs about 5% of the substep.
-constexpr uint32_t kSortRunsPerWave = 8;
+// A chunk keeps its stored order when that order's runs (cell changes between
+// neighbouring lanes, counted per wave) are within 5/4 of what the sort would
+// leave, one run per distinct cell and wave: right after a physical reorder,
+// where particles barely move, and in sparse scenes whose cells hold few
+// particles, the sort costs time and shortens nothing (1M-particle sand: 75.1
+// us per substep never sorting, 68.7 sorting; a four-material scene of 4
+// particles per cell 9 % faster never sorting than above 8 runs per wave).
+// The distinct cells come from the sort's own counting pass: integer LDS
+// atomics run at full rate on gfx950.
 __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
                                             uint32_t nvalid, uint32_t* s_cnt, uint16_t* s_perm, uint32_t* s_scan) {
     if (!box.lds) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
@@ -1064,10 +1087,10 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
         __syncthreads();
         return;
     }
-    __shared__ uint32_t s_runs;
+    __shared__ uint32_t s_runs, s_distinct;
     const int cells = box.w * box.h;
     for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
-    if (threadIdx.x == 0) s_runs = 0u;
+    if (threadIdx.x == 0) s_runs = s_distinct = 0u;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t cell[kPer], rank[kPer];
@@ -1085,22 +1108,22 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
         const uint32_t prev = (uint32_t)__shfl_up((int)cell[k], 1);
         const uint64_t starts = __ballot(valid && (lane == 0u || prev != cell[k]));
         if (lane == 0u && starts != 0ull) atomicAdd(&s_runs, (uint32_t)__popcll(starts));
+        // one returning LDS atomic per particle: same-cell lanes serialise in
+        // the LDS unit, with no per-cell loop of dependent round trips
+        rank[k] = valid ? atomicAdd(&s_cnt[cell[k]], 1u) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint64_t firsts = __ballot(k * kBlock + threadIdx.x < nvalid && rank[k] == 0u);
+        if (lane == 0u && firsts != 0ull) atomicAdd(&s_distinct, (uint32_t)__popcll(firsts));
     }
     __syncthreads();
-    if (s_runs <= kSortRunsPerWave * (kChunk / 64)) {
+    if (4u * s_runs <= 5u * (s_distinct + (uint32_t)(kChunk / 64))) {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = (uint16_t)(k * kBlock + threadIdx.x);
         __syncthreads();
         return;
     }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        // one returning LDS atomic per particle: same-cell lanes serialise in
-        // the LDS unit, with no per-cell loop of dependent round trips
-        const uint32_t q = k * kBlock + threadIdx.x;
-        rank[k] = q < nvalid ? atomicAdd(&s_cnt[cell[k]], 1u) : 0u;
-    }
-    __syncthreads();
     constexpr int kCellsPerThread = kLdsCells / kBlock;
     uint32_t cnt[kCellsPerThread], sum = 0;
 #pragma unroll
