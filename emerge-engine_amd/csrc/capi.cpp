@@ -557,9 +557,8 @@ struct emh_sim {
     }
 
     uint32_t res() const { return cfg.grid_res; }
-    uint32_t tpd() const { return (cfg.grid_res + 15u) / 16u; }
-    uint32_t n_tiles() const { return tpd() * tpd(); }
-    uint32_t n_bins() const { return n_tiles() * (uint32_t)kTileCells; }
+    uint32_t key_order() const { return cell_key_order(cfg.grid_res); }
+    uint32_t n_bins() const { return 1u << (2u * key_order()); }
 
     int init_device() {
         if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
@@ -674,7 +673,7 @@ struct emh_sim {
         const uint32_t bt = cfg.boundary_thickness;
         d.proj_lo = (float)(bt > 0 ? bt - 1 : 0u);
         d.proj_hi = (float)(cfg.grid_res > bt ? cfg.grid_res - bt : 0u);
-        d.tiles_per_dim = tpd();
+        d.key_order = key_order();
         d.n_bins = n_bins();
         bool dens = cfg.recompute_density_each_step != 0;
         for (auto& m : mats)
@@ -2023,7 +2022,7 @@ uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t R) {
     int32_t hi = (int32_t)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
-    return emh::cell_key(cx, cy, R, (R + 15u) / 16u);
+    return emh::cell_key(cx, cy, R, emh::cell_key_order(R));
 }
 
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out, uint32_t* perm_out) {
@@ -2031,7 +2030,7 @@ int emh_sort_cell_keys(const float* xy, size_t n, uint32_t R, uint32_t* keys_out
     if (R < 1 || R > 16384) return fail(EMH_ERR_INVALID_ARG, "grid_res out of range");
     if (n > 0x7fffffffu) return fail(EMH_ERR_INVALID_ARG, "too many keys");
     if (device_present() == 0) return fail(EMH_ERR_DEVICE, "no HIP device is available");
-    const uint32_t tpd = (R + 15u) / 16u, n_bins = tpd * tpd * 256u;
+    const uint32_t n_bins = 1u << (2u * cell_key_order(R));
     const uint32_t nn = (uint32_t)n;
     if (nn == 0) return EMH_OK;
     hipStream_t st = nullptr;

@@ -64,8 +64,8 @@ struct DevConfig {
     // derived
     float proj_lo, proj_hi;  // particle clamp range of project_particle_state_to_admissible
     float bnd_lo[8], bnd_hi[8];   // each boundary's particle clamp box
-    uint32_t tiles_per_dim;   // 16x16-cell tiles per grid side
-    uint32_t n_bins;          // tiles_per_dim^2 * 256
+    uint32_t key_order;       // cell keys: Hilbert curve over a 2^key_order square (cell_key)
+    uint32_t n_bins;          // 4^key_order
     uint32_t density_pass;    // any material needs_density_recompute || config flag
     uint32_t fields;          // kF* bits: optional particle fields the particles' materials use
     uint32_t only_model;      // the model every particle's material has, or kModelMixed
@@ -904,22 +904,40 @@ EMH_DEV float affine_cfl(const M2& c, float cell) {
     return g * kCorner * cell;
 }
 
-// Cell key: 16x16-cell tiles in serpentine order (tile column tx runs up in
-// y when tx is even and down when odd), and inside a tile rows of 16 cells
-// (x within the row) taken in the same vertical direction as the tile column.
-// Consecutive rows are then always vertically adjacent (across tiles too), so
-// any run of keys covers a compact strip at most 16 cells wide (32 at a turn
-// between tile columns).  Cell coordinates are clamped into the grid.
-__host__ __device__ inline uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t tpd) {
+// Cell key: the cell's index along the Hilbert curve over the 2^order x
+// 2^order square that holds the grid (order = cell_key_order(R)).  Cells
+// whose keys are consecutive are edge neighbours, and every aligned run of
+// 4^j keys fills a 2^j x 2^j square, so any run of keys -- a workgroup's
+// chunk of particles, or a wave's -- covers a compact patch also where the
+// cells it takes particles from are sparse (a thin layer at a body's edge).
+// Cell coordinates are clamped into the grid.  The curve is the classic one
+// (quadrant digits (3 rx) ^ ry from the top bit down, the lower bits rotated
+// or reflected when ry = 0).
+__host__ __device__ inline uint32_t cell_key_order(uint32_t R) {
+    uint32_t k = 1;
+    while (k < 16u && (1u << k) < R) ++k;
+    return k;
+}
+__host__ __device__ inline uint32_t cell_key(int cx, int cy, uint32_t R, uint32_t order) {
     int hi = (int)R - 1;
     cx = cx < 0 ? 0 : (cx > hi ? hi : cx);
     cy = cy < 0 ? 0 : (cy > hi ? hi : cy);
-    const uint32_t tx = (uint32_t)cx >> 4, ty = (uint32_t)cy >> 4;
-    const uint32_t odd = tx & 1u;
-    const uint32_t tile = tx * tpd + (odd ? (tpd - 1u - ty) : ty);
-    const uint32_t ly = (uint32_t)cy & 15u;
-    const uint32_t row = odd ? 15u - ly : ly;
-    return tile * 256u + (row << 4) + ((uint32_t)cx & 15u);
+    const uint32_t n = 1u << order;
+    uint32_t x = (uint32_t)cx, y = (uint32_t)cy, d = 0;
+    for (uint32_t s = n >> 1; s > 0u; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0u) {
+            if (rx == 1u) {
+                x = n - 1u - x;
+                y = n - 1u - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
 }
 
 }  // namespace emh

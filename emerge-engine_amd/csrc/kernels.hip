@@ -706,11 +706,23 @@ __device__ __forceinline__ P2GTerms scatter_terms(const P2GTerms& t, bool valid)
     return z;
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-// run of chunks so that neighbouring chunks share that XCD's L2.
+// Workgroups are dealt to the 8 XCDs round-robin.  Chunks go to them in
+// groups of kXcdGroup consecutive chunks (a compact patch of cells along the
+// key curve, whose grid cells its workgroups then share in one XCD's L2), the
+// groups dealt round-robin, so that every XCD gets patches from all over the
+// scene: a launch ends with its slowest XCD, and one contiguous 1/8 of the
+// scene each would give one XCD all of a flowing region and another a resting
+// one.  Measured on the 1M-particle sand column (500 steps): 84.8 us per
+// substep in groups of 4, 84.4 in groups of 16, 91.1 with one contiguous run
+// per XCD; one chunk at a time (no groups) loses the L2 sharing (the
+// 4-per-cell interior: 78.0 against 70.3 us).  Workgroups beyond the last
+// whole round of groups keep their own chunk.
+constexpr uint32_t kXcdGroup = 4;
 __device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
-    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
+    const uint32_t m = nb / (8u * kXcdGroup) * (8u * kXcdGroup);
+    if (b >= m) return b;
+    const uint32_t x = b & 7u, i = b >> 3;
+    return ((i / kXcdGroup) * 8u + x) * kXcdGroup + i % kXcdGroup;
 }
 
 // Block reduction of the CFL contribution, one atomic pair per workgroup.
@@ -779,7 +791,7 @@ struct SortJob {
     const uint32_t* mat;    // material words (model segments only)
     const DevMat* mats;     // material table (model segments only)
     uint32_t n_mats;
-    uint32_t n, grid_res, tpd, n_bins;
+    uint32_t n, grid_res, key_order, n_bins;
     uint32_t cell_bits, key_bits;  // key = model << cell_bits | cell key when s.model_bits
     SortBuf s;
 };
@@ -817,7 +829,7 @@ __device__ __forceinline__ uint32_t sort_key_in(const SortJob& J, uint32_t pass,
     if (pass > 0) return pick(J.s.kbuf, (pass - 1u) & 1u)[i];
     float2 x = J.x[i];
     if (J.project) project_x(x.x, x.y, J.proj_lo, J.proj_hi);
-    const uint32_t key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.tpd);
+    const uint32_t key = cell_key(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)), J.grid_res, J.key_order);
     if (!J.s.model_bits) return key;
     const uint32_t slot = J.mat[i] & kMatMask;  // the slot mat_of reads (slot 0 for an unregistered id)
     const uint32_t model = min(J.mats[slot < J.n_mats ? slot : 0u].model, (1u << kModelKeyBits) - 1u);
@@ -994,7 +1006,7 @@ __device__ __forceinline__ SortJob pipeline_sort_job(const Pipeline& p, uint64_t
     J.n_mats = p.n_mats;
     J.n = p.n;
     J.grid_res = p.cfg.grid_res;
-    J.tpd = p.cfg.tiles_per_dim;
+    J.key_order = p.cfg.key_order;
     J.n_bins = p.cfg.n_bins;
     J.cell_bits = cell_key_bits(p.cfg.n_bins);
     J.key_bits = J.cell_bits + p.sort.model_bits;
@@ -2068,8 +2080,8 @@ hipError_t sort_cell_keys_device(const float2* xy, uint32_t n, uint32_t grid_res
     J.proj_lo = J.proj_hi = 0.0f;
     J.n = n;
     J.grid_res = grid_res;
-    J.tpd = (grid_res + 15u) / 16u;
-    J.n_bins = J.tpd * J.tpd * (uint32_t)kTileCells;
+    J.key_order = cell_key_order(grid_res);
+    J.n_bins = 1u << (2u * J.key_order);
     J.mat = nullptr;
     J.mats = nullptr;
     J.n_mats = 0;

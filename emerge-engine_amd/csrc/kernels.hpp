@@ -17,7 +17,6 @@ constexpr int kPer = kChunk / kBlock;
 // velocity per cell (48 B: 36 KiB, with the u16 processing order under the
 // 40 KiB that keeps four 256-lane workgroups per CU).  A multiple of kBlock.
 constexpr int kLdsCells = 768;
-constexpr int kTileCells = 256;    // 16 x 16 cells per sort tile
 // Default interval of the physical cell-key reorder, in substeps (the upload
 // is always reordered at the first step).  The in-chunk sort keeps the runs
 // short between reorders, and on the 1M-particle sand column no interval pays:

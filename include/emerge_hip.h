@@ -487,10 +487,9 @@ int emh_sim_autotune(emh_sim* sim, uint32_t trial_steps, emh_tuning* chosen, emh
  * counting sort with a non-deterministic order inside a block,
  * src/systems/gpu/shaders/particle_sort.wgsl:94-245).  This is the device
  * cell key the substep reorders particles by: key = emh_cell_key(floor(x),
- * floor(y)) -- 16x16-cell tiles numbered column by column with odd columns
- * reversed (serpentine); inside a tile 16 rows of 16 cells, the rows in the
- * tile column's vertical direction, x ascending in a row -- stable by input
- * index.  keys_out[i] is the key of input i; perm_out[j] is the input
+ * floor(y)) -- the cell's index along the Hilbert curve over the smallest
+ * 2^k x 2^k square (k >= 1) that holds the grid, coordinates clamped into the
+ * grid -- stable by input index.  keys_out[i] is the key of input i; perm_out[j] is the input
  * index at sorted position j. */
 uint32_t emh_cell_key(int32_t cx, int32_t cy, uint32_t grid_res);
 int emh_sort_cell_keys(const float* xy, size_t n, uint32_t grid_res,

@@ -678,21 +678,31 @@ void orc_material_nacc(float mu, float kappa, float friction, float cohesion, fl
 }
 
 // ---------------------------------------------------------------- sort
-// Cell key (no reference counterpart: the reference's CPU path has no sort).
-// Tiles of 16x16 cells are numbered column by column, the odd columns
-// top-down (serpentine); inside a tile the 16 rows follow the same vertical
-// direction as the tile column, x ascending within a row.
+// Cell key (no reference counterpart: the reference's CPU path has no sort):
+// the index of the clamped cell along the Hilbert curve over the smallest
+// 2^k x 2^k square (k >= 1) holding the R x R grid, built from the top
+// quadrant down: quadrant digit (3 rx) ^ ry, then the lower bits reflected
+// (ry = 0, rx = 1) and transposed (ry = 0).
 uint32_t orc_cell_key(int32_t cx, int32_t cy, uint32_t R) {
     int32_t hi = (int32_t)R - 1;
     cx = std::min(std::max(cx, 0), hi);
     cy = std::min(std::max(cy, 0), hi);
-    const uint32_t tiles = (R + 15u) / 16u;
-    const uint32_t col = (uint32_t)cx / 16u, row = (uint32_t)cy / 16u;
-    const bool down = col % 2u == 1u;
-    const uint32_t row_in_col = down ? tiles - 1u - row : row;
-    const uint32_t lx = (uint32_t)cx % 16u, ly = (uint32_t)cy % 16u;
-    const uint32_t local_row = down ? 15u - ly : ly;
-    return (col * tiles + row_in_col) * 256u + local_row * 16u + lx;
+    uint32_t k = 1;
+    while (k < 16u && (1u << k) < R) ++k;
+    const uint32_t side = 1u << k;
+    uint32_t x = (uint32_t)cx, y = (uint32_t)cy, key = 0;
+    for (uint32_t q = side / 2u; q >= 1u; q /= 2u) {
+        const uint32_t rx = (x & q) != 0u, ry = (y & q) != 0u;
+        key += q * q * ((3u * rx) ^ ry);
+        if (ry == 0u) {
+            if (rx == 1u) {
+                x = side - 1u - x;
+                y = side - 1u - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return key;
 }
 
 // std::stable_sort of input indices by the cell key of floor(x).

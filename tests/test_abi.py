@@ -139,42 +139,39 @@ def test_cell_key_matches_oracle(oracle):
 
 @pytest.mark.parametrize("r", [16, 64, 100])
 def test_cell_key_layout(r):
-    """Properties that fix the key without restating it: a bijection from the
-    R x R cells into [0, 256 * ceil(R/16)^2); 256 consecutive keys per 16x16
-    tile; consecutive tiles share an edge (serpentine); inside the key order,
-    consecutive rows are vertically adjacent, so every run of 64 keys covers a
-    compact strip: a bounding box of at most 5 partial rows of 16 cells (80)
-    inside a tile column, or 4 rows across two tile columns at a turn (128).
-    A tile-local column-major order would reach 16 x 32 = 512 where a run
-    crosses from one tile into the next."""
+    """Properties that fix the key without restating it: a one-to-one map of
+    the R x R cells into [0, side^2), side the least power of two >= R;
+    cells with consecutive keys are edge neighbours (a continuous curve); an
+    aligned run of 4^j keys fills an aligned 2^j x 2^j square; so every run of
+    64 keys covers at most two neighbouring 8 x 8 squares (bounding box <= 128
+    cells).  A tiled or row order fails the neighbour check, a row order
+    inside quadrants the square check."""
     keys = np.array([[eh.cell_key(x, y, r) for y in range(r)] for x in range(r)])
-    tpd = (r + 15) // 16
-    assert len(np.unique(keys)) == r * r and keys.min() >= 0 and keys.max() < 256 * tpd * tpd
-    tiles = keys // 256
-    for tx in range(tpd):
-        for ty in range(tpd):
-            blk = tiles[16 * tx:16 * tx + 16, 16 * ty:16 * ty + 16]
-            assert np.all(blk == blk.flat[0])
-    origin = {}
-    for tx in range(tpd):
-        for ty in range(tpd):
-            origin[int(tiles[16 * tx, 16 * ty])] = (tx, ty)
-    order = [origin[t] for t in sorted(origin)]
-    for (ax, ay), (bx, by) in zip(order, order[1:]):
-        assert abs(ax - bx) + abs(ay - by) == 1
+    side = 2
+    while side < r:
+        side *= 2
+    assert len(np.unique(keys)) == r * r and keys.min() >= 0 and keys.max() < side * side
+    cell_of = {int(keys[x, y]): (x, y) for x in range(r) for y in range(r)}
+    for k, (x, y) in cell_of.items():
+        if k + 1 in cell_of:
+            nx, ny = cell_of[k + 1]
+            assert abs(nx - x) + abs(ny - y) == 1, (k, (x, y), (nx, ny))
+    if r == side:
+        assert keys.max() == r * r - 1 and cell_of[0] == (0, 0)
+        for j in (1, 2, 3):
+            n = 4 ** j
+            for k0 in range(0, r * r, n):
+                c = np.array([cell_of[k] for k in range(k0, k0 + n)])
+                lo = c.min(axis=0)
+                assert np.all(c.max(axis=0) - lo == 2 ** j - 1) and np.all(lo % 2 ** j == 0), (j, k0)
+        ordered = np.array([cell_of[k] for k in range(r * r)])
+        worst = 0
+        for s0 in range(0, len(ordered) - 63):
+            wnd = ordered[s0:s0 + 64]
+            worst = max(worst, int(np.ptp(wnd[:, 0]) + 1) * int(np.ptp(wnd[:, 1]) + 1))
+        assert worst <= 128, worst
     # clamped outside the grid
     assert eh.cell_key(-3, 5, r) == eh.cell_key(0, 5, r) and eh.cell_key(r + 9, r, r) == eh.cell_key(r - 1, r - 1, r)
-    # compact runs: cells in key order, every window of 64 (full tiles only: a
-    # ragged last tile row / column skips the keys of cells outside the grid)
-    if r % 16:
-        return
-    xy = np.argwhere(np.ones((r, r), bool))
-    ordered = xy[np.argsort(keys[xy[:, 0], xy[:, 1]])]
-    worst = 0
-    for s in range(0, len(ordered) - 63):
-        wnd = ordered[s:s + 64]
-        worst = max(worst, int(np.ptp(wnd[:, 0]) + 1) * int(np.ptp(wnd[:, 1]) + 1))
-    assert worst <= 128, worst
 
 
 def _no_gpu():
