@@ -439,6 +439,11 @@ struct emh_sim {
     uint32_t substeps_since_reorder = 0;
     uint32_t reorder_substeps = kReorderSubsteps;  // emh_sim_set_reorder_interval
     bool reorder_pending = false;                  // an upload: the next step reorders
+    // One-shot upload order (autotune's restore): slot s of the next upload
+    // takes host[upload_order[s]], and the reorder is pending only if it was
+    // when the order was saved, so a restored scene keeps its stored order.
+    std::vector<uint32_t> upload_order;
+    bool upload_order_pending = false;
     uint64_t seq = 0;  // sequence number of the next control launch (see Ctrl)
     // learned NACC return mapping: 0 plain, 1 harvest trial states, 2 infer
     uint32_t sg_mode = 0;
@@ -766,8 +771,10 @@ struct emh_sim {
         std::vector<float> mass(n), v0(n), vol(n), dens(n), jp(n), h(n), q(n), lvs(n);
         std::vector<uint32_t> mat(n), id(n);
         bool any_sleeping = false;
+        const bool keep_order = upload_order.size() == n;
         for (uint32_t i = 0; i < n; ++i) {
-            const emh_particle& p = host[i];
+            const uint32_t src = keep_order ? upload_order[i] : i;
+            const emh_particle& p = host[src];
             x[i] = make_float2(p.x[0], p.x[1]);
             v[i] = make_float2(p.v[0], p.v[1]);
             c[i] = make_float4(p.velocity_gradient[0], p.velocity_gradient[1], p.velocity_gradient[2],
@@ -787,7 +794,7 @@ struct emh_sim {
             if (p.sleeping) mat[i] |= kSleepBit;
             if (p.activation != 0.0f) mat[i] |= kNoSleepBit;
             any_sleeping = any_sleeping || p.sleeping != 0;
-            id[i] = i;
+            id[i] = src;
         }
         const PBuf& b = pipe.buf[0];
         auto put = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
@@ -826,7 +833,10 @@ struct emh_sim {
         HIP_TRY(launch_zero_all(pipe, stream));
         HIP_TRY(launch_prologue(pipe, seq++, stream, sleep));
         HIP_TRY(hipStreamSynchronize(stream));
-        reorder_pending = true;  // the first step puts the upload in cell-key order
+        // the first step puts the upload in cell-key order (unless it restores
+        // a saved stored order)
+        reorder_pending = keep_order ? upload_order_pending : true;
+        upload_order.clear();
         host_dirty = false;
         device_ahead = false;
         cold_valid = false;
@@ -887,10 +897,13 @@ struct emh_sim {
     }
 
     // Time each tuning candidate on this scene: from the same saved state,
-    // one untimed step (upload, reorder, graph capture), then `trial` steps
-    // timed on the host clock up to their completion.  The state is put back
-    // afterwards (re-uploaded from the saved copy at the next step, stats
-    // restored) and the fastest candidate applied.
+    // one untimed step (upload, graph capture), then `trial` steps timed on
+    // the host clock up to their completion.  The state is put back afterwards
+    // (re-uploaded from the saved copy at the next step in the stored order it
+    // had, stats restored) and the fastest candidate applied.  Keeping the
+    // stored order matters: a freshly reordered state of the 1M-particle sand
+    // column runs later substeps up to ~8 us slower than the order it replaced,
+    // so a restore that reordered would both bias the trials and slow the run.
     int autotune(uint32_t trial, emh_tuning* best, emh_tuning* timed, uint32_t cap, uint32_t* n_timed) {
         if (trial == 0) return fail(EMH_ERR_INVALID_ARG, "trial_steps must be at least 1");
         if (profiling) return fail(EMH_ERR_STATE, "turn profiling off before autotuning");
@@ -901,6 +914,13 @@ struct emh_sim {
         int rc = host_current();
         if (rc) return rc;
         const std::vector<emh_particle> saved = host;
+        std::vector<uint32_t> saved_order;  // caller index of each slot, when on the device
+        const bool saved_pending = reorder_pending;
+        if (!host_dirty && pipe.n == (uint32_t)host.size() && pipe.n > 0) {
+            saved_order.resize(pipe.n);
+            rc = slot_order(saved_order.data());
+            if (rc) return rc;
+        }
         const emh_step_stats saved_stats = stats;
         const uint32_t saved_since = substeps_since_reorder, saved_pred = predicted_substeps;
         const emh_tuning saved_tuning = tuning();
@@ -912,6 +932,8 @@ struct emh_sim {
         std::vector<float> us(cands.size(), 0.0f);
         auto restore = [&]() {
             host = saved;
+            upload_order = saved_order;
+            upload_order_pending = saved_pending;
             host_dirty = true;
             device_ahead = false;  // the host copy, not the last trial's device state, is current
             stats = saved_stats;
