@@ -930,10 +930,12 @@ struct emh_sim {
         }
         const std::vector<emh_tuning> cands = tuning_candidates(mixed_models());
         std::vector<float> us(cands.size(), 0.0f);
-        auto restore = [&]() {
+        // a candidate whose reorder key differs from the one the stored order
+        // was sorted by (per-material segments) starts with a reorder
+        auto restore = [&](bool rekey) {
             host = saved;
             upload_order = saved_order;
-            upload_order_pending = saved_pending;
+            upload_order_pending = saved_pending || rekey;
             host_dirty = true;
             device_ahead = false;  // the host copy, not the last trial's device state, is current
             stats = saved_stats;
@@ -941,7 +943,7 @@ struct emh_sim {
             predicted_substeps = saved_pred;
         };
         for (size_t c = 0; c < cands.size() && rc == EMH_OK; ++c) {
-            restore();
+            restore(cands[c].model_segments != saved_tuning.model_segments);
             apply_tuning(cands[c]);
             rc = step(1);
             if (rc) break;
@@ -952,12 +954,12 @@ struct emh_sim {
             const double work = (double)std::max<uint64_t>(stats.total_substeps - s0, 1) * (double)pipe.n;
             us[c] = (float)(std::chrono::duration<double, std::nano>(t1 - t0).count() / work);
         }
-        restore();
-        apply_tuning(saved_tuning);
-        if (rc) return rc;
         size_t bi = 0;
         for (size_t c = 1; c < cands.size(); ++c)
             if (us[c] < us[bi]) bi = c;
+        restore(rc == EMH_OK && cands[bi].model_segments != saved_tuning.model_segments);
+        apply_tuning(saved_tuning);
+        if (rc) return rc;
         for (size_t c = 0; c < cands.size() && c < cap && timed; ++c) {
             timed[c] = cands[c];
             timed[c].ns_per_particle_substep = us[c];
