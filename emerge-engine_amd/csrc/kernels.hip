@@ -241,13 +241,36 @@ __device__ __forceinline__ const DevMat& mat_of(const Pipeline& p, uint32_t word
 
 // CFL contribution of one particle as choose_substep_dt reads it, before
 // projection (step.rs:445-466): speed incl. affine term, material dt bound.
+// For the models whose bound is elastic_wave_dt(lambda, mu, 1, density, ...)
+// the bound depends on the particle only through max(density, kMinJ) and
+// never decreases with it (a correctly rounded quotient and square root are
+// monotone, and c <= eps gives +inf), so in a scene of one material the
+// minimum over particles is the bound at the smallest clamped density:
+// density_bound_models() scenes reduce that density (mdt holds it) and
+// cfl_bound() evaluates the bound once per workgroup.
+template <int ONLY>
+constexpr bool density_bound_model() {
+    return ONLY == (int)kModelDP || ONLY == (int)kModelVonMises || ONLY == (int)kModelRankine ||
+           ONLY == (int)kModelMuI;
+}
 template <int ONLY>
 __device__ __forceinline__ void cfl_of(const Pipeline& p, const PState& s, float& speed, float& mdt) {
     float sp = len2(s.vx, s.vy);
     if (p.cfg.cfl_affine) sp = sp + affine_cfl(s.c, p.cfg.cell);
     speed = fmaxf(speed, sp);
+    if (density_bound_model<ONLY>() && p.n_mats == 1u) {
+        mdt = fminf(mdt, rmax(s.dens, kMinJ));
+        return;
+    }
     const float b = timestep_bound<ONLY>(mat_of(p, s.mat), s.dens, s.h, p.cfg);
     if (__builtin_isfinite(b) && b > 0.0f) mdt = fminf(mdt, b);
+}
+// The workgroup's material dt bound from what cfl_of accumulated.
+template <int ONLY>
+__device__ __forceinline__ float cfl_bound(const Pipeline& p, float mdt) {
+    if (!(density_bound_model<ONLY>() && p.n_mats == 1u) || !(mdt < kInf)) return mdt;
+    const float b = timestep_bound<ONLY>(p.mats[0], mdt, 1.0f, p.cfg);
+    return (__builtin_isfinite(b) && b > 0.0f) ? b : kInf;
 }
 
 // The decision of control launch s from the state launch s - 1 left and its
@@ -1606,7 +1629,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         if (wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
         if (wd) atomicOr(&red->dirty, 1u);
     }
-    reduce_cfl(red, speed, mdt, s_red);
+    reduce_cfl(red, speed, cfl_bound<ONLY>(p, mdt), s_red);
 }
 
 template <int MODE, int ONLY>
