@@ -93,6 +93,29 @@ def test_autotune_restores_state_and_applies_choice():
 
 
 @pytest.mark.gpu
+def test_autotune_keeps_the_stored_order():
+    """The trials and the restored scene start from the particles' stored slot order, not from
+    a fresh cell-key sort: after tuning, a step with no reorder due leaves every slot as it was
+    before tuning (a restore that re-sorted the upload would put the particles that changed
+    cells since the last reorder into other slots)."""
+    sim = S.sand_column("gpu")
+    sim.set_reorder_interval(1_000_000)
+    sim.step_n(3)
+    order = sim.slot_order()
+    sim.autotune(trial_steps=2)
+    sim.set_reorder_interval(1_000_000)
+    sim.step_n(1)
+    assert np.array_equal(sim.slot_order(), order)
+    # the same scene re-sorted differs from that order: the check can fail
+    fresh = S.sand_column("gpu")
+    fresh.set_particles(sim.particles())
+    fresh.step_n(1)
+    assert not np.array_equal(fresh.slot_order(), order)
+    sim.close()
+    fresh.close()
+
+
+@pytest.mark.gpu
 def test_batch_autotune_and_cache(tmp_path):
     names = ["corotated_drop", "snow_drop", "sand_column"]
     a = [S.SCENES[n].build("gpu") for n in names]
