@@ -700,19 +700,7 @@ struct emh_sim {
             else if (only != mats[i].model)
                 only = kModelMixed;
             first = false;
-            switch (mats[i].model) {
-                case EMH_MODEL_NEOHOOKEAN: fields |= kFQ; break;
-                case EMH_MODEL_DRUCKER_PRAGER: fields |= kFQ | kFLvs; break;
-                case EMH_MODEL_VON_MISES: fields |= kFQ; break;
-                case EMH_MODEL_RANKINE: fields |= kFQ; break;
-                case EMH_MODEL_MU_I: fields |= kFQ; break;
-                case EMH_MODEL_GRANULAR_FLUID: fields |= kFJp | kFH | kFVolDens; break;
-                case EMH_MODEL_VISCOELASTIC: break;
-                case EMH_MODEL_COROTATED: fields |= kFH; break;
-                case EMH_MODEL_SNOW: fields |= kFJp | kFH; break;
-                case EMH_MODEL_NACC: fields |= kFLvs | kFH | kFVolDens; break;
-                default: fields |= kFVolDens; break;  // fluid, fallback: volume/density carried over
-            }
+            fields |= model_fields(mats[i].model);  // fluid, fallback: volume/density carried over
         }
         d.fields = fields;
         d.only_model = only;

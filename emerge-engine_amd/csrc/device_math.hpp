@@ -124,6 +124,18 @@ struct DevMat {
 constexpr uint32_t kModelFallback = 0, kModelFluid = 1, kModelNH = 2, kModelCorot = 3, kModelSnow = 4,
                    kModelDP = 5, kModelVonMises = 6, kModelRankine = 7, kModelMuI = 8, kModelViscoelastic = 9,
                    kModelNacc = 10, kModelGranular = 11, kModelMixed = 0xffffffffu;
+// The optional particle fields (kF*) a particle of `model` reads or writes;
+// DevConfig::fields is their union over the scene's models.
+__host__ __device__ constexpr uint32_t model_fields(uint32_t model) {
+    return (model == kModelNH || model == kModelVonMises || model == kModelRankine || model == kModelMuI) ? kFQ
+           : model == kModelDP                                                                      ? (kFQ | kFLvs)
+           : model == kModelGranular                                                     ? (kFJp | kFH | kFVolDens)
+           : model == kModelViscoelastic                                                                   ? 0u
+           : model == kModelCorot                                                                          ? kFH
+           : model == kModelSnow                                                                  ? (kFJp | kFH)
+           : model == kModelNacc                                                       ? (kFLvs | kFH | kFVolDens)
+                                                                                                    : kFVolDens;
+}
 constexpr uint32_t kMatNaccHardening = 1u;  // NACC: hardening enabled
 constexpr uint32_t kMatNaccAdvance = 2u;    // NACC: project (I + dt C) F
 constexpr uint32_t kMatBingham = 4u;        // Fluid: Bingham viscoplastic deviatoric law

@@ -52,6 +52,16 @@ namespace {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
 __device__ __forceinline__ PBuf sel_buf(const Pipeline& p, uint32_t i) { return i ? p.buf[1] : p.buf[0]; }
+// The buffer a reorder substep stores into, read where it stores: the index
+// passes through an empty asm definition, so the compiler cannot hoist the
+// buffer's fourteen pointers out of the particle loop and keep them live in
+// SGPRs through every substep (those spilled into VGPR lanes, and their
+// reloads were VALU instructions on the loop's path).
+__device__ __forceinline__ PBuf reorder_buf(const Pipeline& p, uint32_t i) {
+    i = __builtin_amdgcn_readfirstlane(i);  // uniform: the pipeline's control state
+    asm volatile("" : "+s"(i));
+    return p.buf[i & 1u];
+}
 
 // Global float add (every caller passes grid memory): the address-space cast
 // keeps it a global atomic where the pointer was loaded from memory (the
@@ -1440,9 +1450,10 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     const bool project_in = cfg.project && (G2P ? (Q.dirty && !cfg.density_pass) : true);
     const bool reorder = G2P && P.reorder;
     const bool full = !G2P || project_in || reorder;
-    const uint32_t mask = full ? kFAll : cfg.fields;
+    // a one-model instance's fields are a constant (the host's fields of that model alone)
+    const uint32_t mask = full ? kFAll : (ONLY >= 0 ? model_fields((uint32_t)ONLY) : cfg.fields);
     const PBuf a = sel_buf(p, P.cur_buf);
-    const PBuf o = reorder ? sel_buf(p, P.cur_buf ^ 1u) : a;
+    const uint32_t other_buf = P.cur_buf ^ 1u;
     const uint32_t chunk = BATCH ? blk : xcd_chunk(blk, nblk);
     const uint32_t base = chunk * kChunk;
 
@@ -1567,10 +1578,16 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                     any_dirty |= needs_projection(s, sp, cfg);
                 }
                 if (G2P && (!SLEEP || !asleep || mat_changed || reorder)) {
-                    const uint32_t dest = reorder ? p.sort.out[j] : j;
-                    store_fields(o, dest, s, mask, full);
-                    if (reorder) o.id[dest] = a.id[j];
-                    if (SLEEP && mat_changed && !full) o.mat[dest] = s.mat;
+                    if (reorder) {
+                        const PBuf o = reorder_buf(p, other_buf);
+                        const uint32_t dest = p.sort.out[j];
+                        store_fields(o, dest, s, mask, full);
+                        o.id[dest] = a.id[j];
+                        if (SLEEP && mat_changed && !full) o.mat[dest] = s.mat;
+                    } else {
+                        store_fields(a, j, s, mask, full);
+                        if (SLEEP && mat_changed && !full) a.mat[j] = s.mat;
+                    }
                 }
             }
         }
@@ -1581,8 +1598,13 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 cfl_of<ONLY>(p, s, speed, mdt);
                 any_dirty |= needs_projection(s, sp, cfg);
                 const uint32_t dest = reorder ? p.sort.out[j] : j;
-                store_fields(o, dest, s, mask, full);
-                if (reorder) o.id[dest] = a.id[j];
+                if (reorder) {
+                    const PBuf o = reorder_buf(p, other_buf);
+                    store_fields(o, dest, s, mask, full);
+                    o.id[dest] = a.id[j];
+                } else {
+                    store_fields(a, dest, s, mask, full);
+                }
             }
         }
         // 5. P2G of the projected state into the next accumulator set (the
