@@ -57,6 +57,13 @@ __device__ __forceinline__ PBuf sel_buf(const Pipeline& p, uint32_t i) { return 
 // buffer's fourteen pointers out of the particle loop and keep them live in
 // SGPRs through every substep (those spilled into VGPR lanes, and their
 // reloads were VALU instructions on the loop's path).
+// A uniform index made opaque to the optimizer, for loads that only a rare
+// path needs inside the particle loop (see reorder_buf).
+__device__ __forceinline__ uint32_t opaque_index(uint32_t i) {
+    i = __builtin_amdgcn_readfirstlane(i);
+    asm volatile("" : "+s"(i));
+    return i;
+}
 __device__ __forceinline__ PBuf reorder_buf(const Pipeline& p, uint32_t i) {
     i = __builtin_amdgcn_readfirstlane(i);  // uniform: the pipeline's control state
     asm volatile("" : "+s"(i));
@@ -698,9 +705,22 @@ __device__ __forceinline__ void p2g_channel(const P2GTerms& t, float (&v)[9]) {
 // (or to the global grid when the stencil leaves the box).  Lanes that do not
 // scatter carry zero terms (scatter_terms), so their values are zero.
 // at: the stencil's LDS index (Box::stencil_at), -1 for the global grid.
+// The accumulator set a P2G scatters into, for its global-memory fallback:
+// channel ch of acc[set], or the density pass's mass grid dm[set].  The
+// pointer is read where a stencil outside the LDS box needs it, through an
+// opaque index, instead of being held in SGPRs through the particle loop.
+struct GridDst {
+    uint32_t set;
+    uint32_t mass;
+};
+__device__ __forceinline__ float* grid_dst(const Pipeline& p, GridDst d, int ch) {
+    const uint32_t s = opaque_index(d.set);
+    return d.mass ? p.grid.dm[s] : p.grid.acc[s][ch];
+}
+
 template <bool FMA, int CH>
 __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool tail, const ScanPred& sq, const Box& box, int at,
-                                                double* s_ch, float* g_ch, int R) {
+                                                double* s_ch, const Pipeline& p, GridDst gd, int R) {
     float v[9];
     p2g_channel<CH>(t, v);
     seg_scan<FMA>(v, sq);
@@ -713,6 +733,7 @@ __device__ __forceinline__ void scatter_channel(const P2GTerms& t, bool tail, co
 #pragma unroll
                 for (int gy = 0; gy < 3; ++gy) latomic_add(&c[gx * box.h + gy], (double)v[gx * 3 + gy]);
         } else {
+            float* g_ch = grid_dst(p, gd, CH);
 #pragma unroll
             for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
@@ -1217,15 +1238,15 @@ __device__ __forceinline__ P2GTerms p2g_terms(const Pipeline& p, const PState& s
 
 template <bool FMA, int NCH>
 __device__ __forceinline__ void scatter_channels(const P2GTerms& z, bool tail, const ScanPred& sq, const Box& box, int at,
-                                                 double (*s_acc)[kLdsCells], float* const* g, int R) {
-    scatter_channel<FMA, kAccM>(z, tail, sq, box, at, s_acc[kAccM], g[kAccM], R);
+                                                 double (*s_acc)[kLdsCells], const Pipeline& p, GridDst gd, int R) {
+    scatter_channel<FMA, kAccM>(z, tail, sq, box, at, s_acc[kAccM], p, gd, R);
     if constexpr (NCH > 1) {
-        scatter_channel<FMA, kAccPx>(z, tail, sq, box, at, s_acc[kAccPx], g[kAccPx], R);
-        scatter_channel<FMA, kAccPy>(z, tail, sq, box, at, s_acc[kAccPy], g[kAccPy], R);
+        scatter_channel<FMA, kAccPx>(z, tail, sq, box, at, s_acc[kAccPx], p, gd, R);
+        scatter_channel<FMA, kAccPy>(z, tail, sq, box, at, s_acc[kAccPy], p, gd, R);
     }
     if constexpr (NCH == kAccChannels) {
-        scatter_channel<FMA, kAccSx>(z, tail, sq, box, at, s_acc[kAccSx], g[kAccSx], R);
-        scatter_channel<FMA, kAccSy>(z, tail, sq, box, at, s_acc[kAccSy], g[kAccSy], R);
+        scatter_channel<FMA, kAccSx>(z, tail, sq, box, at, s_acc[kAccSx], p, gd, R);
+        scatter_channel<FMA, kAccSy>(z, tail, sq, box, at, s_acc[kAccSy], p, gd, R);
     }
 }
 
@@ -1236,7 +1257,7 @@ __device__ __forceinline__ void scatter_channels(const P2GTerms& z, bool tail, c
 // finite, so the fma form of the scan is exact.
 template <int NCH = kAccChannels>
 __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const Box& box,
-                                            double (*s_acc)[kLdsCells], float* const* g, int R) {
+                                            double (*s_acc)[kLdsCells], const Pipeline& p, GridDst gd, int R) {
     static_assert(NCH == kAccChannels || NCH == kAccPy + 1 || NCH == 1, "all channels, mass and momentum, or mass");
     const SegMask sm = run_mask(valid ? t.w.bx : INT_MIN + (int)lane_id(), valid ? t.w.by : 0);
     const ScanPred sq = scan_pred(sm);
@@ -1252,9 +1273,9 @@ __device__ __forceinline__ void scatter_p2g(const P2GTerms& t, bool valid, const
     // one branch per particle round: the scans of the fast path feed the
     // atomics directly
     if (fma_ok)
-        scatter_channels<true, NCH>(z, tail, sq, box, at, s_acc, g, R);
+        scatter_channels<true, NCH>(z, tail, sq, box, at, s_acc, p, gd, R);
     else
-        scatter_channels<false, NCH>(z, tail, sq, box, at, s_acc, g, R);
+        scatter_channels<false, NCH>(z, tail, sq, box, at, s_acc, p, gd, R);
 }
 
 // Processing order of slice k of a sliced chunk (slots k*kBlock .. +kBlock),
@@ -1328,9 +1349,13 @@ __device__ __forceinline__ void stage_box(const Pipeline& p, const Box& box, uin
 
 // Add the LDS box into the global grid: one atomic per touched cell and channel.
 template <int NCH>
-__device__ __forceinline__ void flush_box(const Box& box, double (*s_acc)[kLdsCells], float* const* g, int R) {
+__device__ __forceinline__ void flush_box(const Box& box, double (*s_acc)[kLdsCells], const Pipeline& p, GridDst gd,
+                                          int R) {
     if (!box.lds) return;
     __syncthreads();
+    float* g[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) g[ch] = grid_dst(p, gd, ch);
     for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
         const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
         if (cx < 0 || cy < 0 || cx >= R || cy >= R) continue;
@@ -1474,9 +1499,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     // 2. clear the LDS accumulators; stage the updated grid velocity
     if (!sliced) stage_box<G2P, NCH>(p, box, cur, dt, ddt, s_acc, s_vel);
 
-    float* gdst[NCH];
-#pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) gdst[ch] = P2G ? p.grid.acc[nxt][ch] : p.grid.dm[nxt];
+    const GridDst gd{nxt, P2G ? 0u : 1u};
 
     float speed = 0.0f, mdt = kInf;
     uint32_t n_clamp = 0, n_proj = 0;
@@ -1541,7 +1564,9 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
                 for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
                     for (int gy = 0; gy < 3; ++gy) {
-                        if (!lds) nv[gx * 3 + gy] = cell_velocity(p, cur, dt, ddt, w.bx + gx - 1, w.by + gy - 1);
+                        if (!lds)
+                            nv[gx * 3 + gy] =
+                                cell_velocity(p, opaque_index(cur), dt, ddt, w.bx + gx - 1, w.by + gy - 1);
                         gather(gx, gy, nv[gx * 3 + gy]);
                     }
                 if (s.mat & kPinnedBit) {
@@ -1612,7 +1637,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         const bool scat = valid && !asleep;
         if (P2G) {
             const P2GTerms t = p2g_terms<ONLY>(p, sp, pred);
-            scatter_p2g(t, scat, box, s_acc, gdst, R);
+            scatter_p2g(t, scat, box, s_acc, p, gd, R);
         } else {
             P2GTerms t;
             t.w = weights(sp.x, sp.y);
@@ -1621,16 +1646,16 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
             t.mass = sp.mass;
             t.ux = t.uy = 0.0f;
             t.k = t.st = M2{0.0f, 0.0f, 0.0f, 0.0f};
-            scatter_p2g<1>(t, scat, box, s_acc, gdst, R);
+            scatter_p2g<1>(t, scat, box, s_acc, p, gd, R);
         }
         if (sliced) {
-            flush_box<NCH>(box, s_acc, gdst, R);
+            flush_box<NCH>(box, s_acc, p, gd, R);
             __syncthreads();  // the next slice stages into the same LDS
         }
     }
 
     // 6. flush the LDS box into the next accumulator set
-    if (!sliced) flush_box<NCH>(box, s_acc, gdst, R);
+    if (!sliced) flush_box<NCH>(box, s_acc, p, gd, R);
     // 7. clear this workgroup's slice of the set the substep after next uses
     if (G2P) {
         const uint32_t R2 = (uint32_t)R * (uint32_t)R;
@@ -1686,9 +1711,7 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     const uint32_t chunk = BATCH ? blk : xcd_chunk(blk, nblk);
     const uint32_t base = chunk * kChunk;
     const float* dm = p.grid.dm[cur];
-    float* gdst[kDensCh];
-#pragma unroll
-    for (int ch = 0; ch < kDensCh; ++ch) gdst[ch] = p.grid.acc[cur][ch];
+    const GridDst gd{cur, 0u};
 
     const Box box = chunk_box(p, a, base, project_in, 1, s_box);
     if (box.lds) {
@@ -1742,9 +1765,9 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
             }
         }
         const P2GTerms t = p2g_terms<-1>(p, s, N.dt);
-        scatter_p2g<kDensCh>(t, valid && !asleep, box, s_acc, gdst, R);
+        scatter_p2g<kDensCh>(t, valid && !asleep, box, s_acc, p, gd, R);
     }
-    flush_box<kDensCh>(box, s_acc, gdst, R);
+    flush_box<kDensCh>(box, s_acc, p, gd, R);
     const uint32_t wp = wave_sum(n_proj);
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
