@@ -294,7 +294,7 @@ __device__ __forceinline__ float cfl_bound(const Pipeline& p, float mdt) {
 // CFL reduction: the loop condition of Simulation::step (step.rs:34) --
 // finishing the step and starting the next queued one when it fails -- then
 // choose_substep_dt / cfl_bound (step.rs:434-479).
-__device__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const DevConfig& cfg) {
+__device__ __forceinline__ CtrlState decide_substep(const CtrlState& P, const CtrlRed& Q, const DevConfig& cfg) {
     CtrlState N = P;
     N.active = 0;
     bool running = P.in_step && P.remaining > kF32Eps && P.substeps < cfg.max_substeps;
@@ -437,15 +437,23 @@ __device__ __forceinline__ float2 cell_velocity(const Pipeline& p, uint32_t set,
     const int R = (int)cfg.grid_res;
     if (cx < 0 || cy < 0 || cx >= R || cy >= R) return make_float2(0.0f, 0.0f);
     const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
+    // every channel the cell needs is read before the mass is tested: one
+    // round trip to memory per cell instead of two (the stencil's nine cells
+    // of a particle outside its LDS box then go out together)
     const float m = p.grid.acc[set][kAccM][gi];
+    float vx = p.grid.acc[set][kAccPx][gi];
+    float vy = p.grid.acc[set][kAccPy][gi];
+    float sx = 0.0f, sy = 0.0f;
+    if (ddt != 0.0f) {
+        sx = p.grid.acc[set][kAccSx][gi];
+        sy = p.grid.acc[set][kAccSy][gi];
+    }
     if (!(m > 0.0f)) return make_float2(0.0f, 0.0f);
     // momentum = A + (dt - dt') S (ddt = dt - dt' is 0 when the prediction held,
     // and always for scenes with a density pass, which never write S)
-    float vx = p.grid.acc[set][kAccPx][gi];
-    float vy = p.grid.acc[set][kAccPy][gi];
     if (ddt != 0.0f) {
-        vx = vx + ddt * p.grid.acc[set][kAccSx][gi];
-        vy = vy + ddt * p.grid.acc[set][kAccSy][gi];
+        vx = vx + ddt * sx;
+        vy = vy + ddt * sy;
     }
     vx = vx / m;
     vy = vy / m;
@@ -607,12 +615,15 @@ __device__ __forceinline__ bool run_tail(const SegMask& m) {
 
 // ------------------------------------------------------------- chunk box
 // LDS box of grid cells: [ox, ox + w) x [oy, oy + h), cell (cx, cy) at
-// (cx - ox) * h + (cy - oy).  `lds` is false when the box exceeds the budget
-// (sparse chunks); those chunks read and scatter the global grid directly.
+// lo + (cx - ox) * h + (cy - oy) of the LDS arrays (lo: 0, or the offset of
+// one slice's box among the boxes of a chunk's slices).  `lds` is false when
+// the box exceeds the budget (sparse particles); those read and scatter the
+// global grid directly.
 struct Box {
     int ox, oy, w, h;
+    int lo;
     bool lds;
-    __device__ __forceinline__ int index(int cx, int cy) const { return (cx - ox) * h + (cy - oy); }
+    __device__ __forceinline__ int index(int cx, int cy) const { return lo + (cx - ox) * h + (cy - oy); }
     // LDS index of cell (bx - 1, by - 1) when the 3x3 stencil at base cell
     // (bx, by) (cells bx-1..bx+1, by-1..by+1) lies in the box, else -1;
     // stencil cell (gx, gy) is then at + gx * h + gy.  The test 0 <= bx - 1 -
@@ -622,7 +633,7 @@ struct Box {
     // w, h >= 3: margin >= 1).
     __device__ __forceinline__ int stencil_at(int bx, int by) const {
         const uint32_t lx = (uint32_t)bx - 1u - (uint32_t)ox, ly = (uint32_t)by - 1u - (uint32_t)oy;
-        return (lds && lx <= (uint32_t)(w - 3) && ly <= (uint32_t)(h - 3)) ? (int)(lx * (uint32_t)h + ly) : -1;
+        return (lds && lx <= (uint32_t)(w - 3) && ly <= (uint32_t)(h - 3)) ? lo + (int)(lx * (uint32_t)h + ly) : -1;
     }
 };
 __device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_box) {
@@ -654,6 +665,7 @@ __device__ Box block_box(int mnx, int mny, int mxx, int mxy, int margin, int* s_
     b.oy = b.lds ? mny - margin : 0;
     b.w = b.lds ? (int)w : 0;
     b.h = b.lds ? (int)h : 0;
+    b.lo = 0;
     return b;
 }
 
@@ -1113,7 +1125,8 @@ __device__ __forceinline__ void sleep_test(float sleep_sq, PState& s, bool& asle
 
 // Bounding box of the chunk's base cells (positions projected when the state
 // is flagged), widened by `margin` cells.
-__device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool project, int margin, int* s_box) {
+__device__ __forceinline__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool project, int margin,
+                                         int* s_box) {
     int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -1131,6 +1144,81 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
     return block_box(mnx, mny, mxx, mxy, margin, s_box);
 }
 
+// Sort bin of a base cell among nb bins for particles that scatter through
+// global memory: a hash of the cell, so that the particles of one cell end up
+// side by side (one run, one set of atomics per run) whatever the cells'
+// spread; cells that share a bin only interleave.
+__device__ __forceinline__ uint32_t hash_bin(int cx, int cy, uint32_t nb) {
+    const uint32_t h = ((uint32_t)cx * 0x9E3779B1u) ^ ((uint32_t)cy * 0x85EBCA77u);
+    return (h ^ (h >> 15)) % nb;
+}
+// Bins of a slice that runs through global memory inside a chunk's slice
+// boxes (the other slices' boxes permitting).
+constexpr int kGlobalSliceBins = 32;
+
+// Boxes of a chunk's kPer slices (slice k: slots k * kBlock .. + kBlock, the
+// particles of round k in stored order), one after another in the LDS
+// budget: a slice whose own box fits takes its w * h cells at offset lo; one
+// whose box does not fit runs through global memory and keeps
+// kGlobalSliceBins cells as sort bins (h holds their count; one for a slice
+// without particles).  Returns the cells used;
+// when that is within the budget the chunk stages, sorts and flushes all its
+// slices' boxes at once, like a chunk with one box, instead of running slice
+// by slice.  s_box: 4 * kPer * kWaves ints; s_mbox receives the boxes.
+__device__ __forceinline__ int slice_boxes(const Pipeline& p, const PBuf& a, uint32_t base, bool project, int margin,
+                                           int* s_box, Box* s_mbox) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t j = base + k * kBlock + threadIdx.x;
+        int bx = INT_MAX, by = INT_MAX, ex = INT_MIN, ey = INT_MIN;
+        if (j < p.n) {
+            float2 x = a.x[j];
+            if (project) project_x(x.x, x.y, p.cfg);
+            bx = ex = sat_i32(floorf(x.x));
+            by = ey = sat_i32(floorf(x.y));
+        }
+        bx = wave_imin(bx);
+        by = wave_imin(by);
+        ex = wave_imax(ex);
+        ey = wave_imax(ey);
+        if (lane == 0) {
+            int* o = s_box + 4 * (k * kWaves + (int)wave);
+            o[0] = bx;
+            o[1] = by;
+            o[2] = ex;
+            o[3] = ey;
+        }
+    }
+    __syncthreads();
+    int lo = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        int mnx = INT_MAX, mny = INT_MAX, mxx = INT_MIN, mxy = INT_MIN;
+        for (int w = 0; w < kWaves; ++w) {
+            const int* o = s_box + 4 * (k * kWaves + w);
+            mnx = min(mnx, o[0]);
+            mny = min(mny, o[1]);
+            mxx = max(mxx, o[2]);
+            mxy = max(mxy, o[3]);
+        }
+        const long long w = (long long)mxx - (long long)mnx + 1 + 2 * margin;
+        const long long h = (long long)mxy - (long long)mny + 1 + 2 * margin;
+        Box b;
+        b.lds = (mnx <= mxx) && (long long)mnx - margin > INT_MIN / 2 && (long long)mxx + margin < INT_MAX / 2 &&
+                (long long)mny - margin > INT_MIN / 2 && (long long)mxy + margin < INT_MAX / 2 && w * h <= kLdsCells;
+        b.ox = b.lds ? mnx - margin : 0;
+        b.oy = b.lds ? mny - margin : 0;
+        b.w = b.lds ? (int)w : 0;
+        b.h = b.lds ? (int)h : (mnx <= mxx ? kGlobalSliceBins : 1);
+        b.lo = lo;
+        lo += b.lds ? b.w * b.h : b.h;
+        if (threadIdx.x == 0) s_mbox[k] = b;
+    }
+    __syncthreads();
+    return lo;
+}
+
 // Processing order of a chunk: its nvalid particles sorted by base cell (a
 // counting sort over the box cells in LDS, arrival order inside a cell), so
 // that the lanes of a wave hold long runs of one cell however far the stored
@@ -1145,16 +1233,21 @@ __device__ Box chunk_box(const Pipeline& p, const PBuf& a, uint32_t base, bool p
 // particles per cell 9 % faster never sorting than above 8 runs per wave).
 // The distinct cells come from the sort's own counting pass: integer LDS
 // atomics run at full rate on gfx950.
+// With slice boxes (`mbox`, nbins cells in all), a particle's bin is its cell
+// in its slice's box, or its slice's single bin when that slice runs through
+// global memory; every slice holds kBlock particles but the last, so each
+// slice's particles stay in the slots of its round.
 __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, uint32_t base, bool project, const Box& box,
-                                            uint32_t nvalid, uint32_t* s_cnt, uint16_t* s_perm, uint32_t* s_scan) {
-    if (!box.lds) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
+                                            const Box* mbox, int nbins, uint32_t nvalid, uint32_t* s_cnt,
+                                            uint16_t* s_perm, uint32_t* s_scan) {
+    if (!box.lds && !mbox) {  // the chunk runs slice by slice; each slice is sorted on its own (slice_order)
 #pragma unroll
         for (int k = 0; k < kPer; ++k) s_perm[k * kBlock + threadIdx.x] = (uint16_t)(k * kBlock + threadIdx.x);
         __syncthreads();
         return;
     }
     __shared__ uint32_t s_runs, s_distinct;
-    const int cells = box.w * box.h;
+    const int cells = mbox ? nbins : box.w * box.h;
     for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
     if (threadIdx.x == 0) s_runs = s_distinct = 0u;
     __syncthreads();
@@ -1168,7 +1261,13 @@ __device__ __forceinline__ void chunk_order(const Pipeline& p, const PBuf& a, ui
         if (valid) {  // the base cell chunk_box saw (same projection)
             float2 x = a.x[base + q];
             if (project) project_x(x.x, x.y, p.cfg);
-            cell[k] = (uint32_t)box.index(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)));
+            const int cx = sat_i32(floorf(x.x)), cy = sat_i32(floorf(x.y));
+            if (mbox) {
+                const Box& b = mbox[k];
+                cell[k] = (uint32_t)(b.lds ? b.index(cx, cy) : b.lo + (int)hash_bin(cx, cy, (uint32_t)b.h));
+            } else {
+                cell[k] = (uint32_t)box.index(cx, cy);
+            }
         }
         // runs of the stored order: lanes whose cell differs from the lane before
         const uint32_t prev = (uint32_t)__shfl_up((int)cell[k], 1);
@@ -1285,18 +1384,17 @@ __device__ __forceinline__ void slice_order(const Pipeline& p, const PBuf& a, ui
                                             uint32_t* s_scan) {
     const uint32_t q = k * kBlock + threadIdx.x;
     const bool valid = q < nvalid;
-    if (!box.lds) {
-        __syncthreads();
-        return;
-    }
-    const int cells = box.w * box.h;
+    // a slice without an LDS box is sorted by hashed cell (its P2G atomics go
+    // to global memory once per run)
+    const int cells = box.lds ? box.w * box.h : kLdsCells;
     for (int i = threadIdx.x; i < cells; i += kBlock) s_cnt[i] = 0u;
     __syncthreads();
     uint32_t cell = 0u, rank = 0u;
     if (valid) {
         float2 x = a.x[base + q];
         if (project) project_x(x.x, x.y, p.cfg);
-        cell = (uint32_t)box.index(sat_i32(floorf(x.x)), sat_i32(floorf(x.y)));
+        const int cx = sat_i32(floorf(x.x)), cy = sat_i32(floorf(x.y));
+        cell = box.lds ? (uint32_t)box.index(cx, cy) : hash_bin(cx, cy, (uint32_t)kLdsCells);
         rank = atomicAdd(&s_cnt[cell], 1u);
     }
     __syncthreads();
@@ -1347,12 +1445,30 @@ __device__ __forceinline__ void stage_box(const Pipeline& p, const Box& box, uin
     __syncthreads();
 }
 
+// Stage and clear every LDS slice box of a chunk (one barrier for all).
+template <bool G2P, int NCH>
+__device__ __forceinline__ void stage_boxes(const Pipeline& p, const Box* mbox, uint32_t cur, float dt, float ddt,
+                                            double (*s_acc)[kLdsCells], float2* s_vel) {
+#pragma unroll 1
+    for (int k = 0; k < kPer; ++k) {
+        const Box b = mbox[k];
+        if (!b.lds) continue;
+        for (int i = threadIdx.x; i < b.w * b.h; i += kBlock) {
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) s_acc[ch][b.lo + i] = 0.0;
+            if (G2P) s_vel[b.lo + i] = cell_velocity(p, cur, dt, ddt, b.ox + i / b.h, b.oy + i % b.h);
+        }
+    }
+    __syncthreads();
+}
+
 // Add the LDS box into the global grid: one atomic per touched cell and channel.
+// (`sync`: barrier first, for the LDS adds of the whole workgroup)
 template <int NCH>
 __device__ __forceinline__ void flush_box(const Box& box, double (*s_acc)[kLdsCells], const Pipeline& p, GridDst gd,
-                                          int R) {
+                                          int R, bool sync = true) {
     if (!box.lds) return;
-    __syncthreads();
+    if (sync) __syncthreads();
     float* g[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) g[ch] = grid_dst(p, gd, ch);
@@ -1362,7 +1478,7 @@ __device__ __forceinline__ void flush_box(const Box& box, double (*s_acc)[kLdsCe
         const uint32_t gi = (uint32_t)cx * (uint32_t)R + (uint32_t)cy;
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
-            const float v = (float)s_acc[ch][i];
+            const float v = (float)s_acc[ch][box.lo + i];
             if (v != 0.0f) gatomic_add(&g[ch][gi], v);
         }
     }
@@ -1439,7 +1555,8 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     __shared__ float2 s_vel[G2P ? kLdsCells : 1];
     __shared__ uint16_t s_perm[kChunk];
     __shared__ uint32_t s_scan[17];
-    __shared__ int s_box[4 * kWaves];
+    __shared__ int s_box[4 * kPer * kWaves];
+    __shared__ Box s_mbox[kPer];
     __shared__ float s_red[2 * kWaves];
 
     // 0. this launch's substep, as every workgroup derives it
@@ -1490,14 +1607,25 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     //    particles) runs slice by slice instead: each kBlock-particle slice
     //    with a box of its own, so that one such chunk does not fall back to
     //    global-memory G2P and P2G and set the launch's duration.
+    //    When the chunk's box does not fit but its slices' boxes fit side by
+    //    side, each round uses its slice's box, all staged, sorted and
+    //    flushed once (a chunk at a body's edge, or one whose run of keys
+    //    spans two patches, without the four passes of slicing).
     const int margin = G2P ? 2 : 1;
     Box box = chunk_box(p, a, base, project_in, margin, s_box);
-    const bool sliced = !box.lds;
+    int nbins = 0;
+    if (!box.lds) nbins = slice_boxes(p, a, base, project_in, margin, s_box, s_mbox);
+    const bool multi = !box.lds && nbins <= kLdsCells;
+    const bool sliced = !box.lds && !multi;
     const uint32_t nvalid = base < p.n ? min((uint32_t)kChunk, p.n - base) : 0u;
-    chunk_order(p, a, base, project_in, box, nvalid, reinterpret_cast<uint32_t*>(s_acc[0]), s_perm, s_scan);
+    chunk_order(p, a, base, project_in, box, multi ? s_mbox : nullptr, nbins, nvalid,
+                reinterpret_cast<uint32_t*>(s_acc[0]), s_perm, s_scan);
 
     // 2. clear the LDS accumulators; stage the updated grid velocity
-    if (!sliced) stage_box<G2P, NCH>(p, box, cur, dt, ddt, s_acc, s_vel);
+    if (multi)
+        stage_boxes<G2P, NCH>(p, s_mbox, cur, dt, ddt, s_acc, s_vel);
+    else if (!sliced)
+        stage_box<G2P, NCH>(p, box, cur, dt, ddt, s_acc, s_vel);
 
     const GridDst gd{nxt, P2G ? 0u : 1u};
 
@@ -1508,6 +1636,7 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
     for (int k = 0; k < kPer; ++k) {
         const uint32_t q = k * kBlock + threadIdx.x;
         const bool valid = q < nvalid;
+        if (multi) box = s_mbox[k];
         if (sliced) {
             box = slice_box(p, a, base + q, valid, project_in, margin, s_box);
             slice_order(p, a, base, project_in, box, k, nvalid, reinterpret_cast<uint32_t*>(s_acc[0]), s_perm, s_scan);
@@ -1654,8 +1783,14 @@ __device__ __forceinline__ void substep_body(const Pipeline& p, uint64_t seq, ui
         }
     }
 
-    // 6. flush the LDS box into the next accumulator set
-    if (!sliced) flush_box<NCH>(box, s_acc, p, gd, R);
+    // 6. flush the LDS box (or boxes) into the next accumulator set
+    if (multi) {
+        __syncthreads();
+#pragma unroll 1
+        for (int k = 0; k < kPer; ++k) flush_box<NCH>(s_mbox[k], s_acc, p, gd, R, false);
+    } else if (!sliced) {
+        flush_box<NCH>(box, s_acc, p, gd, R);
+    }
     // 7. clear this workgroup's slice of the set the substep after next uses
     if (G2P) {
         const uint32_t R2 = (uint32_t)R * (uint32_t)R;

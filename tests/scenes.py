@@ -173,6 +173,57 @@ def sand_column(backend: str, res=64, width=16, height=24, spacing=0.5, dt=0.02,
     return sim
 
 
+def _sand_with_clumps(backend: str, n_clumps: int, seed: int):
+    """A 12 x 16-cell sand column at 4 particles per cell (768 particles) on a
+    64^2 grid, plus n_clumps 2 x 2-cell clumps of sand (16 particles each) on
+    the floor at seeded places away from it.  Fewer than 1,024 particles in
+    all, so the GPU runs them as one chunk whose bounding box exceeds its LDS
+    budget."""
+    cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.drucker_prager(3.5e5, 0.3, friction_deg=30.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(12, 16), box_center=(12.0, 10.0), position_jitter=0.3,
+                                rng_seed=seed))
+    xs = np.sort(np.random.default_rng(seed).choice(np.arange(24, 60, 3), size=n_clumps, replace=False))
+    for k, x in enumerate(xs):
+        sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(2, 2), box_center=(float(x), 3.0), position_jitter=0.3,
+                                    rng_seed=seed + 1 + k))
+    return sim
+
+
+def sand_strays(backend: str):
+    """A column with 3 small clumps away from it (48 particles): the chunk
+    runs slice by slice, each slice in a box of its own or, where a slice's
+    box exceeds the budget, through global memory."""
+    return _sand_with_clumps(backend, 3, 11)
+
+
+def sand_scattered(backend: str):
+    """A column with 10 small clumps away from it (160 particles): slice by
+    slice as well, with more of the slices mixing column and clumps."""
+    return _sand_with_clumps(backend, 10, 12)
+
+
+def scattered_clumps(backend: str):
+    """300 small NeoHookean clumps (2 x 2 cells at 4 per cell, 4,800 particles)
+    at seeded places all over a 128^2 grid, falling: every 1,024-particle chunk
+    spans clumps far apart, so every chunk runs slice by slice, and its slices
+    are shared out among the launch's workgroups."""
+    cfg = eh.SimConfig.standard(128, 0.02, (0.0, -9.81))
+    m = Mats(backend)
+    sim = (new_sim(backend, cfg)
+           .with_default_material(m.neohookean(400.0, 200.0))
+           .with_boundary(eh.SlipBoundary.new(2)))
+    rng = np.random.default_rng(21)
+    for k in range(300):
+        x, y = (float(v) for v in rng.uniform(8.0, 120.0, size=2))
+        sim.add_body(eh.SpawnRegion(spacing=0.5, box_size=(2, 2), box_center=(x, y), position_jitter=0.2,
+                                    rng_seed=500 + k))
+    return sim
+
+
 def snow_drop(backend: str):
     cfg = eh.SimConfig.standard(64, 0.02, (0.0, -9.81))
     m = Mats(backend)
@@ -392,6 +443,9 @@ SCENES: Dict[str, Scene] = {
     "quickstart": Scene("quickstart", 60, quickstart),
     "quickstart_impact": Scene("quickstart_impact", 30, quickstart_impact),
     "sand_column": Scene("sand_column", 10, sand_column),
+    "sand_strays": Scene("sand_strays", 10, sand_strays),
+    "sand_scattered": Scene("sand_scattered", 10, sand_scattered),
+    "scattered_clumps": Scene("scattered_clumps", 5, scattered_clumps),
     "snow_drop": Scene("snow_drop", 10, snow_drop),
     "corotated_drop": Scene("corotated_drop", 20, corotated_drop),
     "nacc_block": Scene("nacc_block", 10, nacc_block),
