@@ -43,8 +43,8 @@ ALGO_BYTES = {"k_substep": SUBSTEP_BYTES}
 # HBM bytes per k_substep launch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (one run each, median over the launches) and
 # scaled by the same counters' reading for a 256 MiB device-to-device copy
-# (profiles/k_substep_sand_1m_512.md): 28680 KiB x 2.000 + 77955 KiB x 1.000.
-MEASURED_TRAFFIC = {"k_substep": (28679.9 * 2.000 + 77955.1 * 1.000) * 1024.0}
+# (profiles/k_substep_sand_1m_512.md): 29039 KiB x 2.000 + 80910 KiB x 1.000.
+MEASURED_TRAFFIC = {"k_substep": (29039.0 * 2.000 + 80910.0 * 1.000) * 1024.0}
 
 
 def build_workload(backend: str = "gpu", geometry: str = "column"):
