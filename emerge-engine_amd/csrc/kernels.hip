@@ -1830,7 +1830,8 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     constexpr int kDensCh = kAccPy + 1;
     __shared__ double s_acc[kDensCh][kLdsCells];
     __shared__ float s_dm[kLdsCells];
-    __shared__ int s_box[4 * kWaves];
+    __shared__ int s_box[4 * kPer * kWaves];
+    __shared__ Box s_mbox[kPer];
     Ctrl* c = p.ctrl;
     const DevConfig& cfg = p.cfg;
     // same launch number as the substep kernel that follows: same decision
@@ -1848,13 +1849,24 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     const float* dm = p.grid.dm[cur];
     const GridDst gd{cur, 0u};
 
-    const Box box = chunk_box(p, a, base, project_in, 1, s_box);
-    if (box.lds) {
-        for (int i = threadIdx.x; i < box.w * box.h; i += kBlock) {
+    // a chunk whose box exceeds the budget takes its slices' boxes side by
+    // side when they fit together (slice_boxes), the particles of round k in
+    // slice k's box
+    Box box = chunk_box(p, a, base, project_in, 1, s_box);
+    bool multi = false;
+    if (!box.lds) multi = slice_boxes(p, a, base, project_in, 1, s_box, s_mbox) <= kLdsCells;
+    if (box.lds || multi) {
+#pragma unroll 1
+        for (int b = 0; b < (multi ? kPer : 1); ++b) {
+            const Box x = multi ? s_mbox[b] : box;
+            if (!x.lds) continue;
+            for (int i = threadIdx.x; i < x.w * x.h; i += kBlock) {
 #pragma unroll
-            for (int ch = 0; ch < kDensCh; ++ch) s_acc[ch][i] = 0.0;
-            const int cx = box.ox + i / box.h, cy = box.oy + i % box.h;
-            s_dm[i] = (cx >= 0 && cy >= 0 && cx < R && cy < R) ? dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] : 0.0f;
+                for (int ch = 0; ch < kDensCh; ++ch) s_acc[ch][x.lo + i] = 0.0;
+                const int cx = x.ox + i / x.h, cy = x.oy + i % x.h;
+                s_dm[x.lo + i] =
+                    (cx >= 0 && cy >= 0 && cx < R && cy < R) ? dm[(uint32_t)cx * (uint32_t)R + (uint32_t)cy] : 0.0f;
+            }
         }
         __syncthreads();
     }
@@ -1863,6 +1875,7 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
     for (int k = 0; k < kPer; ++k) {
         const uint32_t j = base + k * kBlock + threadIdx.x;
         const bool valid = j < p.n;
+        if (multi) box = s_mbox[k];
         PState s = neutral_state();
         bool asleep = false;
         if (valid) {
@@ -1902,7 +1915,13 @@ __device__ __forceinline__ void density_p2g_body(const Pipeline& p, uint64_t seq
         const P2GTerms t = p2g_terms<-1>(p, s, N.dt);
         scatter_p2g<kDensCh>(t, valid && !asleep, box, s_acc, p, gd, R);
     }
-    flush_box<kDensCh>(box, s_acc, p, gd, R);
+    if (multi) {
+        __syncthreads();
+#pragma unroll 1
+        for (int k = 0; k < kPer; ++k) flush_box<kDensCh>(s_mbox[k], s_acc, p, gd, R, false);
+    } else {
+        flush_box<kDensCh>(box, s_acc, p, gd, R);
+    }
     const uint32_t wp = wave_sum(n_proj);
     if (lane_id() == 0 && wp) atomicAdd(&c->proj_count[N.step % kStepRing], wp);
 }
