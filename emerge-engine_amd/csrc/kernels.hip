@@ -1154,7 +1154,7 @@ __device__ __forceinline__ uint32_t hash_bin(int cx, int cy, uint32_t nb) {
 }
 // Bins of a slice that runs through global memory inside a chunk's slice
 // boxes (the other slices' boxes permitting).
-constexpr int kGlobalSliceBins = 32;
+constexpr int kGlobalSliceBins = 12;
 
 // Boxes of a chunk's kPer slices (slice k: slots k * kBlock .. + kBlock, the
 // particles of round k in stored order), one after another in the LDS
